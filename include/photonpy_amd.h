@@ -1,0 +1,148 @@
+/* photonpy_amd — MI355X-native batched Poisson-MLE spot fitting.
+ *
+ * C ABI of libphotonpy_amd.so.  The first block keeps the names, argument
+ * order and meaning of the reference's exported functions so that the
+ * reference's ctypes bindings (photonpy/cpp/estimator.py, gaussian.py,
+ * cspline.py, context.py) bind to it unchanged; every pointer there is host
+ * memory and every call is synchronous.  Each declaration cites the reference
+ * function it replaces.  The second block adds device-pointer entry points
+ * (inputs already resident in HBM, asynchronous on a caller-supplied HIP
+ * stream) that have no counterpart in the reference.
+ *
+ * Errors: reference-style calls report failures through
+ * SMLM_GetLastError() (creators also return NULL); device-pointer calls
+ * return 0 on success and a nonzero code otherwise.  numspots == 0 is a
+ * no-op everywhere, as in the reference.
+ */
+#ifndef PHOTONPY_AMD_H
+#define PHOTONPY_AMD_H
+
+#include <stdbool.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct Context Context;
+typedef struct Estimator Estimator;
+
+/* Estimators/Estimator.h:18-26 */
+typedef struct LMParams {
+    int iterations;
+    float lambda;
+    float factor; /* accepted and stored; unused by the fit, as in the reference */
+} LMParams;
+
+/* Estimators/Estimator.h:28-31 */
+typedef struct ParamLimit {
+    float min, max;
+} ParamLimit;
+
+/* Estimators/Estimator.h:151-154 */
+typedef struct EstimatorProperties {
+    int numParams, sampleCount, numDiag, numConst, sampleIndexDims;
+} EstimatorProperties;
+
+/* Estimators/Gaussian/GaussianPSF.h:27-33: x/y = {s0, gamma, d, A} */
+typedef struct Gauss3D_Calibration {
+    float x[4];
+    float y[4];
+    float minz, maxz;
+} Gauss3D_Calibration;
+
+/* Gauss3D_Calibration followed by the quartic coefficients B (per axis) of
+ * sigma(z) = s0 sqrt(1 + t^2/d^2 + A t^3/d^2 + B t^4/d^2), t = z - gamma.
+ * No reference counterpart (the reference model has B = 0). */
+typedef struct Gauss3D_CalibrationQ {
+    float x[4];
+    float y[4];
+    float minz, maxz;
+    float Bx, By;
+} Gauss3D_CalibrationQ;
+
+/* Estimators/CSpline/CubicSplinePSF.h:12-23; shape = {Nz, Ny, Nx} */
+typedef struct CSpline_Calibration {
+    int shape[3];
+    float z_min;
+    float z_max;
+    const float* coefs; /* [Nz, Ny, Nx, 64] */
+} CSpline_Calibration;
+
+/* ---------------------------------------------------------------------- */
+/* Reference-compatible surface (host pointers, synchronous)                */
+
+/* Context.cpp:65-73 */
+Context* Context_Create(void);
+void Context_Destroy(Context* ctx);
+
+/* Estimators/Gaussian/GaussianPSF.cu:87-108 — params x,y,I,bg */
+Estimator* Gauss2D_CreatePSF_XYIBg(int roisize, float sigmaX, float sigmaY, bool cuda, Context* ctx);
+/* Estimators/Gaussian/GaussianPSF.cu:136-158 — params x,y,I,bg,sigma */
+Estimator* Gauss2D_CreatePSF_XYIBgSigma(int roisize, float initialSigma, bool cuda, Context* ctx);
+/* Estimators/Gaussian/GaussianPSF.cu:36-59 — params x,y,z,I,bg */
+Estimator* Gauss2D_CreatePSF_XYZIBg(int roisize, const Gauss3D_Calibration* calib, bool cuda, Context* ctx);
+/* Estimators/CSpline/CubicSplinePSF.cu:55-107 — params x,y,z,I,bg; mode 0 (flat background) only */
+Estimator* CSpline_CreatePSF(int roisize, const CSpline_Calibration* calib, const float* coefs, int mode, bool cuda,
+                             Context* ctx);
+/* Estimators/CSpline/CubicSplinePSF.cu:151-204 — shape {Nz,Ny,Nx};
+ * splineCoefs [(Nz-3+2b),(Ny-3+2b),(Nx-3+2b),64] */
+void CSpline_Compute(const int* shape, const float* data, float* splineCoefs, int border);
+
+/* Estimators/Estimator.cpp:45-100 */
+void Estimator_Delete(Estimator* e);
+const char* Estimator_ParamFormat(Estimator* e);
+void Estimator_GetProperties(Estimator* e, EstimatorProperties* props);
+void Estimator_SampleDims(Estimator* e, int* dims);
+void Estimator_GetParamLimits(Estimator* e, ParamLimit* limits);
+void Estimator_SetLMParams(Estimator* e, const LMParams* p);
+void Estimator_GetLMParams(Estimator* e, LMParams* p);
+
+/* Estimators/Estimator.cpp:102-113 — ev[numspots, H, W] */
+void Estimator_ComputeExpectedValue(Estimator* e, int numspots, const float* params, const float* constants,
+                                    const int* spotpos, float* ev);
+/* Estimators/Estimator.cpp:115-128 — initial may be NULL (centre-of-mass start),
+ * trace [numspots, traceBufLen, K] may be NULL; diag has 0 floats per spot */
+void Estimator_Estimate(Estimator* e, int numspots, const float* sample, const float* constants, const int* spotpos,
+                        const float* initial, float* params, float* diagnostics, int* iterations, float* trace,
+                        int traceBufLen);
+/* Estimators/Estimator.cpp:131-144 — deriv[numspots, K, H, W], ev[numspots, H, W] */
+void Estimator_ComputeDerivatives(Estimator* e, int numspots, const float* params, const float* constants,
+                                  const int* spotpos, float* psf_deriv, float* ev);
+/* Estimators/Estimator.cpp:146-157 — sample may be NULL (chisq then 0) */
+void Estimator_ChiSquareAndCRLB(Estimator* e, const float* params, const float* sample, const float* constants,
+                                const int* spotpos, int numspots, float* crlb, float* chisq);
+
+/* ---------------------------------------------------------------------- */
+/* MI355X additions (no reference counterpart)                              */
+
+/* Last error message of the calling thread ("" if none); cleared by SMLM_ClearError. */
+const char* SMLM_GetLastError(void);
+void SMLM_ClearError(void);
+/* Number of visible HIP devices (0 without a GPU). */
+int SMLM_DeviceCount(void);
+
+/* Astigmatic model with the quartic sigma(z) term. */
+Estimator* Gauss2D_CreatePSF_XYZIBgQ(int roisize, const Gauss3D_CalibrationQ* calib, bool cuda, Context* ctx);
+
+/* Lane-group size (power of two) used by the fit kernel of this estimator;
+ * 0 restores the built-in choice.  Returns nonzero when no kernel is compiled
+ * for (roisize, group). */
+int Estimator_SetGroupSize(Estimator* e, int group);
+int Estimator_GetGroupSize(Estimator* e);
+
+/* Fused fit on device memory: Levenberg-Marquardt from the centre-of-mass start
+ * (or d_initial), then CRLB and chi^2 when d_crlb / d_chisq are non-NULL.
+ * d_sample [n,H,W], d_params [n,K], d_iterations [n], d_trace [n,traceLen,K]. */
+int Estimator_EstimateDevice(Estimator* e, int n, const float* d_sample, const float* d_initial, float* d_params,
+                             int* d_iterations, float* d_trace, int traceLen, float* d_crlb, float* d_chisq,
+                             void* stream);
+int Estimator_ExpectedValueDevice(Estimator* e, int n, const float* d_params, float* d_ev, void* stream);
+int Estimator_DerivativesDevice(Estimator* e, int n, const float* d_params, float* d_deriv, float* d_ev, void* stream);
+int Estimator_ChiSquareAndCRLBDevice(Estimator* e, int n, const float* d_params, const float* d_sample, float* d_crlb,
+                                     float* d_chisq, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHOTONPY_AMD_H */

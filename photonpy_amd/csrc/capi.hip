@@ -1,0 +1,538 @@
+// C ABI of libphotonpy_amd.so (declarations and reference citations in
+// include/photonpy_amd.h).  Host-pointer calls stage through a per-estimator
+// device workspace and run synchronously; device-pointer calls launch on the
+// caller's stream.  Everything computes on the GPU: there is no CPU path.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/photonpy_amd.h"
+#include "dispatch.h"
+
+namespace ppa {
+hipError_t launch_spline_coefs(const int* shape, const float* d_data, float* d_out, int border, hipStream_t st);
+}
+
+using namespace ppa;
+
+// ---------------------------------------------------------------- errors
+static thread_local std::string g_lastError;
+
+static void set_error(const std::string& msg) { g_lastError = msg; }
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+static void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------- objects
+struct Context {
+    std::mutex mtx;
+    std::unordered_set<Estimator*> objects;
+};
+
+struct Estimator {
+    Context* ctx = nullptr;
+    std::mutex mtx;
+    int roisize = 0;
+    ModelDesc desc{};
+    std::string fmt;
+    std::vector<ParamLimit> limits;
+    LMParams lm{30, 1e-15f, 0.0f};  // Estimator.cpp:24
+    int group = 0;                  // 0: default_group(roisize)
+    // spline coefficients: host copy (owned) and lazily uploaded device copy
+    std::vector<float> hostCoefs;
+    float* devCoefs = nullptr;
+    // grow-only device workspace for the host-pointer API
+    void* ws = nullptr;
+    size_t wsBytes = 0;
+
+    int K() const { return num_params(desc.kind); }
+    int S() const { return roisize * roisize; }
+    int G() const { return group > 0 ? group : default_group(roisize); }
+
+    ~Estimator() {
+        if (devCoefs) (void)hipFree(devCoefs);
+        if (ws) (void)hipFree(ws);
+    }
+
+    void ensure_device() {
+        if (desc.kind == kKindSpline && !devCoefs) {
+            const size_t bytes = hostCoefs.size() * sizeof(float);
+            hip_check(hipMalloc(&devCoefs, bytes), "hipMalloc(spline coefficients)");
+            hip_check(hipMemcpy(devCoefs, hostCoefs.data(), bytes, hipMemcpyHostToDevice), "upload spline coefficients");
+            desc.sm.coefs = devCoefs;
+        }
+    }
+
+    void* workspace(size_t bytes) {
+        if (bytes > wsBytes) {
+            if (ws) hip_check(hipFree(ws), "hipFree(workspace)");
+            ws = nullptr;
+            wsBytes = 0;
+            hip_check(hipMalloc(&ws, bytes), "hipMalloc(workspace)");
+            wsBytes = bytes;
+        }
+        return ws;
+    }
+};
+
+static void attach(Estimator* e, Context* ctx) {
+    if (!ctx) return;
+    std::lock_guard<std::mutex> l(ctx->mtx);
+    ctx->objects.insert(e);
+    e->ctx = ctx;
+}
+
+static void detach(Estimator* e) {
+    if (!e->ctx) return;
+    std::lock_guard<std::mutex> l(e->ctx->mtx);
+    e->ctx->objects.erase(e);
+    e->ctx = nullptr;
+}
+
+template <class F>
+static auto guarded(F fn, decltype(fn()) fallback) -> decltype(fn()) {
+    try {
+        return fn();
+    } catch (const std::exception& ex) {
+        set_error(ex.what());
+        return fallback;
+    }
+}
+
+static void require_gpu(bool cuda) {
+    if (!cuda)
+        throw std::runtime_error("photonpy_amd: CPU estimators are not provided; create the estimator with cuda=True");
+}
+
+static Estimator* make_gauss(int kind, int roisize, const GaussCalib& gc) {
+    if (roisize < 1) throw std::runtime_error("photonpy_amd: roisize must be positive");
+    auto* e = new Estimator();
+    e->roisize = roisize;
+    e->desc.kind = kind;
+    e->desc.gc = gc;
+    const float b = 1.5f, hi = (float)roisize - 1 - b;
+    if (kind == kKindFixed) {
+        e->fmt = "x,y,I,bg";
+        e->limits = {{b, hi}, {b, hi}, {10.0f, 1e9f}, {1e-6f, 1e9f}};  // GaussianPSF.cu:92-97
+    } else if (kind == kKindSigma) {
+        e->fmt = "x,y,\tI,bg,sigma";  // literal tab as in GaussianPSFModels.h:316
+        e->limits = {{b, hi}, {b, hi}, {10.0f, 1e9f}, {1e-6f, 1e9f}, {1.0f, roisize * 2.0f}};  // GaussianPSF.cu:141-147
+    } else {
+        e->fmt = "x,y,z,I,bg";
+        e->limits = {{b, hi}, {b, hi}, {gc.minz, gc.maxz}, {10.0f, 1e9f}, {1e-6f, 1e9f}};  // GaussianPSF.cu:42-48
+    }
+    return e;
+}
+
+// ---------------------------------------------------------------- launch helpers
+static hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+static void run_fit(Estimator* e, int n, const float* d_sample, const float* d_initial, float* d_params, int* d_iter,
+                    float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st) {
+    e->ensure_device();
+    const int R = e->roisize;
+    if (R < kMinFitR || R > kMaxFitR)
+        throw std::runtime_error("photonpy_amd: fit kernels are compiled for roisize " + std::to_string(kMinFitR) +
+                                 ".." + std::to_string(kMaxFitR) + ", got " + std::to_string(R));
+    FitArgs a{};
+    a.n = n;
+    a.sample = d_sample;
+    a.initial = d_initial;
+    a.params = d_params;
+    a.iterations = d_iter;
+    a.trace = d_trace;
+    a.traceLen = d_trace ? traceLen : 0;
+    a.crlb = d_crlb;
+    a.chisq = d_chisq;
+    a.maxIter = e->lm.iterations;
+    a.lambda = e->lm.lambda;
+    hipError_t err = launch_fit_any(e->desc, R, e->G(), a, 0, st);
+    if (err == hipErrorInvalidValue)
+        throw std::runtime_error("photonpy_amd: no fit kernel compiled for roisize " + std::to_string(R) + ", group " +
+                                 std::to_string(e->G()));
+    hip_check(err, "fit kernel launch");
+}
+
+// workspace layout helper: consecutive 256-byte aligned slices
+struct Slicer {
+    char* base;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t count) {
+        if (count == 0) return nullptr;
+        T* p = reinterpret_cast<T*>(base + off);
+        off += (count * sizeof(T) + 255) & ~size_t(255);
+        return p;
+    }
+    static size_t bytes(std::initializer_list<size_t> sizes) {
+        size_t t = 0;
+        for (size_t s : sizes) t += (s + 255) & ~size_t(255);
+        return t;
+    }
+};
+
+extern "C" {
+
+const char* SMLM_GetLastError(void) { return g_lastError.c_str(); }
+void SMLM_ClearError(void) { g_lastError.clear(); }
+
+int SMLM_DeviceCount(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+Context* Context_Create(void) { return new Context(); }
+
+void Context_Destroy(Context* ctx) {
+    if (!ctx) return;
+    std::vector<Estimator*> objs;
+    {
+        std::lock_guard<std::mutex> l(ctx->mtx);
+        objs.assign(ctx->objects.begin(), ctx->objects.end());
+        ctx->objects.clear();
+    }
+    for (Estimator* e : objs) {
+        e->ctx = nullptr;
+        delete e;
+    }
+    delete ctx;
+}
+
+Estimator* Gauss2D_CreatePSF_XYIBg(int roisize, float sigmaX, float sigmaY, bool cuda, Context* ctx) {
+    return guarded(
+        [&]() -> Estimator* {
+            require_gpu(cuda);
+            GaussCalib gc{};
+            gc.sx = sigmaX;
+            gc.sy = sigmaY;
+            Estimator* e = make_gauss(kKindFixed, roisize, gc);
+            attach(e, ctx);
+            return e;
+        },
+        nullptr);
+}
+
+Estimator* Gauss2D_CreatePSF_XYIBgSigma(int roisize, float initialSigma, bool cuda, Context* ctx) {
+    return guarded(
+        [&]() -> Estimator* {
+            require_gpu(cuda);
+            GaussCalib gc{};
+            gc.initialSigma = initialSigma;
+            Estimator* e = make_gauss(kKindSigma, roisize, gc);
+            attach(e, ctx);
+            return e;
+        },
+        nullptr);
+}
+
+static Estimator* create_astig(int roisize, const float* x, const float* y, float minz, float maxz, float Bx, float By,
+                               bool cuda, Context* ctx) {
+    require_gpu(cuda);
+    GaussCalib gc{};
+    for (int i = 0; i < 4; i++) {
+        gc.ax[i] = x[i];
+        gc.ay[i] = y[i];
+    }
+    gc.ax[4] = Bx;
+    gc.ay[4] = By;
+    gc.minz = minz;
+    gc.maxz = maxz;
+    Estimator* e = make_gauss(kKindAstig, roisize, gc);
+    attach(e, ctx);
+    return e;
+}
+
+Estimator* Gauss2D_CreatePSF_XYZIBg(int roisize, const Gauss3D_Calibration* c, bool cuda, Context* ctx) {
+    return guarded(
+        [&]() -> Estimator* {
+            if (!c) throw std::runtime_error("photonpy_amd: calibration is NULL");
+            return create_astig(roisize, c->x, c->y, c->minz, c->maxz, 0.0f, 0.0f, cuda, ctx);
+        },
+        nullptr);
+}
+
+Estimator* Gauss2D_CreatePSF_XYZIBgQ(int roisize, const Gauss3D_CalibrationQ* c, bool cuda, Context* ctx) {
+    return guarded(
+        [&]() -> Estimator* {
+            if (!c) throw std::runtime_error("photonpy_amd: calibration is NULL");
+            return create_astig(roisize, c->x, c->y, c->minz, c->maxz, c->Bx, c->By, cuda, ctx);
+        },
+        nullptr);
+}
+
+Estimator* CSpline_CreatePSF(int roisize, const CSpline_Calibration* calib, const float* coefs, int mode, bool cuda,
+                             Context* ctx) {
+    return guarded(
+        [&]() -> Estimator* {
+            require_gpu(cuda);
+            if (!calib) throw std::runtime_error("photonpy_amd: calibration is NULL");
+            if (mode != 0)
+                throw std::runtime_error(mode == 1 || mode == 2
+                                             ? "photonpy_amd: spline background modes other than flat (0) are not implemented"
+                                             : "Invalid model background mode");
+            if (roisize < 1) throw std::runtime_error("photonpy_amd: roisize must be positive");
+            const float* src = coefs ? coefs : calib->coefs;
+            if (!src) throw std::runtime_error("photonpy_amd: spline coefficients are NULL");
+            const int Nz = calib->shape[0], Ny = calib->shape[1], Nx = calib->shape[2];
+            if (Nz < 1 || Ny < 1 || Nx < 1) throw std::runtime_error("photonpy_amd: invalid spline shape");
+            auto* e = new Estimator();
+            e->roisize = roisize;
+            e->desc.kind = kKindSpline;
+            e->desc.sm.Nz = Nz;
+            e->desc.sm.Ny = Ny;
+            e->desc.sm.Nx = Nx;
+            e->desc.sm.zmin = calib->z_min;
+            e->desc.sm.zmax = calib->z_max;
+            e->desc.sm.coefs = nullptr;
+            e->hostCoefs.assign(src, src + (size_t)Nz * Ny * Nx * 64);  // copied, as CubicSplinePSF.cu:48-52
+            e->fmt = "x,y,z,I,bg";
+            const float b = 1.5f, hi = (float)roisize - 1 - b;
+            e->limits = {{b, hi}, {b, hi}, {calib->z_min, calib->z_max}, {10.0f, 1e9f}, {1e-6f, 1e9f}};
+            attach(e, ctx);
+            return e;
+        },
+        nullptr);
+}
+
+void CSpline_Compute(const int* shape, const float* data, float* splineCoefs, int border) {
+    guarded(
+        [&]() -> int {
+            if (!shape || !data || !splineCoefs) throw std::runtime_error("photonpy_amd: NULL argument");
+            const size_t nin = (size_t)shape[0] * shape[1] * shape[2];
+            const int o0 = shape[0] - 3 + 2 * border, o1 = shape[1] - 3 + 2 * border, o2 = shape[2] - 3 + 2 * border;
+            if (o0 <= 0 || o1 <= 0 || o2 <= 0) throw std::runtime_error("photonpy_amd: z-stack too small for a spline");
+            const size_t nout = (size_t)o0 * o1 * o2 * 64;
+            float *din = nullptr, *dout = nullptr;
+            hip_check(hipMalloc(&din, nin * sizeof(float)), "hipMalloc");
+            hipError_t e1 = hipMalloc(&dout, nout * sizeof(float));
+            if (e1 != hipSuccess) {
+                (void)hipFree(din);
+                hip_check(e1, "hipMalloc");
+            }
+            hipError_t err = hipMemcpy(din, data, nin * sizeof(float), hipMemcpyHostToDevice);
+            if (err == hipSuccess) err = launch_spline_coefs(shape, din, dout, border, 0);
+            if (err == hipSuccess) err = hipMemcpy(splineCoefs, dout, nout * sizeof(float), hipMemcpyDeviceToHost);
+            (void)hipFree(din);
+            (void)hipFree(dout);
+            hip_check(err, "CSpline_Compute");
+            return 0;
+        },
+        0);
+}
+
+void Estimator_Delete(Estimator* e) {
+    if (!e) return;
+    detach(e);
+    delete e;
+}
+
+const char* Estimator_ParamFormat(Estimator* e) { return e ? e->fmt.c_str() : ""; }
+
+void Estimator_GetProperties(Estimator* e, EstimatorProperties* p) {
+    if (!e || !p) return;
+    p->numParams = e->K();
+    p->sampleCount = e->S();
+    p->numDiag = 0;  // the reference's GPU estimators report no diagnostics (EstimatorImpl.h:34)
+    p->numConst = 0;
+    p->sampleIndexDims = 2;
+}
+
+void Estimator_SampleDims(Estimator* e, int* dims) {
+    if (!e || !dims) return;
+    dims[0] = e->roisize;
+    dims[1] = e->roisize;
+}
+
+void Estimator_GetParamLimits(Estimator* e, ParamLimit* limits) {
+    if (!e || !limits) return;
+    for (size_t i = 0; i < e->limits.size(); i++) limits[i] = e->limits[i];
+}
+
+void Estimator_SetLMParams(Estimator* e, const LMParams* p) {
+    if (e && p) e->lm = *p;
+}
+
+void Estimator_GetLMParams(Estimator* e, LMParams* p) {
+    if (e && p) *p = e->lm;
+}
+
+int Estimator_SetGroupSize(Estimator* e, int group) {
+    if (!e) return 1;
+    if (group < 0 || group > 64 || (group & (group - 1)) != 0) return 1;
+    e->group = group;
+    return 0;
+}
+
+int Estimator_GetGroupSize(Estimator* e) { return e ? e->G() : 0; }
+
+void Estimator_ComputeExpectedValue(Estimator* e, int n, const float* params, const float* constants,
+                                    const int* spotpos, float* ev) {
+    (void)constants;
+    (void)spotpos;
+    if (!e || n <= 0) return;
+    guarded(
+        [&]() -> int {
+            std::lock_guard<std::mutex> l(e->mtx);
+            e->ensure_device();
+            const size_t K = e->K(), S = e->S();
+            Slicer sl{(char*)e->workspace(Slicer::bytes({n * K * 4, n * S * 4}))};
+            float* dp = sl.take<float>(n * K);
+            float* dev = sl.take<float>(n * S);
+            hip_check(hipMemcpy(dp, params, n * K * 4, hipMemcpyHostToDevice), "H2D params");
+            hip_check(launch_ev_any(e->desc, e->roisize, n, dp, dev, nullptr, 0), "expected value kernel");
+            hip_check(hipMemcpy(ev, dev, n * S * 4, hipMemcpyDeviceToHost), "D2H ev");
+            return 0;
+        },
+        0);
+}
+
+void Estimator_ComputeDerivatives(Estimator* e, int n, const float* params, const float* constants,
+                                  const int* spotpos, float* deriv, float* ev) {
+    (void)constants;
+    (void)spotpos;
+    if (!e || n <= 0) return;
+    guarded(
+        [&]() -> int {
+            std::lock_guard<std::mutex> l(e->mtx);
+            e->ensure_device();
+            const size_t K = e->K(), S = e->S();
+            Slicer sl{(char*)e->workspace(Slicer::bytes({n * K * 4, n * S * 4, n * K * S * 4}))};
+            float* dp = sl.take<float>(n * K);
+            float* dev = sl.take<float>(n * S);
+            float* dd = sl.take<float>(n * K * S);
+            hip_check(hipMemcpy(dp, params, n * K * 4, hipMemcpyHostToDevice), "H2D params");
+            hip_check(launch_ev_any(e->desc, e->roisize, n, dp, dev, dd, 0), "derivatives kernel");
+            if (ev) hip_check(hipMemcpy(ev, dev, n * S * 4, hipMemcpyDeviceToHost), "D2H ev");
+            if (deriv) hip_check(hipMemcpy(deriv, dd, n * K * S * 4, hipMemcpyDeviceToHost), "D2H deriv");
+            return 0;
+        },
+        0);
+}
+
+void Estimator_Estimate(Estimator* e, int n, const float* sample, const float* constants, const int* spotpos,
+                        const float* initial, float* params, float* diagnostics, int* iterations, float* trace,
+                        int traceBufLen) {
+    (void)constants;
+    (void)spotpos;
+    (void)diagnostics;
+    if (!e || n <= 0) return;
+    guarded(
+        [&]() -> int {
+            std::lock_guard<std::mutex> l(e->mtx);
+            const size_t K = e->K(), S = e->S();
+            const size_t tl = (trace && traceBufLen > 0) ? (size_t)traceBufLen : 0;
+            Slicer sl{(char*)e->workspace(
+                Slicer::bytes({n * S * 4, initial ? n * K * 4 : 0, n * K * 4, (size_t)n * 4, n * tl * K * 4}))};
+            float* ds = sl.take<float>(n * S);
+            float* di = initial ? sl.take<float>(n * K) : nullptr;
+            float* dp = sl.take<float>(n * K);
+            int* dit = sl.take<int>(n);
+            float* dt = tl ? sl.take<float>(n * tl * K) : nullptr;
+            hip_check(hipMemcpy(ds, sample, n * S * 4, hipMemcpyHostToDevice), "H2D samples");
+            if (di) hip_check(hipMemcpy(di, initial, n * K * 4, hipMemcpyHostToDevice), "H2D initial");
+            if (dt) hip_check(hipMemset(dt, 0, n * tl * K * 4), "clear trace");
+            run_fit(e, n, ds, di, dp, dit, dt, (int)tl, nullptr, nullptr, 0);
+            hip_check(hipMemcpy(params, dp, n * K * 4, hipMemcpyDeviceToHost), "D2H params");
+            if (iterations) hip_check(hipMemcpy(iterations, dit, (size_t)n * 4, hipMemcpyDeviceToHost), "D2H iterations");
+            if (dt) hip_check(hipMemcpy(trace, dt, n * tl * K * 4, hipMemcpyDeviceToHost), "D2H trace");
+            return 0;
+        },
+        0);
+}
+
+void Estimator_ChiSquareAndCRLB(Estimator* e, const float* params, const float* sample, const float* constants,
+                                const int* spotpos, int n, float* crlb, float* chisq) {
+    (void)constants;
+    (void)spotpos;
+    if (!e || n <= 0) return;
+    guarded(
+        [&]() -> int {
+            std::lock_guard<std::mutex> l(e->mtx);
+            e->ensure_device();
+            const size_t K = e->K(), S = e->S();
+            Slicer sl{(char*)e->workspace(Slicer::bytes({n * K * 4, sample ? n * S * 4 : 0, n * K * 4, (size_t)n * 4}))};
+            float* dp = sl.take<float>(n * K);
+            float* ds = sample ? sl.take<float>(n * S) : nullptr;
+            float* dc = sl.take<float>(n * K);
+            float* dx = sl.take<float>(n);
+            hip_check(hipMemcpy(dp, params, n * K * 4, hipMemcpyHostToDevice), "H2D params");
+            if (ds) hip_check(hipMemcpy(ds, sample, n * S * 4, hipMemcpyHostToDevice), "H2D samples");
+            hip_check(launch_crlb_any(e->desc, e->roisize, n, dp, ds, dc, dx, 0), "crlb kernel");
+            if (crlb) hip_check(hipMemcpy(crlb, dc, n * K * 4, hipMemcpyDeviceToHost), "D2H crlb");
+            if (chisq) hip_check(hipMemcpy(chisq, dx, (size_t)n * 4, hipMemcpyDeviceToHost), "D2H chisq");
+            return 0;
+        },
+        0);
+}
+
+// ---------------------------------------------------------------- device-pointer API
+int Estimator_EstimateDevice(Estimator* e, int n, const float* d_sample, const float* d_initial, float* d_params,
+                             int* d_iterations, float* d_trace, int traceLen, float* d_crlb, float* d_chisq,
+                             void* stream) {
+    if (!e) return 1;
+    if (n <= 0) return 0;
+    return guarded(
+        [&]() -> int {
+            std::lock_guard<std::mutex> l(e->mtx);
+            run_fit(e, n, d_sample, d_initial, d_params, d_iterations, d_trace, traceLen, d_crlb, d_chisq,
+                    as_stream(stream));
+            return 0;
+        },
+        2);
+}
+
+int Estimator_ExpectedValueDevice(Estimator* e, int n, const float* d_params, float* d_ev, void* stream) {
+    if (!e) return 1;
+    if (n <= 0) return 0;
+    return guarded(
+        [&]() -> int {
+            std::lock_guard<std::mutex> l(e->mtx);
+            e->ensure_device();
+            hip_check(launch_ev_any(e->desc, e->roisize, n, d_params, d_ev, nullptr, as_stream(stream)), "ev kernel");
+            return 0;
+        },
+        2);
+}
+
+int Estimator_DerivativesDevice(Estimator* e, int n, const float* d_params, float* d_deriv, float* d_ev, void* stream) {
+    if (!e) return 1;
+    if (n <= 0) return 0;
+    return guarded(
+        [&]() -> int {
+            std::lock_guard<std::mutex> l(e->mtx);
+            e->ensure_device();
+            hip_check(launch_ev_any(e->desc, e->roisize, n, d_params, d_ev, d_deriv, as_stream(stream)),
+                      "derivatives kernel");
+            return 0;
+        },
+        2);
+}
+
+int Estimator_ChiSquareAndCRLBDevice(Estimator* e, int n, const float* d_params, const float* d_sample, float* d_crlb,
+                                     float* d_chisq, void* stream) {
+    if (!e) return 1;
+    if (n <= 0) return 0;
+    return guarded(
+        [&]() -> int {
+            std::lock_guard<std::mutex> l(e->mtx);
+            e->ensure_device();
+            hip_check(launch_crlb_any(e->desc, e->roisize, n, d_params, d_sample, d_crlb, d_chisq, as_stream(stream)),
+                      "crlb kernel");
+            return 0;
+        },
+        2);
+}
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+// Launchers of the any-size kernels: expected value / derivatives, stand-alone
+// CRLB + chi^2, and the spline coefficient solve.
+#include "dispatch.h"
+
+namespace ppa {
+
+template <class M>
+static hipError_t ev_t(const M& m, int R, int n, const float* params, float* ev, float* deriv, hipStream_t st) {
+    const size_t total = (size_t)n * R * R;
+    const unsigned blocks = (unsigned)((total + 255) / 256);
+    hipLaunchKernelGGL(ev_kernel<M>, dim3(blocks), dim3(256), 0, st, m, n, R, params, ev, deriv);
+    return hipGetLastError();
+}
+
+template <class M>
+static hipError_t crlb_t(const M& m, int R, int n, const float* params, const float* sample, float* crlb,
+                         float* chisq, hipStream_t st) {
+    const unsigned blocks = (unsigned)(((size_t)n * kWave + 255) / 256);
+    hipLaunchKernelGGL(crlb_kernel<M>, dim3(blocks), dim3(256), 0, st, m, n, R, params, sample, crlb, chisq);
+    return hipGetLastError();
+}
+
+hipError_t launch_ev_any(const ModelDesc& d, int R, int n, const float* params, float* ev, float* deriv, hipStream_t st) {
+    switch (d.kind) {
+    case kKindFixed: return ev_t(GaussModel<kFixed>{d.gc}, R, n, params, ev, deriv, st);
+    case kKindSigma: return ev_t(GaussModel<kSigma>{d.gc}, R, n, params, ev, deriv, st);
+    case kKindAstig: return ev_t(GaussModel<kAstig>{d.gc}, R, n, params, ev, deriv, st);
+    case kKindSpline: return ev_t(d.sm, R, n, params, ev, deriv, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_crlb_any(const ModelDesc& d, int R, int n, const float* params, const float* sample, float* crlb,
+                           float* chisq, hipStream_t st) {
+    switch (d.kind) {
+    case kKindFixed: return crlb_t(GaussModel<kFixed>{d.gc}, R, n, params, sample, crlb, chisq, st);
+    case kKindSigma: return crlb_t(GaussModel<kSigma>{d.gc}, R, n, params, sample, crlb, chisq, st);
+    case kKindAstig: return crlb_t(GaussModel<kAstig>{d.gc}, R, n, params, sample, crlb, chisq, st);
+    case kKindSpline: return crlb_t(d.sm, R, n, params, sample, crlb, chisq, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Spline coefficients (CSpline_Compute, CubicSplinePSF.cu:151-204).  Cell
+// (i,j,k) interpolates the 4x4x4 data neighbourhood starting at it, at offsets
+// t = -1..2 per axis.  The reference solves the 64x64 tensor-product
+// Vandermonde system A = V (x) V (x) V with V[t][p] = (t-1)^p by LU; here the
+// exact inverse Vi of the 4x4 factor is applied along each axis in FP64.
+__constant__ double kVinv[16] = {
+    0.0,        1.0,  0.0,        0.0,         // c0 = f(0)
+    -1.0 / 3.0, -0.5, 1.0,        -1.0 / 6.0,  // c1
+    0.5,        -1.0, 0.5,        0.0,         // c2
+    -1.0 / 6.0, 0.5,  -0.5,       1.0 / 6.0};  // c3
+
+__global__ void __launch_bounds__(256) spline_coef_kernel(int s0, int s1, int s2, const float* __restrict__ data,
+                                                          float* __restrict__ out, int o0, int o1, int o2, int border) {
+    const size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= (size_t)o0 * o1 * o2) return;
+    const int k = (int)(cell % o2);
+    const int j = (int)((cell / o2) % o1);
+    const int i = (int)(cell / ((size_t)o1 * o2));
+    double b[64];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int z = clampi(i + m - border, 0, s0 - 1);
+                const int y = clampi(j + n - border, 0, s1 - 1);
+                const int x = clampi(k + q - border, 0, s2 - 1);
+                b[m * 16 + n * 4 + q] = (double)data[((size_t)z * s1 + y) * s2 + x];
+            }
+    double t1[64], t2[64];
+    // along x (last index)
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                double s = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; q++) s += kVinv[p * 4 + q] * b[m * 16 + n * 4 + q];
+                t1[m * 16 + n * 4 + p] = s;
+            }
+    // along y
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                double s = 0.0;
+#pragma unroll
+                for (int n = 0; n < 4; n++) s += kVinv[p * 4 + n] * t1[m * 16 + n * 4 + q];
+                t2[m * 16 + p * 4 + q] = s;
+            }
+    // along z
+    float* dst = out + cell * 64;
+#pragma unroll
+    for (int p = 0; p < 4; p++)
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                double s = 0.0;
+#pragma unroll
+                for (int m = 0; m < 4; m++) s += kVinv[p * 4 + m] * t2[m * 16 + n * 4 + q];
+                dst[p * 16 + n * 4 + q] = (float)s;
+            }
+}
+
+hipError_t launch_spline_coefs(const int* shape, const float* d_data, float* d_out, int border, hipStream_t st) {
+    const int o0 = shape[0] - 3 + 2 * border, o1 = shape[1] - 3 + 2 * border, o2 = shape[2] - 3 + 2 * border;
+    if (o0 <= 0 || o1 <= 0 || o2 <= 0) return hipErrorInvalidValue;
+    const size_t cells = (size_t)o0 * o1 * o2;
+    hipLaunchKernelGGL(spline_coef_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, shape[0], shape[1],
+                       shape[2], d_data, d_out, o0, o1, o2, border);
+    return hipGetLastError();
+}
+
+}  // namespace ppa

@@ -1,0 +1,80 @@
+// Expected value / derivatives and stand-alone CRLB + chi^2 kernels (any ROI size).
+//
+// ev_kernel: one lane per (spot, pixel); writes ev[n,R,R] and deriv[n,K,R,R]
+//   (layout of ComputeDerivatives, Estimation.h:169-181).
+// crlb_kernel: one wavefront per spot; Fisher matrix FI_ij = sum J_i J_j / max(mu,1e-8)
+//   and chi^2 = sum (x-mu)^2/mu (Estimation.h:63-102), FP64 inversion.
+#pragma once
+
+#include "models.h"
+
+namespace ppa {
+
+template <class M>
+__global__ void __launch_bounds__(256) ev_kernel(M model, int n, int R, const float* __restrict__ params,
+                                                 float* __restrict__ ev, float* __restrict__ deriv) {
+    constexpr int K = M::K;
+    const int S = R * R;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)n * S) return;
+    const int spot = (int)(idx / S);
+    const int p = (int)(idx - (size_t)spot * S);
+    const int y = p / R, x = p - (p / R) * R;
+    float th[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) th[k] = params[(size_t)spot * K + k];
+    float mu, J[K];
+    model.eval_pixel(th, x, y, mu, J);
+    if (ev) ev[idx] = mu;
+    if (deriv) {
+#pragma unroll
+        for (int k = 0; k < K; k++) deriv[((size_t)spot * K + k) * S + p] = J[k];
+    }
+}
+
+template <class M>
+__global__ void __launch_bounds__(256) crlb_kernel(M model, int n, int R, const float* __restrict__ params,
+                                                   const float* __restrict__ sample, float* __restrict__ crlb,
+                                                   float* __restrict__ chisq) {
+    constexpr int K = M::K, NA = K * (K + 1) / 2;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int spot = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    if (spot >= n) return;
+    const int S = R * R;
+    float th[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) th[k] = params[(size_t)spot * K + k];
+    float fi[NA + 1];
+#pragma unroll
+    for (int i = 0; i < NA + 1; i++) fi[i] = 0.0f;
+    for (int p = lane; p < S; p += kWave) {
+        const int y = p / R, x = p - (p / R) * R;
+        float mu, J[K];
+        model.eval_pixel(th, x, y, mu, J);
+        const float m = fmaxf(1e-8f, mu);
+        const float inv = rcpf(m);
+        if (sample) {
+            const float e = sample[(size_t)spot * S + p] - m;
+            fi[NA] += e * e * inv;
+        }
+        int q = 0;
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+            const float w = J[i] * inv;
+#pragma unroll
+            for (int j = i; j < K; j++) fi[q++] += w * J[j];
+        }
+    }
+    group_sum_n<kWave>(fi);
+    if (lane == 0) {
+        float c[K];
+        crlb_from_fisher<K>(fi, c);
+        if (crlb) {
+#pragma unroll
+            for (int k = 0; k < K; k++) crlb[(size_t)spot * K + k] = c[k];
+        }
+        if (chisq) chisq[spot] = fi[NA];
+    }
+}
+
+}  // namespace ppa

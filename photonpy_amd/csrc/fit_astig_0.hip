@@ -1,0 +1,30 @@
+// Fit-kernel instantiations: astig model, size list part 0 (written by gen_fit_sources.py).
+#include "dispatch.h"
+#include "fit_sizes.h"
+
+namespace ppa {
+
+hipError_t launch_fit_astig_1(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st);
+hipError_t launch_fit_astig_2(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st);
+hipError_t launch_fit_astig_3(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st);
+
+hipError_t launch_fit_astig(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st) {
+    using Model = GaussModel<kAstig>;
+    Model m{d.gc};
+    PPA_FIT_SIZES_0(PPA_FIT_CASE)
+    {
+        hipError_t e = launch_fit_astig_1(d, R, G, a, maxBlocks, st);
+        if (e != hipErrorInvalidValue) return e;
+    }
+    {
+        hipError_t e = launch_fit_astig_2(d, R, G, a, maxBlocks, st);
+        if (e != hipErrorInvalidValue) return e;
+    }
+    {
+        hipError_t e = launch_fit_astig_3(d, R, G, a, maxBlocks, st);
+        if (e != hipErrorInvalidValue) return e;
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace ppa

@@ -1,0 +1,15 @@
+// Fit-kernel instantiations: spline model, size list part 2 (written by gen_fit_sources.py).
+#include "dispatch.h"
+#include "fit_sizes.h"
+
+namespace ppa {
+
+
+hipError_t launch_fit_spline_2(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st) {
+    using Model = SplineModel;
+    Model m = d.sm;
+    PPA_FIT_SIZES_2(PPA_FIT_CASE)
+    return hipErrorInvalidValue;
+}
+
+}  // namespace ppa

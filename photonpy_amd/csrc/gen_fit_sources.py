@@ -1,0 +1,43 @@
+"""Writes the fit-kernel instantiation sources fit_<model>_<part>.hip.
+
+Run from this directory after changing the model list; the outputs are
+committed.  Each file instantiates one part of PPA_FIT_SIZES for one model so
+that `make -j` compiles them in parallel.
+"""
+MODELS = [("fixed", "GaussModel<kFixed>", "Model m{d.gc};"),
+          ("sigma", "GaussModel<kSigma>", "Model m{d.gc};"),
+          ("astig", "GaussModel<kAstig>", "Model m{d.gc};"),
+          ("spline", "SplineModel", "Model m = d.sm;")]
+PARTS = 4
+
+for name, ty, ctor in MODELS:
+    for part in range(PARTS):
+        decls = ""
+        tail = "    return hipErrorInvalidValue;\n"
+        if part == 0:
+            decls = "".join(
+                f"hipError_t launch_fit_{name}_{p}(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks,"
+                f" hipStream_t st);\n" for p in range(1, PARTS))
+            tail = "".join(
+                f"    {{\n        hipError_t e = launch_fit_{name}_{p}(d, R, G, a, maxBlocks, st);\n"
+                f"        if (e != hipErrorInvalidValue) return e;\n    }}\n" for p in range(1, PARTS)) + tail
+            fname = f"launch_fit_{name}"
+        else:
+            fname = f"launch_fit_{name}_{part}"
+        src = f"""// Fit-kernel instantiations: {name} model, size list part {part} (written by gen_fit_sources.py).
+#include "dispatch.h"
+#include "fit_sizes.h"
+
+namespace ppa {{
+
+{decls}
+hipError_t {fname}(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st) {{
+    using Model = {ty};
+    {ctor}
+    PPA_FIT_SIZES_{part}(PPA_FIT_CASE)
+{tail}}}
+
+}}  // namespace ppa
+"""
+        with open(f"fit_{name}_{part}.hip", "w") as f:
+            f.write(src)

@@ -1,0 +1,328 @@
+// PSF models of the MI355X spot fitter (device side).
+//
+// Four models, with the reference's parameter order, clamps, stop limits and
+// initial estimates:
+//   GaussModel<kFixed>  x,y,I,bg        Gauss2D_Model_XYIBg      GaussianPSFModels.h:35-152
+//   GaussModel<kSigma>  x,y,I,bg,sigma  Gauss2D_Model_XYIBgSigma GaussianPSFModels.h:308-387
+//   GaussModel<kAstig>  x,y,z,I,bg      Gauss2D_Model_XYZIBg     GaussianPSFModels.h:636-751
+//                       (sigma(z) gains a quartic term B*t^4/d^2 under the square root;
+//                        B = 0 is the reference model)
+//   SplineModel         x,y,z,I,bg      CSpline_Model_XYZIBg     CubicSplinePSFModels.h:14-154
+//
+// The Gaussians are pixel-integrated: for pixel edge e the argument is
+// a_e = (e - t - 1/2) / (sqrt(2) sigma), and each pixel's terms are differences
+// of erf(a_e), exp(-a_e^2) and a_e exp(-a_e^2) at its two edges.  A fit kernel
+// evaluates the R+1 edges of each axis once per iteration (the reference
+// re-evaluates both edges for every pixel); eval_pixel() evaluates the same
+// expressions for a single pixel so both paths give bit-identical mu and J.
+#pragma once
+
+#include "device_common.h"
+
+namespace ppa {
+
+enum GaussVariant { kFixed = 0, kSigma = 1, kAstig = 2 };
+
+constexpr float kPiF = 3.141592653589793f;
+
+struct Edge {
+    float E, G, H;  // erf(a), exp(-a^2), a*exp(-a^2)
+};
+
+PP_DEV Edge gauss_edge(float e, float t, float c2) {
+    float a = (e - t - 0.5f) * c2;
+    float g = expf(-a * a);
+    return {erff(a), g, a * g};
+}
+
+// Per-axis terms of one pixel from its left (e0) and right (e1) edges.
+struct AxisTerms {
+    float D, dE, S;  // integrated Gaussian, d/dt (without I), shape-derivative term
+};
+
+PP_DEV AxisTerms axis_terms(const Edge& e0, const Edge& e1, float cp, float cs) {
+    AxisTerms t;
+    t.D = 0.5f * e1.E - 0.5f * e0.E;
+    t.dE = cp * (e0.G - e1.G);
+    t.S = (e0.H - e1.H) * cs;
+    return t;
+}
+
+// Calibration of the Gaussian models.  ax/ay = {s0, gamma, d, A, B} per axis.
+struct GaussCalib {
+    float sx, sy;          // kFixed
+    float initialSigma;    // kSigma
+    float ax[5], ay[5];    // kAstig
+    float minz, maxz;      // kAstig
+};
+
+// Per-spot, per-iteration shape constants of one axis.
+struct AxisShape {
+    float cp;   // 1/(sqrt(2 pi) sigma)
+    float c2;   // 1/(sqrt(2) sigma)
+    float cs;   // multiplier of (H0 - H1) in the shape column
+};
+
+template <int V>
+struct GaussModel {
+    static constexpr int K = (V == kFixed) ? 4 : 5;
+    static constexpr int iX = 0, iY = 1;
+    static constexpr int iZ = 2;                               // kAstig only
+    static constexpr int iI = (V == kAstig) ? 3 : 2;
+    static constexpr int iBg = (V == kAstig) ? 4 : 3;
+    static constexpr int iS = (V == kSigma) ? 4 : (V == kAstig ? 2 : -1);  // column fed by S terms
+
+    GaussCalib c;
+
+    PP_DEV static float stop_limit(int k) {
+        if (V == kFixed) return (k == 2) ? 1e-1f : 1e-5f;
+        if (V == kSigma) return (k == 2) ? 1e-1f : (k == 3 ? 1e-4f : 1e-5f);
+        return (k == 2) ? 1e-1f : 1e-4f;
+    }
+
+    PP_DEV void check_limits(float (&t)[K], int R) const {
+        const float hi = (float)R - 1.5f - 1.0f;
+        t[0] = clampf(t[0], 1.5f, hi);
+        t[1] = clampf(t[1], 1.5f, hi);
+        if (V == kFixed) {
+            if (t[2] < 25.0f) t[2] = 25.0f;
+            if (t[3] < 0.0f) t[3] = 0.0f;
+        } else if (V == kSigma) {
+            if (t[2] < 10.0f) t[2] = 10.0f;
+            if (t[3] < 0.0f) t[3] = 0.0f;
+            if (t[4] < 1.0f) t[4] = 1.0f;
+            if (t[4] > (float)R * 0.5f) t[4] = (float)R * 0.5f;
+        } else {
+            t[2] = clampf(t[2], c.minz, c.maxz);
+            t[3] = fmaxf(t[3], 10.f);
+            t[4] = fmaxf(t[4], 0.0f);
+        }
+    }
+
+    PP_DEV void initial(float cx, float cy, float sum, float (&t)[K]) const {
+        t[0] = cx;
+        t[1] = cy;
+        if (V == kFixed) { t[2] = sum * 0.7f; t[3] = 0.0f; }
+        else if (V == kSigma) { t[2] = sum; t[3] = 0.0f; t[4] = c.initialSigma; }
+        else { t[2] = 0.0f; t[3] = sum; t[4] = 0.01f; }
+    }
+
+    // sigma_x(z) and its z-derivative for the astigmatic model.
+    PP_DEV static void astig_sigma(const float* a, float z, float& sigma, float& dsdz) {
+        const float s0 = a[0], gam = a[1], d = a[2], A = a[3], B = a[4];
+        const float d2 = d * d;
+        const float t = z - gam, t2 = t * t, t3 = t2 * t, t4 = t3 * t;
+        const float q = 1.0f + t2 / d2 + A * t3 / d2 + B * t4 / d2;
+        const float sq = sqrtf(q);
+        sigma = s0 * sq;
+        dsdz = s0 * (2 * t / d2 + A * 3 * t2 / d2 + B * 4 * t3 / d2) / (2 * sq);
+    }
+
+    PP_DEV void shapes(const float (&t)[K], AxisShape& X, AxisShape& Y) const {
+        const float r2pi = sqrtf(2 * kPiF), r2 = sqrtf(2.0f), ispi = 1.0f / sqrtf(kPiF);
+        if (V == kFixed) {
+            X.cp = 1.0f / (r2pi * c.sx); X.c2 = 1.0f / (r2 * c.sx); X.cs = 0.0f;
+            Y.cp = 1.0f / (r2pi * c.sy); Y.c2 = 1.0f / (r2 * c.sy); Y.cs = 0.0f;
+        } else if (V == kSigma) {
+            const float s = t[4];
+            X.cp = Y.cp = 1.0f / (r2pi * s);
+            X.c2 = Y.c2 = 1.0f / (r2 * s);
+            // The reference's d(DeltaX)/d(sigma) omits the 1/sigma factor
+            // (GaussianPSFModels.h:359,368); kept for parity of Estimate,
+            // Derivatives and CRLB.
+            X.cs = Y.cs = ispi;
+        } else {
+            float sgx, dsx, sgy, dsy;
+            astig_sigma(c.ax, t[iZ], sgx, dsx);
+            astig_sigma(c.ay, t[iZ], sgy, dsy);
+            X.cp = 1.0f / (r2pi * sgx); X.c2 = 1.0f / (r2 * sgx); X.cs = ispi / sgx * dsx;
+            Y.cp = 1.0f / (r2pi * sgy); Y.c2 = 1.0f / (r2 * sgy); Y.cs = ispi / sgy * dsy;
+        }
+    }
+
+    // mu and Jacobian of one pixel from its column (X) and row (Y) terms.
+    PP_DEV void pixel(const AxisTerms& X, const AxisTerms& Y, const float (&t)[K], float& mu, float (&J)[K]) const {
+        const float I = t[iI], bg = t[iBg];
+        const float p = X.D * Y.D;
+        mu = bg + I * p;
+        J[0] = I * X.dE * Y.D;
+        J[1] = I * Y.dE * X.D;
+        J[iI] = p;
+        J[iBg] = 1.0f;
+        if (V != kFixed) J[iS] = I * (X.S * Y.D + Y.S * X.D);
+    }
+
+    // Single-pixel evaluation (expected value / derivatives kernels).
+    PP_DEV void eval_pixel(const float (&t)[K], int x, int y, float& mu, float (&J)[K]) const {
+        AxisShape SX, SY;
+        shapes(t, SX, SY);
+        const Edge x0 = gauss_edge((float)x, t[0], SX.c2), x1 = gauss_edge((float)(x + 1), t[0], SX.c2);
+        const Edge y0 = gauss_edge((float)y, t[1], SY.c2), y1 = gauss_edge((float)(y + 1), t[1], SY.c2);
+        pixel(axis_terms(x0, x1, SX.cp, SX.cs), axis_terms(y0, y1, SY.cp, SY.cs), t, mu, J);
+    }
+
+    // Fit-kernel evaluation over a block of RL rows starting at y0 of an RxR ROI.
+    // f(r, x, mu, J) is called for every pixel of rows y0+r < R.
+    template <int R, int RL, class F>
+    PP_DEV void for_rows(const float (&t)[K], int y0, F&& f) const {
+        AxisShape SX, SY;
+        shapes(t, SX, SY);
+        AxisTerms col[R];
+        {
+            Edge prev = gauss_edge(0.0f, t[0], SX.c2);
+#pragma unroll
+            for (int x = 0; x < R; x++) {
+                Edge nx = gauss_edge((float)(x + 1), t[0], SX.c2);
+                col[x] = axis_terms(prev, nx, SX.cp, SX.cs);
+                prev = nx;
+            }
+        }
+        Edge prev = gauss_edge((float)y0, t[1], SY.c2);
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            const int y = y0 + r;
+            Edge nx = gauss_edge((float)(y + 1), t[1], SY.c2);
+            const AxisTerms row = axis_terms(prev, nx, SY.cp, SY.cs);
+            prev = nx;
+            if (y < R) {
+#pragma unroll
+                for (int x = 0; x < R; x++) {
+                    float mu, J[K];
+                    pixel(col[x], row, t, mu, J);
+                    f(r, x, mu, J);
+                }
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Cubic-spline PSF: coefs[Nz][Ny][Nx][64], coefficient q = 16*m + 4*n + o
+// multiplies z^m y^n x^o of the sub-voxel position.
+struct SplineModel {
+    static constexpr int K = 5;
+    static constexpr int iI = 3, iBg = 4;
+    int Nz, Ny, Nx;
+    float zmin, zmax;
+    const float* coefs;
+
+    PP_DEV static float stop_limit(int k) { return (k == 2) ? 1e-7f : 1e-4f; }
+
+    PP_DEV void check_limits(float (&t)[K], int R) const {
+        const float hi = (float)R - 1.0f - 1.5f;
+        t[0] = clampf(t[0], 1.5f, hi);
+        t[1] = clampf(t[1], 1.5f, hi);
+        t[2] = clampf(t[2], zmin + 1e-3f, zmax - 1e-3f);
+        t[3] = fmaxf(t[3], 10.f);
+        t[4] = fmaxf(t[4], 0.0f);
+    }
+
+    PP_DEV void initial(float cx, float cy, float sum, float (&t)[K]) const {
+        t[0] = cx; t[1] = cy; t[2] = 0.0f; t[3] = sum; t[4] = 0.0f;
+    }
+
+    struct Frame {
+        float tx, ty;              // position relative to the spline grid centre
+        float du;
+        float x1, x2, x3;          // x, x^2, x^3 of the sub-pixel x
+        float dx2, dx3;            // 2x, 3x^2
+        float yp[4], dyp[4];       // y^n, n y^(n-1)
+        float zp[4], dzp[4];       // z^m, m z^(m-1)
+        const float* slab;         // coefficients of z-slab k
+    };
+
+    PP_DEV void frame(const float (&t)[K], Frame& F) const {
+        F.du = (zmax - zmin) / (float)Nz;
+        F.tx = t[0] - (float)Nx * 0.5f;
+        F.ty = t[1] - (float)Ny * 0.5f;
+        float spx = -F.tx; spx -= (float)(int)spx;   // truncation toward zero (reference quirk)
+        float spy = -F.ty; spy -= (float)(int)spy;
+        float spz = (t[2] - zmin) / F.du;
+        const int k = clampi((int)spz, 0, Nz - 1);
+        spz = clampf(spz - (float)k, 0.0f, 1.0f);
+        F.x1 = spx; F.x2 = spx * spx; F.x3 = F.x2 * spx;
+        F.dx2 = 2.0f * spx; F.dx3 = 3.0f * F.x2;
+        F.yp[0] = 1.0f; F.yp[1] = spy; F.yp[2] = spy * spy; F.yp[3] = F.yp[2] * spy;
+        F.dyp[0] = 0.0f; F.dyp[1] = 1.0f; F.dyp[2] = 2.0f * spy; F.dyp[3] = 3.0f * F.yp[2];
+        F.zp[0] = 1.0f; F.zp[1] = spz; F.zp[2] = spz * spz; F.zp[3] = F.zp[2] * spz;
+        F.dzp[0] = 0.0f; F.dzp[1] = 1.0f; F.dzp[2] = 2.0f * spz; F.dzp[3] = 3.0f * F.zp[2];
+        F.slab = coefs + (size_t)64 * ((size_t)k * Nx * Ny);
+    }
+
+    // value and x/y/z derivatives of the spline cell c at the frame's sub-voxel position
+    PP_DEV static void cell(const float* c, const Frame& F, float& f, float& fx, float& fy, float& fz) {
+        const float4* c4 = reinterpret_cast<const float4*>(c);
+        float fm[4], fxm[4], fym[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            float a = 0.0f, ax = 0.0f, ay = 0.0f;
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                const float4 q = c4[m * 4 + n];
+                const float px = q.x + q.y * F.x1 + q.z * F.x2 + q.w * F.x3;
+                const float dpx = q.y + q.z * F.dx2 + q.w * F.dx3;
+                a += px * F.yp[n];
+                ax += dpx * F.yp[n];
+                ay += px * F.dyp[n];
+            }
+            fm[m] = a; fxm[m] = ax; fym[m] = ay;
+        }
+        f = fm[0] + fm[1] * F.zp[1] + fm[2] * F.zp[2] + fm[3] * F.zp[3];
+        fx = fxm[0] + fxm[1] * F.zp[1] + fxm[2] * F.zp[2] + fxm[3] * F.zp[3];
+        fy = fym[0] + fym[1] * F.zp[1] + fym[2] * F.zp[2] + fym[3] * F.zp[3];
+        fz = fm[1] + fm[2] * F.dzp[2] + fm[3] * F.dzp[3];
+    }
+
+    PP_DEV const float* row_ptr(const Frame& F, int y) const {
+        const int j = clampi((int)((float)y - F.ty), 0, Ny - 1);
+        return F.slab + (size_t)64 * ((size_t)j * Nx);
+    }
+
+    PP_DEV int col_index(const Frame& F, int x) const { return clampi((int)((float)x - F.tx), 0, Nx - 1); }
+
+    // mu and J from the raw spline value and derivatives.  Also returns the
+    // unclamped value (its sign gates the coefficient gradient).
+    PP_DEV static void finish(const Frame& F, const float (&t)[K], float f, float fx, float fy, float fz, float& mu,
+                              float (&J)[K]) {
+        const float I = t[3];
+        J[0] = fx * -I;
+        J[1] = fy * -I;
+        J[2] = fz * (I / F.du);
+        const float psf = f < 0.0f ? 0.0f : f;
+        J[3] = psf;
+        J[4] = 1.0f;
+        mu = I * psf + t[4];
+    }
+
+    PP_DEV void eval_pixel(const float (&t)[K], int x, int y, float& mu, float (&J)[K]) const {
+        Frame F;
+        frame(t, F);
+        const float* c = row_ptr(F, y) + 64 * col_index(F, x);
+        float f, fx, fy, fz;
+        cell(c, F, f, fx, fy, fz);
+        finish(F, t, f, fx, fy, fz, mu, J);
+    }
+
+    template <int R, int RL, class Fn>
+    PP_DEV void for_rows(const float (&t)[K], int y0, Fn&& fn) const {
+        Frame F;
+        frame(t, F);
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            const int y = y0 + r;
+            if (y < R) {
+                const float* rp = row_ptr(F, y);
+#pragma unroll
+                for (int x = 0; x < R; x++) {
+                    const float* c = rp + 64 * col_index(F, x);
+                    float f, fx, fy, fz, mu, J[K];
+                    cell(c, F, f, fx, fy, fz);
+                    finish(F, t, f, fx, fy, fz, mu, J);
+                    fn(r, x, mu, J);
+                }
+            }
+        }
+    }
+};
+
+}  // namespace ppa
