@@ -31,7 +31,7 @@ class OrModel(ct.Structure):
     _fields_ = [
         ("kind", ct.c_int32),
         ("roisize", ct.c_int32),
-        ("p", ct.c_float * 16),
+        ("p", ct.c_double * 16),
         ("shape", ct.c_int32 * 3),
         ("coefs", ct.c_void_p),
     ]

@@ -67,7 +67,9 @@ void pool_for(int n, int nthreads, F fn) {
     for (auto& t : th) t.join();
 }
 
-constexpr float kPi = 3.141592653589793f;  // MathUtils.h:95 (a float literal)
+// MathUtils.h:95 defines pi as a float literal; the FP64 instantiation uses the double value.
+template <typename T> constexpr T kPiT = T(3.141592653589793);
+template <> constexpr float kPiT<float> = 3.141592653589793f;
 
 // ------------------------------------------------------------- model desc
 // kind: 0 = 2D Gaussian fixed sigma (x,y,I,bg)
@@ -77,7 +79,7 @@ constexpr float kPi = 3.141592653589793f;  // MathUtils.h:95 (a float literal)
 struct OrModel {
     int kind;
     int roisize;
-    float p[16];  // 0: sx,sy | 1: initialSigma | 2: x[4],y[4],minz,maxz,Bx,By | 3: zmin,zmax
+    double p[16]; // 0: sx,sy | 1: initialSigma | 2: x[4],y[4],minz,maxz,Bx,By | 3: zmin,zmax
     int shape[3]; // spline (Nz,Ny,Nx)
     const float* coefs;
 };
@@ -96,9 +98,9 @@ struct Gauss2DFixed {
     }
     void Initial(const T com[3], T* t) const { t[0] = com[0]; t[1] = com[1]; t[2] = com[2] * T(0.7f); t[3] = 0; }
     template <typename CB> void Eval(const T* th, CB cb) const {
-        const T cpx = T(1.0f) / (std::sqrt(T(2 * kPi)) * sx);
+        const T cpx = T(1.0f) / (std::sqrt(T(2) * kPiT<T>) * sx);
         const T c2x = T(1.0f) / (std::sqrt(T(2)) * sx);
-        const T cpy = T(1.0f) / (std::sqrt(T(2 * kPi)) * sy);
+        const T cpy = T(1.0f) / (std::sqrt(T(2) * kPiT<T>) * sy);
         const T c2y = T(1.0f) / (std::sqrt(T(2)) * sy);
         const T tx = th[0], ty = th[1], tI = th[2], tbg = th[3];
         for (int y = 0; y < R; y++) {
@@ -136,8 +138,8 @@ struct Gauss2DSigma {
     void Initial(const T com[3], T* t) const { t[0] = com[0]; t[1] = com[1]; t[2] = com[2]; t[3] = 0; t[4] = initialSigma; }
     template <typename CB> void Eval(const T* th, CB cb) const {
         const T tx = th[0], ty = th[1], tI = th[2], tbg = th[3], ts = th[4];
-        const T isp = T(1.0f) / std::sqrt(T(kPi));
-        const T cp = T(1.0f) / (std::sqrt(T(2 * kPi)) * ts);
+        const T isp = T(1.0f) / std::sqrt(kPiT<T>);
+        const T cp = T(1.0f) / (std::sqrt(T(2) * kPiT<T>) * ts);
         const T c2 = T(1.0f) / (std::sqrt(T(2)) * ts);
         for (int y = 0; y < R; y++) {
             T a0 = (T(y) - ty + T(.5f)) * c2;
@@ -186,9 +188,9 @@ struct Astig3D {
         const T qy = T(1.0f) + zy2 / dy2 + Ay * zy3 / dy2 + By * zy4 / dy2;
         const T sgx = s0x * std::sqrt(qx);
         const T sgy = s0y * std::sqrt(qy);
-        const T cpx = T(1.0f) / (std::sqrt(T(2 * kPi)) * sgx);
+        const T cpx = T(1.0f) / (std::sqrt(T(2) * kPiT<T>) * sgx);
         const T c2x = T(1.0f) / (std::sqrt(T(2)) * sgx);
-        const T cpy = T(1.0f) / (std::sqrt(T(2 * kPi)) * sgy);
+        const T cpy = T(1.0f) / (std::sqrt(T(2) * kPiT<T>) * sgy);
         const T c2y = T(1.0f) / (std::sqrt(T(2)) * sgy);
         for (int y = 0; y < R; y++) {
             T a0 = (T(y) - ty + T(.5f)) * c2y;
@@ -196,7 +198,7 @@ struct Astig3D {
             T DY = T(0.5f) * std::erf(a0) - T(0.5f) * std::erf(a1);
             T dEy = cpy * (std::exp(-a1 * a1) - std::exp(-a0 * a0));
             T ym = T(y) - ty - T(0.5f), yp = T(y) - ty + T(0.5f);
-            T G21y = T(1) / (std::sqrt(T(2.0f) * T(kPi)) * sgy * sgy) *
+            T G21y = T(1) / (std::sqrt(T(2.0f) * kPiT<T>) * sgy * sgy) *
                      (ym * std::exp(-ym * ym / (T(2.0f) * sgy * sgy)) - yp * std::exp(-yp * yp / (T(2.0f) * sgy * sgy)));
             for (int x = 0; x < R; x++) {
                 T b0 = (T(x) - tx + T(.5f)) * c2x;
@@ -207,7 +209,7 @@ struct Astig3D {
                 T dmdx = tI * dEx * DY;
                 T dmdy = tI * dEy * DX;
                 T xm = T(x) - tx - T(0.5f), xp = T(x) - tx + T(0.5f);
-                T G21x = T(1) / (std::sqrt(T(2.0f) * T(kPi)) * sgx * sgx) *
+                T G21x = T(1) / (std::sqrt(T(2.0f) * kPiT<T>) * sgx * sgx) *
                          (xm * std::exp(-xm * xm / (T(2.0f) * sgx * sgx)) - xp * std::exp(-xp * xp / (T(2.0f) * sgx * sgx)));
                 T dMsx = tI * DY * G21x;
                 T dMsy = tI * DX * G21y;

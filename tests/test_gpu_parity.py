@@ -1,0 +1,250 @@
+"""GPU parity of the HIP estimator with the CPU oracle (reference restatement).
+
+Bar (spot-fitting contract): on converged spots the fitted x, y agree within
+1e-3 px, z within 1 nm, and I, bg within 1e-4 relative (floor 1 photon);
+expected values and Jacobians agree to FP32 rounding.  Sizes are small enough
+for the oracle to finish in seconds; the full-size runs use size-independent
+properties (noise-free fixed point, CRLB-normalised error statistics).
+"""
+import numpy as np
+import pytest
+
+import photonpy_amd as pa
+from oracle import oracle as O
+from tests import _closed as C
+from tests import _models as MD
+
+pytestmark = pytest.mark.gpu
+
+KINDS = ["fixed", "sigma", "astig", "spline"]
+TOL = {"xy": 1e-3, "z": 1.0, "rel": 1e-4}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    with pa.Context() as c:
+        yield c
+
+
+def _zcol(kind):
+    return 2 if kind in ("astig", "spline") else None
+
+
+def _on_bounds(p, lo, hi):
+    return np.any((p <= lo + 1e-6) | (p >= hi - 1e-6), axis=1)
+
+
+def _compare(kind, a, b):
+    """Per-spot boolean: a and b agree within the parity tolerance."""
+    ok = (np.abs(a[:, 0] - b[:, 0]) <= TOL["xy"]) & (np.abs(a[:, 1] - b[:, 1]) <= TOL["xy"])
+    iI = 3 if _zcol(kind) else 2
+    ibg = iI + 1
+    ok &= np.abs(a[:, iI] - b[:, iI]) <= TOL["rel"] * np.maximum(np.abs(b[:, iI]), 1)
+    ok &= np.abs(a[:, ibg] - b[:, ibg]) <= TOL["rel"] * np.maximum(np.abs(b[:, ibg]), 1)
+    if _zcol(kind):
+        ok &= np.abs(a[:, 2] - b[:, 2]) <= TOL["z"]
+    if kind == "sigma":
+        ok &= np.abs(a[:, 4] - b[:, 4]) <= 1e-4 * np.maximum(np.abs(b[:, 4]), 1)
+    return ok
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_expected_value_and_derivatives(ctx, kind):
+    th, _ = MD.samples(kind, 200, 11)
+    est, om, fn = MD.make(kind, ctx)
+    ev = est.ExpectedValue(th)
+    dv, ev2 = est.Derivatives(th)
+    ev32 = om.expected_value(th, np.float32)
+    dv32, _ = om.derivatives(th, np.float32)
+    mu64 = fn(th)
+    scale = np.abs(mu64).max(axis=(1, 2))[:, None, None]
+    assert np.max(np.abs(ev - mu64) / scale) < 2e-6
+    assert np.max(np.abs(ev - ev32) / scale) < 2e-6
+    assert np.array_equal(ev, ev2)
+    dscale = np.maximum(np.abs(dv32).max(axis=(2, 3), keepdims=True), 1e-6)
+    assert np.max(np.abs(dv - dv32) / dscale) < 5e-5
+    if kind == "spline":
+        # the GPU contracts the 64 coefficients in a different order: same as FP64 oracle to FP32 rounding
+        dv64, _ = om.derivatives(th, np.float64)
+        assert np.max(np.abs(dv - dv64) / dscale) < 5e-5
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_estimate_parity(ctx, kind):
+    n = {"fixed": 4000, "sigma": 4000, "astig": 1500, "spline": 600}[kind]
+    iters = 30 if kind == "fixed" else 50
+    th, smp = MD.samples(kind, n, 3)
+    est, om, _ = MD.make(kind, ctx)
+    est.SetLevMarParams(1e-15 if kind == "fixed" else 1e-18, iters)
+    p_gpu, _, traces = est.Estimate(smp)
+    it_gpu = est.last_iterations
+    lam = 1e-15 if kind == "fixed" else 1e-18
+    p_ref, it_ref, tr_ref = om.estimate(smp, iterations=iters, lambda_=lam, trace_len=iters)
+    assert np.array_equal([len(t) for t in traces], np.minimum(it_gpu, iters))
+
+    lo, hi = est.ParamLimits()
+    if kind in ("fixed", "sigma"):
+        conv = (it_ref <= iters) & (it_gpu <= iters)
+    else:
+        # the I (and spline z) stop limits are below one FP32 ulp, so these fits run to the
+        # budget; "converged" = the oracle's last step is below 1e-4 px, 0.1 nm and 1e-5 I
+        last = p_ref - tr_ref[:, -1]
+        conv = (np.abs(last[:, 0]) < 1e-4) & (np.abs(last[:, 1]) < 1e-4) & (np.abs(last[:, 2]) < 0.1) & \
+               (np.abs(last[:, 3]) < 1e-5 * np.abs(p_ref[:, 3]))
+    conv &= ~_on_bounds(p_ref, lo, hi) & np.all(np.isfinite(p_ref), axis=1)
+    assert conv.mean() > 0.8, f"only {conv.mean():.3f} of spots converged"
+    ok = _compare(kind, p_gpu[conv], p_ref[conv])
+    bad = np.where(~ok)[0]
+    assert ok.all(), f"{len(bad)} of {conv.sum()} converged spots differ, e.g. gpu={p_gpu[conv][bad[:3]]} ref={p_ref[conv][bad[:3]]}"
+    # iteration counts agree for nearly every spot
+    assert np.mean(it_gpu == it_ref) > 0.9
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_crlb_and_chisq(ctx, kind):
+    th, smp = MD.samples(kind, 500, 5)
+    est, om, _ = MD.make(kind, ctx)
+    crlb, chi = est.CRLBAndChiSquare(th, smp)
+    c64, x64 = om.crlb(th, smp, dtype=np.float64)
+    c32, x32 = om.crlb(th, smp, dtype=np.float32)
+    assert np.max(np.abs(crlb - c64) / c64) < 1e-4
+    assert np.max(np.abs(crlb - c32) / c32) < 2e-3
+    assert np.max(np.abs(chi - x64) / np.maximum(x64, 1)) < 1e-4
+    # without samples chi^2 is zero and the CRLB is unchanged
+    crlb2 = est.CRLBAndChiSquare(th)
+    assert np.allclose(crlb2, crlb, rtol=1e-6)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_fused_fit_crlb_matches_separate_calls(ctx, kind):
+    import torch
+    th, smp = MD.samples(kind, 700, 9)
+    est, _, _ = MD.make(kind, ctx)
+    d_smp = torch.from_numpy(smp).cuda()
+    out = est.estimate_device(d_smp, crlb=True)
+    torch.cuda.synchronize()
+    p = out["params"].cpu().numpy()
+    p_host, _, _ = est.Estimate(smp)
+    assert np.array_equal(p, p_host)
+    assert np.array_equal(out["iterations"].cpu().numpy(), est.last_iterations)
+    crlb, chi = est.CRLBAndChiSquare(p, smp)
+    ok = np.all(np.isfinite(crlb), axis=1)
+    assert np.allclose(out["crlb"].cpu().numpy()[ok], crlb[ok], rtol=1e-4)
+    assert np.allclose(out["chisq"].cpu().numpy()[ok], chi[ok], rtol=1e-4, atol=1e-3)
+
+
+def test_iteration_and_trace_semantics(ctx):
+    th, smp = MD.samples("fixed", 300, 13)
+    est, om, _ = MD.make("fixed", ctx)
+    est.SetLevMarParams(1e-15, 0)
+    p0, _, tr0 = est.Estimate(smp, initial=th.astype(np.float32))
+    assert np.all(est.last_iterations == 1) and np.array_equal(p0, th.astype(np.float32))
+    est.SetLevMarParams(1e-15, 2)
+    est.Estimate(smp)
+    _, it_ref = om.estimate(smp, iterations=2, lambda_=1e-15)
+    assert np.array_equal(est.last_iterations, it_ref)
+    est.SetLevMarParams(1e-15, 30)
+    p, _, tr = est.Estimate(smp, initial=th.astype(np.float32))
+    assert all(np.array_equal(t[0], th[i].astype(np.float32)) for i, t in enumerate(tr))
+    assert np.all(est.last_iterations >= 2)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_degenerate_rois(ctx, kind):
+    R = MD.ROI[kind]
+    est, om, _ = MD.make(kind, ctx)
+    est.SetLevMarParams(1e-18, 20)
+    smp = np.zeros((4, R, R), np.float32)
+    smp[1] = 7.0                      # flat
+    smp[2, 0, 0] = 50.0               # single corner pixel
+    smp[3] = 5.0
+    smp[3, R // 2, R // 2] = np.nan   # NaN pixel
+    p, _, _ = est.Estimate(smp)
+    pr, _ = om.estimate(smp, iterations=20)
+    # same finiteness pattern as the oracle; the clamp box always holds for x, y
+    assert np.array_equal(np.isfinite(p), np.isfinite(pr))
+    fin = np.isfinite(p[:, 0])
+    assert np.all((p[fin, 0] >= 1.5) & (p[fin, 0] <= R - 2.5))
+
+
+def test_unsupported_roisize_raises(ctx):
+    est = pa.GaussianPSFMethods(ctx).CreatePSF_XYIBg(40, 1.5, True)
+    with pytest.raises(pa.NativeError, match="roisize"):
+        est.Estimate(np.ones((2, 40, 40), np.float32))
+    # expected values work at any size
+    assert est.ExpectedValue(np.array([[20.0, 20.0, 100.0, 1.0]])).shape == (1, 40, 40)
+
+
+@pytest.mark.parametrize("R,G", [(10, 4), (11, 4), (16, 8), (20, 16), (7, 2)])
+def test_group_sizes_agree(ctx, R, G):
+    th, smp = MD.samples("sigma", 800, 21, R=R)
+    est, _, _ = MD.make("sigma", ctx, R=R)
+    p1, _, _ = est.Estimate(smp)
+    it1 = est.last_iterations.copy()
+    est.SetGroupSize(G)
+    p2, _, _ = est.Estimate(smp)
+    conv = (it1 <= 50) & (est.last_iterations <= 50)
+    assert _compare("sigma", p2[conv], p1[conv]).all()
+
+
+def test_spline_coefficients_match_oracle(ctx):
+    stack, _ = C.spline_zstack(nz=12, ny=15, nx=15)
+    c_gpu = pa.CSplineMethods(ctx).ComputeSplineCoeff(stack)
+    c_ref = O.spline_compute(stack)
+    assert c_gpu.shape == c_ref.shape
+    assert np.max(np.abs(c_gpu - c_ref)) <= 1e-6 * max(np.abs(c_ref).max(), 1)
+    c_b = pa.CSplineMethods(ctx).ComputeSplineCoeff(stack, border=1)
+    assert np.allclose(c_b, O.spline_compute(stack, border=1), atol=1e-6)
+
+
+# ------------------------------------------------------------- full size
+def _full_size_fixed_point(ctx, kind, n, seed):
+    import torch
+    rng = np.random.default_rng(seed)
+    R = MD.ROI[kind]
+    th = C.synth_params(kind, R, n, rng).astype(np.float32)
+    est, _, _ = MD.make(kind, ctx)
+    d_th = torch.from_numpy(th).cuda()
+    mu = est.expected_value_device(d_th)
+    start = th.copy()
+    start[:, :2] += 0.2
+    iI = 3 if kind in ("astig", "spline") else 2
+    start[:, iI] *= 1.05
+    out = est.estimate_device(mu, initial=torch.from_numpy(start).cuda(), crlb=True)
+    p = out["params"].cpu().numpy()
+    del mu
+    err = np.abs(p - th)
+    ok = (err[:, 0] < 1e-3) & (err[:, 1] < 1e-3) & (err[:, iI] < 1e-3 * th[:, iI])
+    if kind in ("astig", "spline"):
+        ok &= err[:, 2] < 1.0
+    return ok.mean(), out
+
+
+@pytest.mark.parametrize("kind,n", [("sigma", 1_000_000), ("astig", 2_000_000), ("fixed", 10_000)])
+def test_full_size_noise_free_fixed_point(ctx, kind, n):
+    frac, out = _full_size_fixed_point(ctx, kind, n, 99)
+    assert frac > 0.999, frac
+    crlb = out["crlb"].cpu().numpy()
+    assert np.mean(np.all(np.isfinite(crlb) & (crlb > 0), axis=1)) > 0.999
+
+
+def test_full_size_error_statistics_match_crlb(ctx):
+    """C2 at full size: normalised errors (fit - truth)/CRLB have unit spread."""
+    import torch
+    n = 1_000_000
+    rng = np.random.default_rng(5)
+    th = C.synth_params("sigma", 11, n, rng).astype(np.float32)
+    est, _, _ = MD.make("sigma", ctx)
+    mu = est.expected_value_device(torch.from_numpy(th).cuda())
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    smp = torch.poisson(mu, generator=gen)
+    out = est.estimate_device(smp, crlb=True)
+    p = out["params"].cpu().numpy()
+    crlb = out["crlb"].cpu().numpy()
+    it = out["iterations"].cpu().numpy()
+    good = (it <= 50) & (th[:, 2] > 2000) & np.all(np.isfinite(crlb), axis=1)
+    # CRLB of x, y, I (sigma's CRLB carries the reference's 1/sigma scaling)
+    for k in (0, 1, 2):
+        z = (p[good, k] - th[good, k]) / crlb[good, k]
+        assert abs(np.std(z) - 1) < 0.1, (k, np.std(z))
+        assert abs(np.mean(z)) < 0.05
