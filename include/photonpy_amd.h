@@ -141,6 +141,22 @@ int Estimator_DerivativesDevice(Estimator* e, int n, const float* d_params, floa
 int Estimator_ChiSquareAndCRLBDevice(Estimator* e, int n, const float* d_params, const float* d_sample, float* d_crlb,
                                      float* d_chisq, void* stream);
 
+/* Poisson negative log-likelihood per spot, sum_px mu' - x' log mu' with mu' = max(mu,1e-8),
+ * x' = max(x,1e-8) (the negated ComputeLogLikelihood of Estimation.h:138-156), and its adjoint.
+ * mu and the Jacobian are recomputed from d_params; nothing from a forward pass is stored.
+ * d_shared: the shared PSF parameters (NULL = the estimator's calibration) in the layout
+ *   fixed sigma {sigma_x, sigma_y}; astigmatic {s0, gamma, d, A, B} for x then y;
+ *   spline the coefficient table [Nz,Ny,Nx,64]; free sigma: none.
+ * d_gout [n]: upstream gradient of each spot's NLL (NULL = 1).  Outputs (each nullable):
+ * d_nll [n], d_grad_params [n,K] (true derivatives), d_grad_shared (accumulated with +=).
+ * roisize <= 32. */
+int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const float* d_sample, const float* d_shared,
+                            const float* d_gout, float* d_nll, float* d_grad_params, float* d_grad_shared,
+                            void* stream);
+/* Number of shared PSF parameters and a host copy of their current values. */
+int Estimator_NumShared(Estimator* e);
+int Estimator_GetShared(Estimator* e, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
@@ -56,12 +57,42 @@ struct Estimator {
     void* ws = nullptr;
     size_t wsBytes = 0;
 
+    float* devShared = nullptr;  // device copy of the shared PSF parameters (Gaussian models)
+
     int K() const { return num_params(desc.kind); }
+
+    // Shared PSF parameters in the adjoint's layout
+    std::vector<float> shared_values() const {
+        if (desc.kind == kKindFixed) return {desc.gc.sx, desc.gc.sy};
+        if (desc.kind == kKindAstig) {
+            std::vector<float> v(desc.gc.ax, desc.gc.ax + 5);
+            v.insert(v.end(), desc.gc.ay, desc.gc.ay + 5);
+            return v;
+        }
+        if (desc.kind == kKindSpline) return hostCoefs;
+        return {};
+    }
+    size_t num_shared() const {
+        if (desc.kind == kKindSpline) return hostCoefs.size();
+        return shared_values().size();
+    }
+    const float* device_shared() {
+        ensure_device();
+        if (desc.kind == kKindSpline) return devCoefs;
+        std::vector<float> v = shared_values();
+        if (v.empty()) return nullptr;
+        if (!devShared) {
+            hip_check(hipMalloc(&devShared, v.size() * sizeof(float)), "hipMalloc(shared)");
+            hip_check(hipMemcpy(devShared, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice), "upload shared");
+        }
+        return devShared;
+    }
     int S() const { return roisize * roisize; }
     int G() const { return group > 0 ? group : default_group(roisize); }
 
     ~Estimator() {
         if (devCoefs) (void)hipFree(devCoefs);
+        if (devShared) (void)hipFree(devShared);
         if (ws) (void)hipFree(ws);
     }
 
@@ -530,6 +561,40 @@ int Estimator_ChiSquareAndCRLBDevice(Estimator* e, int n, const float* d_params,
             e->ensure_device();
             hip_check(launch_crlb_any(e->desc, e->roisize, n, d_params, d_sample, d_crlb, d_chisq, as_stream(stream)),
                       "crlb kernel");
+            return 0;
+        },
+        2);
+}
+
+int Estimator_NumShared(Estimator* e) { return e ? (int)e->num_shared() : -1; }
+
+int Estimator_GetShared(Estimator* e, float* out) {
+    if (!e || !out) return 1;
+    std::vector<float> v = e->shared_values();
+    std::copy(v.begin(), v.end(), out);
+    return 0;
+}
+
+int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const float* d_sample, const float* d_shared,
+                            const float* d_gout, float* d_nll, float* d_grad_params, float* d_grad_shared,
+                            void* stream) {
+    if (!e) return 1;
+    if (n <= 0) return 0;
+    return guarded(
+        [&]() -> int {
+            std::lock_guard<std::mutex> l(e->mtx);
+            if (e->roisize > 32) throw std::runtime_error("photonpy_amd: the adjoint supports roisize <= 32");
+            AdjArgs a{};
+            a.n = n;
+            a.R = e->roisize;
+            a.params = d_params;
+            a.sample = d_sample;
+            a.gout = d_gout;
+            a.shared = d_shared ? d_shared : e->device_shared();
+            a.nll = d_nll;
+            a.gparams = d_grad_params;
+            a.gshared = d_grad_shared;
+            hip_check(launch_adjoint_any(e->desc, a, as_stream(stream)), "adjoint kernel");
             return 0;
         },
         2);

@@ -41,6 +41,20 @@ hipError_t launch_ev_any(const ModelDesc& d, int R, int n, const float* params, 
 hipError_t launch_crlb_any(const ModelDesc& d, int R, int n, const float* params, const float* sample, float* crlb,
                            float* chisq, hipStream_t st);
 
+// Poisson NLL and its adjoint (adjoint.hip)
+struct AdjArgs {
+    int n, R;
+    const float* params;   // [n,K]
+    const float* sample;   // [n,R,R]
+    const float* gout;     // [n] upstream gradient or null (= 1)
+    const float* shared;   // device copy of the shared PSF parameters
+    float* nll;            // [n] or null
+    float* gparams;        // [n,K] or null
+    float* gshared;        // accumulated (+=) or null
+};
+hipError_t launch_adjoint_any(const ModelDesc& d, const AdjArgs& a, hipStream_t st);
+hipError_t launch_spline_coefs(const int* shape, const float* d_data, float* d_out, int border, hipStream_t st);
+
 inline hipError_t launch_fit_any(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st) {
     switch (d.kind) {
     case kKindFixed: return launch_fit_fixed(d, R, G, a, maxBlocks, st);

@@ -291,6 +291,28 @@ class Estimator:
             raise RuntimeError(f"Estimator_ChiSquareAndCRLBDevice failed ({rc})")
         return crlb, chisq
 
+    # ------------------------------------------------ adjoint (autograd.py)
+    def num_shared(self):
+        """Number of shared PSF parameters (sigma_x/y, astigmatism coefficients or spline table)."""
+        return self._lib.Estimator_NumShared(self.inst)
+
+    def shared_parameters(self, device="cuda"):
+        """Current shared PSF parameters as a float32 tensor (a copy)."""
+        import torch
+        k = self.num_shared()
+        out = np.zeros(k, np.float32)
+        if k:
+            self._lib.Estimator_GetShared(self.inst, out.ctypes.data)
+        return torch.from_numpy(out).to(device)
+
+    def _nll_grad(self, params, samples, shared, gout, nll, gparams, gshared, stream=None):
+        ptr = lambda t: None if t is None else t.data_ptr()
+        rc = self._lib.Estimator_NLLGradDevice(self.inst, params.shape[0], ptr(params), ptr(samples), ptr(shared),
+                                               ptr(gout), ptr(nll), ptr(gparams), ptr(gshared), self._stream(stream))
+        if rc != 0:
+            self._check()
+            raise RuntimeError(f"Estimator_NLLGradDevice failed ({rc})")
+
     def SetGroupSize(self, group):
         """Lane-group size of the fit kernel (power of two, 0 = built-in choice)."""
         if self._lib.Estimator_SetGroupSize(self.inst, int(group)) != 0:

@@ -111,6 +111,9 @@ def _bind(lib):
         "Estimator_ExpectedValueDevice": ([V, I, V, V, V], ct.c_int),
         "Estimator_DerivativesDevice": ([V, I, V, V, V, V], ct.c_int),
         "Estimator_ChiSquareAndCRLBDevice": ([V, I, V, V, V, V, V], ct.c_int),
+        "Estimator_NLLGradDevice": ([V, I, V, V, V, V, V, V, V, V], ct.c_int),
+        "Estimator_NumShared": ([V], ct.c_int),
+        "Estimator_GetShared": ([V, V], ct.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
