@@ -153,6 +153,24 @@ int Estimator_ChiSquareAndCRLBDevice(Estimator* e, int n, const float* d_params,
 int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const float* d_sample, const float* d_shared,
                             const float* d_gout, float* d_nll, float* d_grad_params, float* d_grad_shared,
                             void* stream);
+/* Persistent mixed batch: bins of spots, one estimator (model + ROI size) per bin, fitted by one
+ * persistent launch whose wavefronts claim tiles of (64/G)*spotsPerGroup[b] spots from a work
+ * counter, longest tiles first; with useGraph the reset + launch are replayed from a HIP graph
+ * captured on the first run.  Compiled ROI sizes: 7, 9, 11, 13, 16, 20, 25 (all four models).
+ * spotsPerGroup may be NULL (8 per group); blocks 0 = all resident blocks.  Run takes per-bin
+ * device pointers (arrays of nbins pointers; d_iters/d_crlb/d_chisq may be NULL).  Two runs that
+ * overlap on different streams race on the batch's work counter. */
+typedef struct MixedBatch MixedBatch;
+MixedBatch* MixedBatch_Create(int nbins, Estimator* const* estimators, const int* counts, const int* spotsPerGroup,
+                              int blocks);
+void MixedBatch_Delete(MixedBatch* mb);
+int MixedBatch_NumTiles(MixedBatch* mb);
+int MixedBatch_Run(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
+                   float* const* d_crlb, float* const* d_chisq, int useGraph, void* stream);
+/* The same bins as one plain per-model launch each (the comparison form). */
+int MixedBatch_RunPlain(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
+                        float* const* d_crlb, float* const* d_chisq, void* stream);
+
 /* Number of shared PSF parameters and a host copy of their current values. */
 int Estimator_NumShared(Estimator* e);
 int Estimator_GetShared(Estimator* e, float* out);

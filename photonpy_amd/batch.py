@@ -1,0 +1,101 @@
+"""Persistent mixed-model, mixed-size batch (no reference counterpart).
+
+A ``MixedBatch`` fits several bins of spots, one bin per estimator (model and
+ROI size), in one persistent kernel launch replayed from a HIP graph.
+``run_plain`` fits the same bins with one ordinary launch per bin, which is
+the comparison form.  Tile sizes come from ``photonpy_amd.autotune`` when a
+tuning table exists for the bin's (model, ROI size).
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+
+class MixedBatch:
+    def __init__(self, estimators, counts, spots_per_group=None, blocks=0):
+        if len(estimators) != len(counts) or not estimators:
+            raise ValueError("one spot count per estimator is required")
+        self.estimators = list(estimators)
+        self.counts = [int(c) for c in counts]
+        self._lib = estimators[0]._lib
+        self._smlm = estimators[0]._smlm
+        nb = len(estimators)
+        if spots_per_group is None:
+            from .autotune import lookup_spots_per_group
+            spots_per_group = [lookup_spots_per_group(e) for e in estimators]
+        self.spots_per_group = [int(s) for s in spots_per_group]
+        arr = (ct.c_void_p * nb)(*[e.inst for e in estimators])
+        cnt = (ct.c_int * nb)(*self.counts)
+        spg = (ct.c_int * nb)(*self.spots_per_group)
+        self.inst = self._lib.MixedBatch_Create(nb, arr, cnt, spg, int(blocks))
+        if not self.inst:
+            self._smlm.check()
+            raise RuntimeError("MixedBatch_Create failed")
+
+    def num_tiles(self):
+        return self._lib.MixedBatch_NumTiles(self.inst)
+
+    def allocate_outputs(self, device="cuda", crlb=True):
+        """Per-bin output tensors (params, iterations, crlb, chisq)."""
+        import torch
+        outs = []
+        for e, n in zip(self.estimators, self.counts):
+            o = {"params": torch.empty((n, e.numparams), dtype=torch.float32, device=device),
+                 "iterations": torch.empty(n, dtype=torch.int32, device=device)}
+            if crlb:
+                o["crlb"] = torch.empty((n, e.numparams), dtype=torch.float32, device=device)
+                o["chisq"] = torch.empty(n, dtype=torch.float32, device=device)
+            outs.append(o)
+        return outs
+
+    def _ptrs(self, samples, outs):
+        nb = len(self.estimators)
+        for e, n, s in zip(self.estimators, self.counts, samples):
+            if tuple(s.shape) != (n, *e.sampleshape) or not s.is_contiguous():
+                raise ValueError(f"bin samples must be contiguous [{n}, {e.sampleshape[0]}, {e.sampleshape[1]}]")
+
+        def arr(key, seq):
+            vals = [None if (o is None or o.get(key) is None) else o[key].data_ptr() for o in seq]
+            if all(v is None for v in vals):
+                return None
+            return (ct.c_void_p * nb)(*vals)
+
+        sp = (ct.c_void_p * nb)(*[s.data_ptr() for s in samples])
+        return sp, arr("params", outs), arr("iterations", outs), arr("crlb", outs), arr("chisq", outs)
+
+    def run(self, samples, outs, use_graph=True, stream=None):
+        import torch
+        stream = stream or torch.cuda.current_stream()
+        sp, pp, ip, cp, xp = self._ptrs(samples, outs)
+        rc = self._lib.MixedBatch_Run(self.inst, sp, pp, ip, cp, xp, int(use_graph), stream.cuda_stream)
+        if rc != 0:
+            self._smlm.check()
+            raise RuntimeError(f"MixedBatch_Run failed ({rc})")
+
+    def run_plain(self, samples, outs, stream=None):
+        import torch
+        stream = stream or torch.cuda.current_stream()
+        sp, pp, ip, cp, xp = self._ptrs(samples, outs)
+        rc = self._lib.MixedBatch_RunPlain(self.inst, sp, pp, ip, cp, xp, stream.cuda_stream)
+        if rc != 0:
+            self._smlm.check()
+            raise RuntimeError(f"MixedBatch_RunPlain failed ({rc})")
+
+    def destroy(self):
+        if self.inst:
+            self._lib.MixedBatch_Delete(self.inst)
+            self.inst = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.destroy()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -15,6 +15,7 @@
 
 #include "../../include/photonpy_amd.h"
 #include "dispatch.h"
+#include "mixed.h"
 
 namespace ppa {
 hipError_t launch_spline_coefs(const int* shape, const float* d_data, float* d_out, int border, hipStream_t st);
@@ -595,6 +596,180 @@ int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const fl
             a.gparams = d_grad_params;
             a.gshared = d_grad_shared;
             hip_check(launch_adjoint_any(e->desc, a, as_stream(stream)), "adjoint kernel");
+            return 0;
+        },
+        2);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- mixed batch
+struct MixedBatch {
+    std::vector<Estimator*> est;
+    std::vector<int> counts, groups, variants;
+    std::vector<MixedTile> tiles;
+    MixedBin* hBins = nullptr;  // pinned staging of the bin table
+    MixedBin* dBins = nullptr;
+    MixedTile* dTiles = nullptr;
+    int* dCounter = nullptr;
+    int blocks = 0;
+    hipStream_t capStream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t copied = nullptr;
+    bool copyPending = false;
+
+    ~MixedBatch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (capStream) (void)hipStreamDestroy(capStream);
+        if (copied) (void)hipEventDestroy(copied);
+        if (hBins) (void)hipHostFree(hBins);
+        if (dBins) (void)hipFree(dBins);
+        if (dTiles) (void)hipFree(dTiles);
+        if (dCounter) (void)hipFree(dCounter);
+    }
+};
+
+// relative cost of one spot-iteration per model (pixel work), for longest-first tile order
+static double model_cost(int kind) {
+    switch (kind) {
+    case kKindFixed: return 1.0;
+    case kKindSigma: return 1.3;
+    case kKindAstig: return 1.4;
+    default: return 5.0;
+    }
+}
+
+extern "C" {
+
+MixedBatch* MixedBatch_Create(int nbins, Estimator* const* ests, const int* counts, const int* spotsPerGroup,
+                              int blocks) {
+    return guarded(
+        [&]() -> MixedBatch* {
+            if (nbins <= 0 || !ests || !counts) throw std::runtime_error("photonpy_amd: empty mixed batch");
+            auto* mb = new MixedBatch();
+            try {
+                struct Item { MixedTile t; double cost; };
+                std::vector<Item> items;
+                for (int b = 0; b < nbins; b++) {
+                    Estimator* e = ests[b];
+                    if (!e) throw std::runtime_error("photonpy_amd: NULL estimator in mixed batch");
+                    const int kind = e->desc.kind, R = e->roisize;
+                    int G = e->group > 0 ? e->group : mixed_default_group(kind, R);
+                    const int v = G > 0 ? mixed_variant(kind, R, G) : -1;
+                    if (v < 0)
+                        throw std::runtime_error("photonpy_amd: the persistent batch kernel has no variant for roisize " +
+                                                 std::to_string(R) + " (compiled: 7, 9, 11, 13, 16, 20, 25)");
+                    mb->est.push_back(e);
+                    mb->counts.push_back(counts[b]);
+                    mb->groups.push_back(G);
+                    mb->variants.push_back(v);
+                    const int spg = spotsPerGroup && spotsPerGroup[b] > 0 ? spotsPerGroup[b] : 8;
+                    const int tile = (kWave / G) * spg;
+                    const double c = model_cost(kind) * R * R;
+                    for (int s0 = 0; s0 < counts[b]; s0 += tile) {
+                        const int s1 = std::min(counts[b], s0 + tile);
+                        items.push_back({{b, s0, s1}, c * (s1 - s0) / (kWave / G)});
+                    }
+                }
+                std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
+                for (auto& it : items) mb->tiles.push_back(it.t);
+                hip_check(hipHostMalloc(&mb->hBins, sizeof(MixedBin) * nbins, hipHostMallocDefault), "hipHostMalloc");
+                hip_check(hipMalloc(&mb->dBins, sizeof(MixedBin) * nbins), "hipMalloc");
+                hip_check(hipMalloc(&mb->dCounter, sizeof(int)), "hipMalloc");
+                if (!mb->tiles.empty()) {
+                    hip_check(hipMalloc(&mb->dTiles, sizeof(MixedTile) * mb->tiles.size()), "hipMalloc");
+                    hip_check(hipMemcpy(mb->dTiles, mb->tiles.data(), sizeof(MixedTile) * mb->tiles.size(),
+                                        hipMemcpyHostToDevice), "upload tiles");
+                }
+                hip_check(hipEventCreateWithFlags(&mb->copied, hipEventDisableTiming), "hipEventCreate");
+                mb->blocks = blocks > 0 ? blocks : mixed_resident_blocks();
+                if (mb->blocks <= 0) throw std::runtime_error("photonpy_amd: no HIP device");
+            } catch (...) {
+                delete mb;
+                throw;
+            }
+            return mb;
+        },
+        nullptr);
+}
+
+void MixedBatch_Delete(MixedBatch* mb) { delete mb; }
+
+int MixedBatch_NumTiles(MixedBatch* mb) { return mb ? (int)mb->tiles.size() : -1; }
+
+static void fill_bins(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
+                      float* const* d_crlb, float* const* d_chisq) {
+    if (mb->copyPending) {
+        hip_check(hipEventSynchronize(mb->copied), "wait for bin table copy");
+        mb->copyPending = false;
+    }
+    for (size_t b = 0; b < mb->est.size(); b++) {
+        Estimator* e = mb->est[b];
+        e->ensure_device();
+        MixedBin& B = mb->hBins[b];
+        std::memset(&B, 0, sizeof(B));
+        B.variant = mb->variants[b];
+        B.kind = e->desc.kind;
+        B.gc = e->desc.gc;
+        B.sm = e->desc.sm;
+        B.a.n = mb->counts[b];
+        B.a.sample = d_samples[b];
+        B.a.params = d_params[b];
+        B.a.iterations = d_iters ? d_iters[b] : nullptr;
+        B.a.crlb = d_crlb ? d_crlb[b] : nullptr;
+        B.a.chisq = d_chisq ? d_chisq[b] : nullptr;
+        B.a.maxIter = e->lm.iterations;
+        B.a.lambda = e->lm.lambda;
+        if (!B.a.sample || !B.a.params) throw std::runtime_error("photonpy_amd: NULL sample or params buffer");
+    }
+}
+
+int MixedBatch_Run(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
+                   float* const* d_crlb, float* const* d_chisq, int useGraph, void* stream) {
+    if (!mb) return 1;
+    return guarded(
+        [&]() -> int {
+            hipStream_t st = as_stream(stream);
+            fill_bins(mb, d_samples, d_params, d_iters, d_crlb, d_chisq);
+            hip_check(hipMemcpyAsync(mb->dBins, mb->hBins, sizeof(MixedBin) * mb->est.size(), hipMemcpyHostToDevice, st),
+                      "upload bin table");
+            hip_check(hipEventRecord(mb->copied, st), "hipEventRecord");
+            mb->copyPending = true;
+            const int ntiles = (int)mb->tiles.size();
+            if (!useGraph) {
+                hip_check(launch_mixed(mb->dBins, mb->dTiles, ntiles, mb->dCounter, mb->blocks, st), "mixed kernel");
+                return 0;
+            }
+            if (!mb->exec) {
+                // capture once on a private stream (the caller's may be the legacy default stream)
+                hip_check(hipStreamCreateWithFlags(&mb->capStream, hipStreamNonBlocking), "hipStreamCreate");
+                hip_check(hipStreamBeginCapture(mb->capStream, hipStreamCaptureModeThreadLocal), "begin capture");
+                hipError_t le = launch_mixed(mb->dBins, mb->dTiles, ntiles, mb->dCounter, mb->blocks, mb->capStream);
+                hipError_t ce = hipStreamEndCapture(mb->capStream, &mb->graph);
+                hip_check(le, "mixed kernel (capture)");
+                hip_check(ce, "end capture");
+                hip_check(hipGraphInstantiate(&mb->exec, mb->graph, nullptr, nullptr, 0), "graph instantiate");
+            }
+            hip_check(hipGraphLaunch(mb->exec, st), "graph launch");
+            return 0;
+        },
+        2);
+}
+
+// The same batch as one plain launch per bin (Estimator_EstimateDevice per model and size).
+int MixedBatch_RunPlain(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
+                        float* const* d_crlb, float* const* d_chisq, void* stream) {
+    if (!mb) return 1;
+    return guarded(
+        [&]() -> int {
+            for (size_t b = 0; b < mb->est.size(); b++) {
+                Estimator* e = mb->est[b];
+                std::lock_guard<std::mutex> l(e->mtx);
+                run_fit(e, mb->counts[b], d_samples[b], nullptr, d_params[b], d_iters ? d_iters[b] : nullptr, nullptr,
+                        0, d_crlb ? d_crlb[b] : nullptr, d_chisq ? d_chisq[b] : nullptr, as_stream(stream));
+            }
             return 0;
         },
         2);

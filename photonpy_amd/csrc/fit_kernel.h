@@ -43,8 +43,11 @@ struct FitTraits {
     static constexpr int NA = K * (K + 1) / 2;   // packed alpha
 };
 
+// Fits spots first, first+stride, ... < a.n with the lane groups of the calling
+// wavefront; g_first is this lane group's first spot.  Every lane of the
+// wavefront must call it (the loop runs until all groups of the wave are done).
 template <class M, int R, int G>
-__global__ void __launch_bounds__(256) fit_kernel(M model, FitArgs a) {
+__device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int g_first, int stride) {
     using T = FitTraits<M, R, G>;
     constexpr int K = T::K, RL = T::RL, NA = T::NA;
     static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "G must be a power of two <= 64");
@@ -52,9 +55,7 @@ __global__ void __launch_bounds__(256) fit_kernel(M model, FitArgs a) {
     const int lane = threadIdx.x & (kWave - 1);
     const int g = lane & (G - 1);
     const int y0 = g * RL;
-    const int groupsPerBlock = blockDim.x / G;
-    const int stride = gridDim.x * groupsPerBlock;
-    int spot = blockIdx.x * groupsPerBlock + threadIdx.x / G;
+    int spot = g_first;
 
     float s[RL][R];
     float th[K];
@@ -184,6 +185,12 @@ __global__ void __launch_bounds__(256) fit_kernel(M model, FitArgs a) {
             }
         }
     }
+}
+
+template <class M, int R, int G>
+__global__ void __launch_bounds__(256) fit_kernel(M model, FitArgs a) {
+    const int groupsPerBlock = blockDim.x / G;
+    fit_spots<M, R, G>(model, a, blockIdx.x * groupsPerBlock + threadIdx.x / G, gridDim.x * groupsPerBlock);
 }
 
 }  // namespace ppa

@@ -113,6 +113,13 @@ def _bind(lib):
         "Estimator_ChiSquareAndCRLBDevice": ([V, I, V, V, V, V, V], ct.c_int),
         "Estimator_NLLGradDevice": ([V, I, V, V, V, V, V, V, V, V], ct.c_int),
         "Estimator_NumShared": ([V], ct.c_int),
+        "MixedBatch_Create": ([I, ct.POINTER(V), ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), I], V),
+        "MixedBatch_Delete": ([V], None),
+        "MixedBatch_NumTiles": ([V], ct.c_int),
+        "MixedBatch_Run": ([V, ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), I, V],
+                           ct.c_int),
+        "MixedBatch_RunPlain": ([V, ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), V],
+                                ct.c_int),
         "Estimator_GetShared": ([V, V], ct.c_int),
     }
     for name, (args, res) in sig.items():
