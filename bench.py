@@ -148,6 +148,21 @@ def cpu_baseline(cfg, smp_host, target_s=12.0):
             "sample": f"first {m2} of the same ROIs, oracle Estimate+CRLB+chi^2, {threads} threads, {dt2:.1f} s"}
 
 
+def host_api_rate(est, smp_host):
+    """Localizations/s through Estimator_Estimate with host buffers (H2D + fit + D2H)."""
+    m = len(smp_host)
+    K = est.numparams
+    params = np.zeros((m, K), np.float32)
+    iters = np.zeros(m, np.int32)
+    diag = np.zeros((m, 0), np.float32)
+    cons = np.zeros((m, 0), np.float32)
+    rp = np.zeros((m, 2), np.int32)
+    est._lib.Estimator_Estimate(est.inst, m, smp_host, cons, rp, None, params, diag, iters, None, 0)
+    t1 = time.perf_counter()
+    est._lib.Estimator_Estimate(est.inst, m, smp_host, cons, rp, None, params, diag, iters, None, 0)
+    return m / (time.perf_counter() - t1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +172,9 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override the number of spots")
     ap.add_argument("--group", type=int, default=0, help="lane-group size override (0 = tuned/default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the PCIe-inclusive host-API leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -165,6 +183,16 @@ def main():
 
     import torch
     import photonpy_amd as pa
+
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 5}))
+        return
 
     cfg = CONFIGS[args.config]
     n = args.n or cfg["n"]
@@ -209,25 +237,11 @@ def main():
         achieved = flops / (kernel_ms * 1e-3) / 1e12
 
         # PCIe-inclusive rate through the host-pointer API (samples H2D, results D2H)
-        e2e = None
         m = min(n, 200_000)
         smp_host = smp[:m].cpu().numpy()
-        try:
-            import ctypes as ct
-            K = est.numparams
-            params = np.zeros((m, K), np.float32)
-            iters = np.zeros(m, np.int32)
-            diag = np.zeros((m, 0), np.float32)
-            cons = np.zeros((m, 0), np.float32)
-            rp = np.zeros((m, 2), np.int32)
-            est._lib.Estimator_Estimate(est.inst, m, smp_host, cons, rp, None, params, diag, iters, None, 0)
-            t1 = time.perf_counter()
-            est._lib.Estimator_Estimate(est.inst, m, smp_host, cons, rp, None, params, diag, iters, None, 0)
-            e2e = m / (time.perf_counter() - t1)
-        except Exception as ex:  # pragma: no cover - reporting only
-            e2e = f"failed: {ex}"
+        e2e = None if args.no_e2e else host_api_rate(est, smp_host)
 
-        cpu = None if args.no_cpu else cpu_baseline(cfg, smp_host)
+        cpu = None if args.no_cpu else cpu_baseline(cfg, smp.cpu().numpy())
         group = est.GetGroupSize()
 
     value = n / (ms_per_step * 1e-3)

@@ -15,6 +15,9 @@
 // derivative and clamped spline pixels (negative spline value) contribute no
 // position derivative.
 //
+// The Gaussian adjoint evaluates the model in FP64 (erf/exp of double); the
+// spline adjoint in FP32 (its value has no erf-difference cancellation).
+//
 // Mapping: one wavefront per spot, lanes stride over pixels.  Scalar shared
 // gradients are reduced per block in LDS and added to HBM once per block; the
 // spline coefficient gradient is scattered pixel by pixel with the 64 lanes
@@ -32,43 +35,77 @@ PP_DEV float nll_weight(float mu, float x, float& nll) {
     return mu > 1e-8f ? 1.0f - xv / m : 0.0f;
 }
 
-// Gaussian pixel with true derivatives.  dS[0], dS[1]: dmu/dsigma_x, dmu/dsigma_y.
+// Gaussian pixel with true derivatives, evaluated in FP64: the gradient is a
+// sum of terms of both signs, and FP32 erf differences in the PSF tails would
+// otherwise limit its relative accuracy.  dS[0], dS[1]: dmu/dsigma_x, dmu/dsigma_y.
+struct EdgeD {
+    double E, G, H;
+};
+
+PP_DEV EdgeD gauss_edge_d(double e, double t, double c2) {
+    const double a = (e - t - 0.5) * c2;
+    const double g = exp(-a * a);
+    return {erf(a), g, a * g};
+}
+
 template <int V>
-PP_DEV void gauss_true(const GaussModel<V>& M, const float (&t)[GaussModel<V>::K], int x, int y, float& mu,
-                       float (&J)[GaussModel<V>::K], float (&dS)[2]) {
+PP_DEV void gauss_true(const GaussModel<V>& M, const float (&tf)[GaussModel<V>::K], int x, int y, double& mu,
+                       double (&J)[GaussModel<V>::K], double (&dS)[2]) {
     using Mod = GaussModel<V>;
-    AxisShape SX, SY;
-    M.shapes(t, SX, SY);
-    float sgx, sgy, dsx = 0.0f, dsy = 0.0f;
-    if (V == kFixed) { sgx = M.c.sx; sgy = M.c.sy; }
-    else if (V == kSigma) { sgx = sgy = t[4]; }
-    else { Mod::astig_sigma(M.c.ax, t[2], sgx, dsx); Mod::astig_sigma(M.c.ay, t[2], sgy, dsy); }
-    const float ispi = 1.0f / sqrtf(kPiF);
-    const Edge x0 = gauss_edge((float)x, t[0], SX.c2), x1 = gauss_edge((float)(x + 1), t[0], SX.c2);
-    const Edge y0 = gauss_edge((float)y, t[1], SY.c2), y1 = gauss_edge((float)(y + 1), t[1], SY.c2);
-    // shape column with cs = 1/(sqrt(pi) sigma): d(Delta)/d(sigma)
-    const AxisTerms X = axis_terms(x0, x1, SX.cp, ispi / sgx);
-    const AxisTerms Y = axis_terms(y0, y1, SY.cp, ispi / sgy);
-    const float I = t[Mod::iI], bg = t[Mod::iBg];
-    const float p = X.D * Y.D;
+    constexpr int K = Mod::K;
+    double t[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) t[k] = (double)tf[k];
+    double sgx, sgy, dsx = 0.0, dsy = 0.0;
+    if (V == kFixed) {
+        sgx = M.c.sx;
+        sgy = M.c.sy;
+    } else if (V == kSigma) {
+        sgx = sgy = t[4];
+    } else {
+        auto sig = [&](const float* a, double& s, double& ds) {
+            const double s0 = a[0], gam = a[1], d = a[2], A = a[3], B = a[4], d2 = d * d;
+            const double u = t[2] - gam, u2 = u * u, u3 = u2 * u, u4 = u3 * u;
+            const double sq = sqrt(1.0 + u2 / d2 + A * u3 / d2 + B * u4 / d2);
+            s = s0 * sq;
+            ds = s0 * (2 * u / d2 + A * 3 * u2 / d2 + B * 4 * u3 / d2) / (2 * sq);
+        };
+        sig(M.c.ax, sgx, dsx);
+        sig(M.c.ay, sgy, dsy);
+    }
+    const double r2 = 1.4142135623730951, r2pi = 2.5066282746310002, ispi = 0.5641895835477563;
+    const double c2x = 1.0 / (r2 * sgx), c2y = 1.0 / (r2 * sgy);
+    const EdgeD x0 = gauss_edge_d(x, t[0], c2x), x1 = gauss_edge_d(x + 1, t[0], c2x);
+    const EdgeD y0 = gauss_edge_d(y, t[1], c2y), y1 = gauss_edge_d(y + 1, t[1], c2y);
+    const double DX = 0.5 * (x1.E - x0.E), DY = 0.5 * (y1.E - y0.E);
+    const double dEX = (x0.G - x1.G) / (r2pi * sgx), dEY = (y0.G - y1.G) / (r2pi * sgy);
+    const double SX = (x0.H - x1.H) * ispi / sgx, SY = (y0.H - y1.H) * ispi / sgy;  // d(Delta)/d(sigma)
+    const double I = t[Mod::iI], bg = t[Mod::iBg];
+    const double p = DX * DY;
     mu = bg + I * p;
-    J[0] = I * X.dE * Y.D;
-    J[1] = I * Y.dE * X.D;
+    J[0] = I * dEX * DY;
+    J[1] = I * dEY * DX;
     J[Mod::iI] = p;
-    J[Mod::iBg] = 1.0f;
-    dS[0] = I * X.S * Y.D;
-    dS[1] = I * Y.S * X.D;
+    J[Mod::iBg] = 1.0;
+    dS[0] = I * SX * DY;
+    dS[1] = I * SY * DX;
     if (V == kSigma) J[4] = dS[0] + dS[1];
     if (V == kAstig) J[2] = dS[0] * dsx + dS[1] * dsy;
 }
 
+PP_DEV double nll_weight_d(double mu, float x, double& nll) {
+    const double m = fmax(mu, 1e-8), xv = fmax((double)x, 1e-8);
+    nll += m - xv * log(m);
+    return mu > 1e-8 ? 1.0 - xv / m : 0.0;
+}
+
 // d sigma / d {s0, gamma, d, A, B} of one astigmatic axis
-PP_DEV void astig_sigma_grads(const float* a, float z, float (&g)[5]) {
-    const float s0 = a[0], gam = a[1], d = a[2], A = a[3], B = a[4];
-    const float d2 = d * d;
-    const float t = z - gam, t2 = t * t, t3 = t2 * t, t4 = t3 * t;
-    const float poly = t2 + A * t3 + B * t4;
-    const float sq = sqrtf(1.0f + poly / d2);
+PP_DEV void astig_sigma_grads(const float* a, double z, double (&g)[5]) {
+    const double s0 = a[0], gam = a[1], d = a[2], A = a[3], B = a[4];
+    const double d2 = d * d;
+    const double t = z - gam, t2 = t * t, t3 = t2 * t, t4 = t3 * t;
+    const double poly = t2 + A * t3 + B * t4;
+    const double sq = sqrt(1.0 + poly / d2);
     g[0] = sq;
     g[1] = -s0 * (2 * t + 3 * A * t2 + 4 * B * t3) / (2 * d2 * sq);
     g[2] = -s0 * poly / (d2 * d * sq);
@@ -96,39 +133,43 @@ __global__ void __launch_bounds__(256) gauss_adjoint_kernel(GaussModel<V> M, Adj
         float t[K];
 #pragma unroll
         for (int k = 0; k < K; k++) t[k] = a.params[(size_t)spot * K + k];
-        float acc[K + 3];
+        double acc[K + 3];
 #pragma unroll
-        for (int i = 0; i < K + 3; i++) acc[i] = 0.0f;
+        for (int i = 0; i < K + 3; i++) acc[i] = 0.0;
         for (int p = lane; p < S; p += kWave) {
             const int y = p / R, x = p - (p / R) * R;
-            float mu, J[K], dS[2];
+            double mu, J[K], dS[2];
             gauss_true<V>(M, t, x, y, mu, J, dS);
-            const float w = nll_weight(mu, a.sample[(size_t)spot * S + p], acc[K]);
+            const double w = nll_weight_d(mu, a.sample[(size_t)spot * S + p], acc[K]);
 #pragma unroll
             for (int k = 0; k < K; k++) acc[k] += w * J[k];
             acc[K + 1] += w * dS[0];
             acc[K + 2] += w * dS[1];
         }
-        group_sum_n<kWave>(acc);
+#pragma unroll
+        for (int i = 0; i < K + 3; i++) {
+#pragma unroll
+            for (int m = 1; m < kWave; m <<= 1) acc[i] += __shfl_xor(acc[i], m, kWave);
+        }
         if (lane == 0) {
-            const float go = a.gout ? a.gout[spot] : 1.0f;
-            if (a.nll) a.nll[spot] = acc[K];
+            const double go = a.gout ? a.gout[spot] : 1.0f;
+            if (a.nll) a.nll[spot] = (float)acc[K];
             if (a.gparams) {
 #pragma unroll
-                for (int k = 0; k < K; k++) a.gparams[(size_t)spot * K + k] = go * acc[k];
+                for (int k = 0; k < K; k++) a.gparams[(size_t)spot * K + k] = (float)(go * acc[k]);
             }
             if (a.gshared && NS > 0) {
                 if (V == kFixed) {
-                    atomicAdd(&blockShared[0], go * acc[K + 1]);
-                    atomicAdd(&blockShared[1], go * acc[K + 2]);
+                    atomicAdd(&blockShared[0], (float)(go * acc[K + 1]));
+                    atomicAdd(&blockShared[1], (float)(go * acc[K + 2]));
                 } else if (V == kAstig) {
-                    float gx[5], gy[5];
+                    double gx[5], gy[5];
                     astig_sigma_grads(M.c.ax, t[2], gx);
                     astig_sigma_grads(M.c.ay, t[2], gy);
 #pragma unroll
                     for (int i = 0; i < 5; i++) {
-                        atomicAdd(&blockShared[i], go * acc[K + 1] * gx[i]);
-                        atomicAdd(&blockShared[5 + i], go * acc[K + 2] * gy[i]);
+                        atomicAdd(&blockShared[i], (float)(go * acc[K + 1] * gx[i]));
+                        atomicAdd(&blockShared[5 + i], (float)(go * acc[K + 2] * gy[i]));
                     }
                 }
             }

@@ -45,68 +45,70 @@ constexpr int tri(int a, int b) { return a * K - a * (a - 1) / 2 + (b - a); }
 // Levenberg-Marquardt step in FP64 (reference: Estimation.h:213-228 and
 // SolveMatrix.h:107-163).  alpha is the packed upper triangle, beta the
 // gradient term.  Returns false where the reference breaks out of the loop
-// (an exactly zero pivot).
+// (an exactly zero pivot).  The factorisation works in place on the packed
+// lower triangle, so the solve holds K(K+1)/2 + 2K doubles.
+template <int K>
+constexpr int ltri(int i, int j) { return i * (i + 1) / 2 + j; }  // j <= i
+
 template <int K>
 PP_DEV bool lm_step_f64(const float* alphaTri, const float* beta, float lambda, double* step) {
-    double A[K][K];
+    constexpr int NP = K * (K + 1) / 2;
+    double P[NP];
 #pragma unroll
-    for (int a = 0; a < K; a++)
+    for (int i = 0; i < K; i++)
 #pragma unroll
-        for (int b = a; b < K; b++) {
-            double v = (double)alphaTri[tri<K>(a, b)];
-            A[a][b] = v;
-            A[b][a] = v;
-        }
-    // scale-invariant (lambda >= 0) or additive (lambda < 0) damping
+        for (int j = 0; j <= i; j++) P[ltri<K>(i, j)] = (double)alphaTri[tri<K>(j, i)];
+    // scale-invariant (lambda >= 0) or additive (lambda < 0) damping; every column
+    // sum uses the undamped matrix, as the reference's loop does
     double dg[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
         if (lambda < 0.0f) {
-            dg[k] = A[k][k] - (double)lambda;
+            dg[k] = -(double)lambda;
         } else {
             double s = 0.0;
 #pragma unroll
-            for (int j = 0; j < K; j++) s += A[j][k] * A[j][k];
-            dg[k] = A[k][k] + s * (double)lambda;
+            for (int j = 0; j < K; j++) {
+                const double v = P[j >= k ? ltri<K>(j, k) : ltri<K>(k, j)];
+                s += v * v;
+            }
+            dg[k] = s * (double)lambda;
         }
     }
 #pragma unroll
-    for (int k = 0; k < K; k++) A[k][k] = dg[k];
-    // Cholesky A = L L^T
-    double L[K][K];
+    for (int k = 0; k < K; k++) P[ltri<K>(k, k)] += dg[k];
+    // in-place Cholesky: P <- L
     double invd[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        double s = 0.0;
+        double v = P[ltri<K>(k, k)];
 #pragma unroll
-        for (int p = 0; p < k; p++) s += L[k][p] * L[k][p];
-        double v = A[k][k] - s;
+        for (int p = 0; p < k; p++) v -= P[ltri<K>(k, p)] * P[ltri<K>(k, p)];
         if (v == 0.0) return false;
-        double d = sqrt(v);
-        L[k][k] = d;
+        const double d = sqrt(v);
         invd[k] = 1.0 / d;
 #pragma unroll
         for (int i = k + 1; i < K; i++) {
-            double s2 = 0.0;
+            double w = P[ltri<K>(i, k)];
 #pragma unroll
-            for (int p = 0; p < k; p++) s2 += L[i][p] * L[k][p];
-            L[i][k] = (A[i][k] - s2) * invd[k];
+            for (int p = 0; p < k; p++) w -= P[ltri<K>(i, p)] * P[ltri<K>(k, p)];
+            P[ltri<K>(i, k)] = w * invd[k];
         }
     }
-    double y[K];
+    // L y = beta, then L^T x = y (x overwrites y)
 #pragma unroll
     for (int i = 0; i < K; i++) {
-        double s = 0.0;
+        double v = (double)beta[i];
 #pragma unroll
-        for (int k = 0; k < i; k++) s += L[i][k] * y[k];
-        y[i] = ((double)beta[i] - s) * invd[i];
+        for (int k = 0; k < i; k++) v -= P[ltri<K>(i, k)] * step[k];
+        step[i] = v * invd[i];
     }
 #pragma unroll
     for (int i = K - 1; i >= 0; i--) {
-        double s = 0.0;
+        double v = step[i];
 #pragma unroll
-        for (int k = i + 1; k < K; k++) s += L[k][i] * step[k];
-        step[i] = (y[i] - s) * invd[i];
+        for (int k = i + 1; k < K; k++) v -= P[ltri<K>(k, i)] * step[k];
+        step[i] = v * invd[i];
     }
     return true;
 }

@@ -65,6 +65,19 @@ struct ModelFor<kSpline> {
 
 __global__ void mixed_reset_kernel(int* counter) { *counter = 0; }
 
+// One non-inlined function per variant: each keeps the register allocation of
+// its own per-model kernel instead of the union of all variants.
+#define PPA_MV_FUNC(id, m, r, g)                                                                         \
+    __device__ __noinline__ void mixed_variant_##id(const MixedBin& b, int begin, int end) {            \
+        FitArgs a = b.a;                                                                                 \
+        a.n = end;                                                                                       \
+        const auto model = ModelFor<m>::make(b);                                                         \
+        fit_spots<typename ModelFor<m>::type, r, g>(model, a, begin + (int)(threadIdx.x & (kWave - 1)) / g, \
+                                                    kWave / g);                                          \
+    }
+PPA_MIXED_VARIANTS(PPA_MV_FUNC)
+#undef PPA_MV_FUNC
+
 __global__ void __launch_bounds__(256) mixed_kernel(const MixedBin* __restrict__ bins,
                                                     const MixedTile* __restrict__ tiles, int ntiles, int* counter) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -75,15 +88,9 @@ __global__ void __launch_bounds__(256) mixed_kernel(const MixedBin* __restrict__
         if (t >= ntiles) break;
         const MixedTile tile = tiles[t];
         const MixedBin& b = bins[tile.bin];
-        FitArgs a = b.a;
-        a.n = tile.end;
         switch (b.variant) {
-#define PPA_MV_CASE(id, m, r, g)                                                                   \
-    case id: {                                                                                     \
-        const auto model = ModelFor<m>::make(b);                                                   \
-        fit_spots<typename ModelFor<m>::type, r, g>(model, a, tile.begin + lane / g, kWave / g);   \
-        break;                                                                                     \
-    }
+#define PPA_MV_CASE(id, m, r, g) \
+    case id: mixed_variant_##id(b, tile.begin, tile.end); break;
             PPA_MIXED_VARIANTS(PPA_MV_CASE)
 #undef PPA_MV_CASE
         default: break;

@@ -97,44 +97,65 @@ def test_shared_gaussian_gradients(ctx, kind):
     g = shared.grad.cpu().numpy().astype(np.float64)
     base = shared.detach().cpu().numpy().astype(np.float64)
     R = MD.ROI[kind]
-    for i in range(len(base)):
-        h = 1e-3 * max(abs(base[i]), 1e-3) if kind == "astig" else 1e-3
-        if kind == "astig" and i in (4, 9):
-            h = 1e-3 * 1e-6  # B is ~3e-7
+    def nll_sum_diff(i, h):
         sp, sm = base.copy(), base.copy()
         sp[i] += h
         sm[i] -= h
-        fd = (_nll64(_oracle_with_shared(kind, R, sp), th, smp).sum() -
-              _nll64(_oracle_with_shared(kind, R, sm), th, smp).sum()) / (2 * h)
+        # per-spot differences first: the total NLL is large and would swamp them
+        return np.sum(_nll64(_oracle_with_shared(kind, R, sp), th, smp) -
+                      _nll64(_oracle_with_shared(kind, R, sm), th, smp))
+
+    for i in range(len(base)):
+        h = 1e-3 * max(abs(base[i]), 1e-3)
+        if kind == "astig" and i in (4, 9):
+            h = 1e-9  # B is ~3e-7
+        # 4-point central stencil (truncation O(h^4))
+        fd = (8 * nll_sum_diff(i, h) - nll_sum_diff(i, 2 * h)) / (12 * h)
         assert abs(g[i] - fd) <= 1e-4 * max(abs(fd), 1e-3 * np.abs(g).max()), (i, g[i], fd)
 
 
 def test_spline_coefficient_gradients(ctx):
     import torch
+    from tests import _spline_np as SN
     th, smp, est, om, d_th, d_smp = _setup(ctx, "spline", n=32)
     coefs, zmin, zmax = MD.spline_coefs()
-    shared = torch.from_numpy(coefs.ravel().copy()).cuda().requires_grad_(True)
-    nll = poisson_nll(est, d_th, d_smp, shared)
-    (nll * torch.linspace(0.5, 1.5, len(nll), device="cuda")).sum().backward()
-    g = shared.grad.cpu().numpy().reshape(coefs.shape).astype(np.float64)
-    wts = np.linspace(0.5, 1.5, len(th))
-    touched = np.argwhere(np.abs(g) > 1e-3 * np.abs(g).max())
-    rng = np.random.default_rng(0)
-    pick = touched[rng.choice(len(touched), 24, replace=False)]
     R = MD.ROI["spline"]
-    for idx in pick:
-        h = 1e-3
+    shared = torch.from_numpy(coefs.ravel().copy()).cuda().requires_grad_(True)
+    wts = np.linspace(0.5, 1.5, len(th))
+    nll = poisson_nll(est, d_th, d_smp, shared)
+    (nll * torch.from_numpy(wts.astype(np.float32)).cuda()).sum().backward()
+    g = shared.grad.cpu().numpy().reshape(-1, 64).astype(np.float64)
+    # analytic: d NLL / d c[cell, q] = sum over pixels in the cell of w (1 - x/mu) I basis_q, psf > 0
+    cell, B = SN.cells(th, R, coefs.shape[:3], zmin, zmax)
+    C2 = coefs.reshape(-1, 64).astype(np.float64)
+    f = np.einsum("nyxq,nq->nyx", C2[cell], B)
+    ev = om.expected_value(th, np.float64)
+    w = 1 - np.maximum(smp, 1e-8) / np.maximum(ev, 1e-8)
+    contrib = (wts[:, None, None] * w * th[:, 3, None, None] * (f > 0))[..., None] * B[:, None, None, :]
+    ana = np.zeros_like(C2)
+    np.add.at(ana, cell.ravel(), contrib.reshape(-1, 64))
+    assert np.max(np.abs(g - ana)) <= 1e-4 * np.abs(ana).max()
+    touched = np.argwhere(np.abs(ana) > 1e-3 * np.abs(ana).max())
+    rel = np.abs(g - ana)[tuple(touched.T)] / np.abs(ana)[tuple(touched.T)]
+    assert np.max(rel) < 1e-4, np.max(rel)
+    assert np.count_nonzero(g) < g.size  # untouched cells get no gradient
+    # finite differences at eps = 1e-3 on coefficients of cells whose pixels stay away from
+    # the psf >= 0 clamp within the perturbation (mu is piecewise linear in a coefficient)
+    kink = np.zeros(C2.shape[0], bool)
+    near = np.abs(f) < 1e-3 * np.abs(B).max(axis=1)[:, None, None] * 2
+    kink[cell[near]] = True
+    rng = np.random.default_rng(0)
+    cand = [tuple(t) for t in touched if not kink[t[0]]]
+    pick = [cand[i] for i in rng.choice(len(cand), min(24, len(cand)), replace=False)]
+    for (c, q) in pick:
+        idx = np.unravel_index(c, coefs.shape[:3]) + (q,)
         cp, cm = coefs.copy(), coefs.copy()
-        cp[tuple(idx)] += h
-        cm[tuple(idx)] -= h
-        hp = np.float64(cp[tuple(idx)]) - np.float64(coefs[tuple(idx)])
-        hm = np.float64(coefs[tuple(idx)]) - np.float64(cm[tuple(idx)])
-        fp = (wts * _nll64(O.spline(R, cp, zmin, zmax), th, smp)).sum()
-        fm = (wts * _nll64(O.spline(R, cm, zmin, zmax), th, smp)).sum()
-        fd = (fp - fm) / (hp + hm)
-        assert abs(g[tuple(idx)] - fd) <= 1e-4 * abs(fd) + 1e-6 * np.abs(g).max(), (idx, g[tuple(idx)], fd)
-    # untouched cells get no gradient
-    assert np.count_nonzero(g) < g.size
+        cp[idx] += 1e-3
+        cm[idx] -= 1e-3
+        h2 = np.float64(cp[idx]) - np.float64(cm[idx])
+        d = wts * (_nll64(O.spline(R, cp, zmin, zmax), th, smp) - _nll64(O.spline(R, cm, zmin, zmax), th, smp))
+        fd = d.sum() / h2
+        assert abs(g[c, q] - fd) <= 1e-4 * abs(fd) + 1e-7 * np.abs(g).max(), ((c, q), g[c, q], fd)
 
 
 def test_calibration_step_reduces_nll(ctx):
