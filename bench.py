@@ -10,9 +10,10 @@ Workloads (the default is c2):
   c2  2D Gaussian free sigma, 11x11 px, 1,000,000 spots, 50 iterations + CRLB
   c3  astigmatic 3D Gaussian (quartic sigma(z)), 16x16 px, 2,000,000 spots, 50 iterations + CRLB
   c4  cubic-spline 3D PSF (61x26x26 cells), 20x20 px, 500,000 spots, 50 iterations + CRLB
-Synthetic data: true parameters from numpy PCG64(seed 1) as the workload states,
-expected images from the library's own expected-value kernel, Poisson noise
-from a seeded torch generator on the GPU.
+Synthetic data (photonpy_amd/synthetic.py): true parameters from numpy
+PCG64(seed 1) as the workload states, expected images from the library's own
+expected-value kernel, Poisson noise from a seeded torch generator on the GPU.
+Tuned lane-group sizes are read from photonpy_amd/tuning.json when present.
 """
 import argparse
 import json
@@ -27,17 +28,6 @@ sys.path.insert(0, ROOT)
 
 METRIC = "MLE localizations/s per model (vs CPU + plain GPU); PSF-adjoint grad evals/s"
 PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 vector
-
-CONFIGS = {
-    "c1": dict(kind="fixed", R=10, n=10_000, iters=30, lam=1e-15,
-               workload="2D Gaussian MLE fixed sigma=1.5, 10x10 px, 10,000 spots, 30 iterations + CRLB"),
-    "c2": dict(kind="sigma", R=11, n=1_000_000, iters=50, lam=1e-18,
-               workload="2D Gaussian MLE free sigma, 11x11 px, 1,000,000 spots, 50 iterations + CRLB"),
-    "c3": dict(kind="astig", R=16, n=2_000_000, iters=50, lam=1e-18,
-               workload="astigmatic 3D Gaussian MLE (A=0.1, B=0.05), 16x16 px, 2,000,000 spots, 50 iterations + CRLB"),
-    "c4": dict(kind="spline", R=20, n=500_000, iters=50, lam=1e-18,
-               workload="cubic-spline 3D PSF MLE (61x26x26x64 coefficients), 20x20 px, 500,000 spots, 50 iterations + CRLB"),
-}
 
 # ---------------------------------------------------------------------------
 # Algorithmic FLOP count per localization (FP32 model + normal equations, FP64
@@ -65,59 +55,7 @@ def hbm_bytes_per_loc(kind, R):
     return 4 * R * R + 4 * (2 * K + 2)  # ROI in; params, CRLB, chi^2, iterations out
 
 
-# ---------------------------------------------------------------------------
-def make_estimator(ctx, cfg):
-    import photonpy_amd as pa
-    kind, R = cfg["kind"], cfg["R"]
-    g = pa.GaussianPSFMethods(ctx)
-    if kind == "fixed":
-        return g.CreatePSF_XYIBg(R, 1.5, True)
-    if kind == "sigma":
-        return g.CreatePSF_XYIBgSigma(R, 1.6, True)
-    if kind == "astig":
-        cal = pa.Gauss3D_Calibration.from_astig_params(1.3, 200.0, 400.0, 0.1, 0.05, [-600.0, 600.0])
-        return g.CreatePSF_XYZIBg(R, cal, True)
-    if kind == "spline":
-        coefs, zmin, zmax = spline_table(ctx)
-        m = pa.CSplineMethods(ctx)
-        return m.CreatePSF_XYZIBg(R, pa.CSplineCalibration(zmin, zmax, coefs), m.FlatBg, True)
-    raise ValueError(kind)
-
-
-def spline_table(ctx):
-    """61x26x26x64 spline of the astigmatic PSF rendered at 13.3 nm z steps (GPU-computed)."""
-    import photonpy_amd as pa
-    import torch
-    cal = pa.Gauss3D_Calibration.from_astig_params(1.3, 200.0, 400.0, 0.1, 0.05, [-600.0, 600.0])
-    ast = pa.GaussianPSFMethods(ctx).CreatePSF_XYZIBg(29, cal, True)
-    zs = -418.95 + 13.3 * np.arange(64)
-    th = np.zeros((64, 5), np.float32)
-    th[:, 0] = th[:, 1] = 14.0
-    th[:, 2] = zs
-    th[:, 3] = 1.0
-    stack = ast.ExpectedValue(th)
-    coefs = pa.CSplineMethods(ctx).ComputeSplineCoeff(stack)
-    return coefs, float(zs[1]), float(zs[1] + 61 * 13.3)
-
-
-def true_params(cfg, n, seed=1):
-    rng = np.random.default_rng(seed)
-    R = cfg["R"]
-    c = (R - 1) / 2
-    x = c + rng.uniform(-1.5, 1.5, n)
-    y = c + rng.uniform(-1.5, 1.5, n)
-    kind = cfg["kind"]
-    if kind == "fixed":
-        cols = [x, y, rng.uniform(500, 5000, n), rng.uniform(1, 20, n)]
-    elif kind == "sigma":
-        cols = [x, y, np.exp(rng.uniform(np.log(200), np.log(20000), n)), rng.uniform(0.5, 30, n),
-                rng.uniform(1.2, 2.0, n)]
-    else:
-        cols = [x, y, rng.uniform(-400, 400, n), rng.uniform(500, 8000, n), rng.uniform(2, 20, n)]
-    return np.stack(cols, 1).astype(np.float32)
-
-
-def cpu_baseline(cfg, smp_host, target_s=12.0):
+def cpu_baseline(cfg, smp_host, est, target_s=12.0):
     """The oracle (FP32 restatement of the reference estimator) on host cores, over a
     bounded prefix of the same ROIs: Estimate + CRLB + chi^2 per spot."""
     from oracle import oracle as O
@@ -130,7 +68,9 @@ def cpu_baseline(cfg, smp_host, target_s=12.0):
         om = O.astig(R, [1.3, 200.0, 400.0, 0.1 / 400], [1.3, -200.0, 400.0, 0.1 / 400], [-600.0, 600.0],
                      B=(0.05 / 400 ** 2, 0.05 / 400 ** 2))
     else:
-        return None
+        coefs = est.shared_parameters("cpu").numpy().reshape(61, 26, 26, 64)
+        lo, hi = est.ParamLimits()
+        om = O.spline(R, coefs, float(lo[2]), float(hi[2]))
     threads = min(16, os.cpu_count() or 1)
 
     def run(m):
@@ -163,14 +103,62 @@ def host_api_rate(est, smp_host):
     return m / (time.perf_counter() - t1)
 
 
+def bench_adjoint(args):
+    """Spline PSF calibration step: forward Poisson NLL + hand-written adjoint on 65,536
+    20x20 ROIs, gradients w.r.t. per-spot parameters and all 2,638,976 spline coefficients
+    (mu and J recomputed in the backward pass, nothing stored)."""
+    import torch
+    import photonpy_amd as pa
+    from photonpy_amd import synthetic
+    from photonpy_amd.autograd import poisson_nll
+    n = args.n or 65_536
+    with pa.Context() as ctx:
+        est = synthetic.make_estimator(ctx, "spline", 20)
+        th, smp = synthetic.make_samples(est, "spline", n, seed=1)
+        params = torch.from_numpy(th).cuda().requires_grad_(True)
+        coefs = est.shared_parameters().requires_grad_(True)
+        stream = torch.cuda.current_stream()
+
+        def step():
+            params.grad = None
+            coefs.grad = None
+            loss = poisson_nll(est, params, smp, coefs).sum()
+            loss.backward()
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            ev0[k].record(stream)
+            step()
+            ev1[k].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        gpu_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev0, ev1)]))
+    ms = wall * 1e3 / args.steps
+    atomics = n * 400 * 64
+    print(json.dumps({
+        "metric": METRIC, "value": n / (ms * 1e-3), "unit": "ROI gradient evaluations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "cubic-spline PSF adjoint: forward NLL + backward w.r.t. params and 61x26x26x64 "
+                               "coefficients, 20x20 px, 65,536 ROIs", "roi": 20, "spots": n},
+        "detail": {"gpu_ms_per_step": gpu_ms, "coefficient_atomics_per_step": atomics,
+                   "atomic_adds_per_s": atomics / (gpu_ms * 1e-3)}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c4adj"])
     ap.add_argument("--n", type=int, default=0, help="override the number of spots")
     ap.add_argument("--group", type=int, default=0, help="lane-group size override (0 = tuned/default)")
+    ap.add_argument("--refill", type=int, default=0, help="refill batch override (0 = tuned/default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-e2e", action="store_true", help="skip the PCIe-inclusive host-API leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -194,21 +182,22 @@ def main():
         print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 5}))
         return
 
-    cfg = CONFIGS[args.config]
+    from photonpy_amd import autotune, synthetic
+
+    if args.config == "c4adj":
+        return bench_adjoint(args)
+    cfg = synthetic.WORKLOADS[args.config]
     n = args.n or cfg["n"]
     torch.cuda.set_device(0)
-    dev = torch.device("cuda:0")
     with pa.Context() as ctx:
-        est = make_estimator(ctx, cfg)
+        est = synthetic.make_estimator(ctx, cfg["kind"], cfg["R"])
         est.SetLevMarParams(cfg["lam"], cfg["iters"])
+        autotune.apply(est)
         if args.group:
             est.SetGroupSize(args.group)
-        th = true_params(cfg, n)
-        d_th = torch.from_numpy(th).to(dev)
-        mu = est.expected_value_device(d_th)
-        gen = torch.Generator(device=dev).manual_seed(1)
-        smp = torch.poisson(mu, generator=gen)
-        del mu
+        if args.refill:
+            est.SetRefill(args.refill)
+        th, smp = synthetic.make_samples(est, cfg["kind"], n, seed=1)
         torch.cuda.synchronize()
 
         for _ in range(args.warmup):
@@ -241,7 +230,7 @@ def main():
         smp_host = smp[:m].cpu().numpy()
         e2e = None if args.no_e2e else host_api_rate(est, smp_host)
 
-        cpu = None if args.no_cpu else cpu_baseline(cfg, smp.cpu().numpy())
+        cpu = None if args.no_cpu else cpu_baseline(cfg, smp.cpu().numpy(), est)
         group = est.GetGroupSize()
 
     value = n / (ms_per_step * 1e-3)

@@ -129,6 +129,9 @@ Estimator* Gauss2D_CreatePSF_XYZIBgQ(int roisize, const Gauss3D_CalibrationQ* ca
  * for (roisize, group). */
 int Estimator_SetGroupSize(Estimator* e, int group);
 int Estimator_GetGroupSize(Estimator* e);
+/* Number of stopped lane groups of a wavefront that are refilled together
+ * (0 = one eighth of the wavefront's groups, at least 1). */
+int Estimator_SetRefill(Estimator* e, int refill);
 
 /* Fused fit on device memory: Levenberg-Marquardt from the centre-of-mass start
  * (or d_initial), then CRLB and chi^2 when d_crlb / d_chisq are non-NULL.

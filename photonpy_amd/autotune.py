@@ -21,7 +21,7 @@ TABLE = os.path.join(_HERE, "tuning.json")
 
 _KIND = {"x,y,I,bg": "fixed", "x,y,\tI,bg,sigma": "sigma"}
 # group sizes compiled for each ROI size (csrc/fit_sizes.h)
-COMPILED_GROUPS = {3: [1], 4: [1], 5: [1], 6: [1], 7: [1, 2], 8: [2], 9: [2, 4], 10: [2, 4], 11: [2, 4],
+COMPILED_GROUPS = {3: [1], 4: [1], 5: [2], 6: [2], 7: [1, 2, 4], 8: [2], 9: [2, 4], 10: [2, 4], 11: [2, 4],
                    12: [4], 13: [4, 8], 14: [4], 15: [4], 16: [4, 8], 17: [8], 18: [8], 19: [8], 20: [8, 16],
                    21: [8], 22: [8], 23: [8], 24: [8], 25: [8, 16], 26: [16], 27: [16], 28: [16], 29: [16],
                    30: [16], 31: [16], 32: [16]}
@@ -52,9 +52,11 @@ def lookup_spots_per_group(est, default=8):
 
 def apply(est):
     """Set the estimator's fit-kernel group size from the tuning table (if tuned)."""
-    g = load().get(key(est), {}).get("group")
-    if g:
-        est.SetGroupSize(int(g))
+    entry = load().get(key(est), {})
+    if entry.get("group"):
+        est.SetGroupSize(int(entry["group"]))
+    if entry.get("refill") is not None:
+        est.SetRefill(int(entry["refill"]))
     return est.GetGroupSize()
 
 
@@ -71,20 +73,26 @@ def _time(fn, reps=3):
     return best
 
 
-def tune_group(est, samples, persist=True):
-    """Time every compiled group size for this estimator's ROI size on device samples [N,R,R]."""
+def tune_group(est, samples, refills=(0, 1, 2, 4, 8), persist=True):
+    """Time every compiled group size x refill batch for this estimator's ROI size on
+    device samples [N,R,R]; keep the fastest pair."""
     R = int(est.sampleshape[0])
     results = {}
     for g in COMPILED_GROUPS.get(R, []):
         est.SetGroupSize(g)
-        results[g] = _time(lambda: est.estimate_device(samples, crlb=True))
+        for rf in refills:
+            if rf > 64 // g:
+                continue
+            est.SetRefill(rf)
+            results[(g, rf)] = _time(lambda: est.estimate_device(samples, crlb=True))
     best = min(results, key=results.get)
-    est.SetGroupSize(best)
+    est.SetGroupSize(best[0])
+    est.SetRefill(best[1])
     if persist:
         t = load()
         e = t.setdefault(key(est), {})
-        e["group"] = int(best)
-        e["group_loc_per_s"] = {str(g): samples.shape[0] / v for g, v in results.items()}
+        e["group"], e["refill"] = int(best[0]), int(best[1])
+        e["loc_per_s"] = {f"G{g}_refill{rf}": samples.shape[0] / v for (g, rf), v in results.items()}
         save(t)
     return best, results
 
@@ -105,3 +113,30 @@ def tune_batch(estimators, samples, candidates=(2, 4, 8, 16, 32), persist=True):
             t.setdefault(key(e), {})["spots_per_group"] = int(best)
         save(t)
     return best, results
+
+
+def main():
+    """python -m photonpy_amd.autotune [--configs c1,c2,c3,c4] [--n 200000]"""
+    import argparse
+    import torch
+    from . import Context
+    from . import synthetic
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--configs", default="c1,c2,c3,c4")
+    ap.add_argument("--n", type=int, default=200_000)
+    args = ap.parse_args()
+    with Context() as ctx:
+        for c in args.configs.split(","):
+            w = synthetic.WORKLOADS[c]
+            est = synthetic.make_estimator(ctx, w["kind"], w["R"])
+            est.SetLevMarParams(w["lam"], w["iters"])
+            _, smp = synthetic.make_samples(est, w["kind"], min(args.n, w["n"]) if c != "c1" else w["n"])
+            best, res = tune_group(est, smp)
+            print(c, key(est), "best", best, {f"{g}/{r}": round(smp.shape[0] / v / 1e6, 2) for (g, r), v in res.items()},
+                  flush=True)
+            del smp
+            torch.cuda.empty_cache()
+
+
+if __name__ == "__main__":
+    main()

@@ -21,7 +21,7 @@ inline int num_params(int kind) { return kind == kKindFixed ? 4 : 5; }
 // Lane-group size used for an R x R ROI unless the tuning table says otherwise:
 // the smallest power of two that keeps <= ~64 samples per lane in registers.
 inline int default_group(int R) {
-    if (R <= 7) return 1;
+    if (R <= 4) return 1;
     if (R <= 11) return 2;
     if (R <= 16) return 4;
     if (R <= 24) return 8;

@@ -1,4 +1,4 @@
-// Batched Levenberg-Marquardt MLE fit with fused CRLB / chi-square.
+// Batched Levenberg-Marquardt MLE fit, followed by CRLB / chi-square.
 //
 // Mapping (gfx950, wave64): a spot is fitted by an aligned group of G lanes of
 // one wavefront (G = 1..16, a power of two chosen per ROI size).  Lane g of a
@@ -8,13 +8,19 @@
 // alpha (K(K+1)/2) and beta (K) in FP32, the group all-reduces them with lane
 // shuffles, and every lane solves the FP64 damped Cholesky system redundantly
 // (identical inputs give identical steps, so the group stays convergent).
-// When a spot stops, its group writes params / iterations (and CRLB + chi^2
-// from the same registers) and refills with the next spot of a static stride,
-// so a wavefront keeps all its lanes busy until the batch is exhausted.
+//
+// Spots stop after different numbers of iterations.  A group whose spot has
+// stopped waits until `refill` groups of its wavefront (or every still-running
+// one) have stopped; those groups then write their results and load their next
+// spot of a static stride together, so the wavefront pays the partly-masked
+// refill once per batch of groups instead of once per spot.  The CRLB and
+// chi^2 are computed afterwards in a second phase that walks the same spots in
+// lockstep (every group does exactly one spot per pass), reading the samples
+// and the fitted parameters back from HBM.
 //
 // Semantics follow LevMarOptimize (Estimation.h:194-251): trace before the
 // step, damping, Cholesky break on a zero pivot, clamp, stop when i > 0 and
-// every |delta| <= StopLimit, iterations = i + 1.
+// every |delta| <= StopLimit, iterations = i + 1 (max + 1 when it never stops).
 #pragma once
 
 #include "models.h"
@@ -22,17 +28,18 @@
 namespace ppa {
 
 struct FitArgs {
-    int n;                  // number of spots
-    const float* sample;    // [n, R*R]
-    const float* initial;   // [n, K] or null (centre-of-mass start)
-    float* params;          // [n, K]
-    int* iterations;        // [n] or null
-    float* trace;           // [n, traceLen, K] or null
+    int n;                  // spots [0, n) are fitted (the persistent batch passes a tile's end)
+    const float* sample;    // [*, R*R]
+    const float* initial;   // [*, K] or null (centre-of-mass start)
+    float* params;          // [*, K]
+    int* iterations;        // [*] or null
+    float* trace;           // [*, traceLen, K] or null
     int traceLen;
-    float* crlb;            // [n, K] or null: skip the CRLB pass
-    float* chisq;           // [n] or null
+    float* crlb;            // [*, K] or null
+    float* chisq;           // [*] or null
     int maxIter;
     float lambda;
+    int refill;             // stopped groups per batched refill (<= 0: one eighth of the wave's groups)
 };
 
 template <class M, int R, int G>
@@ -43,39 +50,83 @@ struct FitTraits {
     static constexpr int NA = K * (K + 1) / 2;   // packed alpha
 };
 
+// Loads rows [y0, y0+RL) of spot sp into s and returns the lane's centre-of-mass partial sums.
+template <int R, int RL>
+PP_DEV void load_rows(const float* src, int y0, float (&s)[RL][R], float& sx, float& sy, float& sum) {
+    sx = sy = sum = 0.0f;
+#pragma unroll
+    for (int r = 0; r < RL; r++) {
+        const int y = y0 + r;
+#pragma unroll
+        for (int x = 0; x < R; x++) {
+            const float v = (y < R) ? __builtin_nontemporal_load(src + y * R + x) : 0.0f;
+            s[r][x] = v;
+            sx += (float)x * v;
+            sy += (float)y * v;
+            sum += v;
+        }
+    }
+}
+
+// Fisher matrix (packed) and chi^2 of one spot, written by lane g == 0 of its group.
+template <class M, int R, int G>
+PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTraits<M, R, G>::RL][R],
+                      const float (&th)[M::K], int y0, int spot, bool lead) {
+    using T = FitTraits<M, R, G>;
+    constexpr int K = T::K, NA = T::NA;
+    float fi[NA + 1];
+#pragma unroll
+    for (int i = 0; i < NA + 1; i++) fi[i] = 0.0f;
+    model.template for_rows<R, T::RL>(th, y0, [&](int r, int x, float mu, const float (&J)[K]) {
+        const float m = fmaxf(1e-8f, mu);
+        const float inv = rcpf(m);
+        const float e = s[r][x] - m;
+        fi[NA] += e * e * inv;
+        int q = 0;
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+            const float w = J[i] * inv;
+#pragma unroll
+            for (int j = i; j < K; j++) fi[q++] += w * J[j];
+        }
+    });
+    group_sum_n<G>(fi);
+    if (lead) {
+        if (a.crlb) {
+            float c[K];
+            crlb_from_fisher<K>(fi, c);
+            float* cp = a.crlb + (size_t)spot * K;
+#pragma unroll
+            for (int k = 0; k < K; k++) cp[k] = c[k];
+        }
+        if (a.chisq) a.chisq[spot] = fi[NA];
+    }
+}
+
 // Fits spots first, first+stride, ... < a.n with the lane groups of the calling
 // wavefront; g_first is this lane group's first spot.  Every lane of the
-// wavefront must call it (the loop runs until all groups of the wave are done).
+// wavefront must call it (the loops run until all groups of the wave are done).
 template <class M, int R, int G>
 __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int g_first, int stride) {
     using T = FitTraits<M, R, G>;
     constexpr int K = T::K, RL = T::RL, NA = T::NA;
+    constexpr int P = kWave / G;  // groups per wavefront
     static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "G must be a power of two <= 64");
 
     const int lane = threadIdx.x & (kWave - 1);
     const int g = lane & (G - 1);
+    const bool lead = (g == 0);
     const int y0 = g * RL;
-    int spot = g_first;
+    const int refill = a.refill > 0 ? a.refill : (P >= 8 ? P / 8 : 1);
 
     float s[RL][R];
     float th[K];
     int iter = 0;
+    int spot = g_first;
 
-    auto load_and_init = [&](int sp) {
-        const float* src = a.sample + (size_t)sp * T::S;
-        float sx = 0.0f, sy = 0.0f, sum = 0.0f;
-#pragma unroll
-        for (int r = 0; r < RL; r++) {
-            const int y = y0 + r;
-#pragma unroll
-            for (int x = 0; x < R; x++) {
-                float v = (y < R) ? __builtin_nontemporal_load(src + y * R + x) : 0.0f;
-                s[r][x] = v;
-                sx += (float)x * v;
-                sy += (float)y * v;
-                sum += v;
-            }
-        }
+    auto start_spot = [&](int sp) {
+        float sx, sy, sum;
+        load_rows<R, RL>(a.sample + (size_t)sp * T::S, y0, s, sx, sy, sum);
         if (a.initial) {
 #pragma unroll
             for (int k = 0; k < K; k++) th[k] = a.initial[(size_t)sp * K + k];
@@ -88,101 +139,99 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
         iter = 0;
     };
 
-    bool active = spot < a.n;
-    if (active) load_and_init(spot);
-
-    while (__any(active)) {
-        if (active) {
-            bool done = false;
-            if (iter >= a.maxIter) {
-                // loop exhausted without a stop: the reference reports i + 1 = max + 1
-                done = true;
-                iter = a.maxIter + 1;
-            } else {
-                if (a.trace && iter < a.traceLen && g == 0) {
-                    float* tp = a.trace + ((size_t)spot * a.traceLen + iter) * K;
-#pragma unroll
-                    for (int k = 0; k < K; k++) tp[k] = th[k];
-                }
-                float acc[NA + K];
-#pragma unroll
-                for (int i = 0; i < NA + K; i++) acc[i] = 0.0f;
-                model.template for_rows<R, RL>(th, y0, [&](int r, int x, float mu, const float (&J)[K]) {
-                    const float xv = fmaxf(s[r][x], 1e-6f);
-                    const float inv = rcpf(fmaxf(mu, 1e-6f));
-                    const float t = xv * inv;
-                    const float xf2 = t * inv;
-                    const float bf = t - 1.0f;
-                    int q = 0;
-#pragma unroll
-                    for (int i = 0; i < K; i++) {
-                        const float w = J[i] * xf2;
-#pragma unroll
-                        for (int j = i; j < K; j++) acc[q++] += w * J[j];
-                    }
-#pragma unroll
-                    for (int i = 0; i < K; i++) acc[NA + i] += bf * J[i];
-                });
-                group_sum_n<G>(acc);
-
-                double step[K];
-                if (!lm_step_f64<K>(acc, acc + NA, a.lambda, step)) {
-                    done = true;  // zero pivot: the reference breaks before updating
-                } else {
-                    float last[K];
-#pragma unroll
-                    for (int k = 0; k < K; k++) {
-                        last[k] = th[k];
-                        th[k] += (float)step[k];
-                    }
-                    model.check_limits(th, R);
-                    bool small = true;
-#pragma unroll
-                    for (int k = 0; k < K; k++)
-                        if (fabsf(last[k] - th[k]) > M::stop_limit(k)) small = false;
-                    done = (iter > 0 && small);
-                }
-                iter++;
-            }
-            if (done) {
-                float crlb[K], chi = 0.0f;
-                if (a.crlb || a.chisq) {
-                    float fi[NA + 1];
-#pragma unroll
-                    for (int i = 0; i < NA + 1; i++) fi[i] = 0.0f;
-                    model.template for_rows<R, RL>(th, y0, [&](int r, int x, float mu, const float (&J)[K]) {
-                        const float m = fmaxf(1e-8f, mu);
-                        const float inv = rcpf(m);
-                        const float e = s[r][x] - m;
-                        fi[NA] += e * e * inv;
-                        int q = 0;
-#pragma unroll
-                        for (int i = 0; i < K; i++) {
-                            const float w = J[i] * inv;
-#pragma unroll
-                            for (int j = i; j < K; j++) fi[q++] += w * J[j];
-                        }
-                    });
-                    group_sum_n<G>(fi);
-                    chi = fi[NA];
-                    if (a.crlb) crlb_from_fisher<K>(fi, crlb);
-                }
-                if (g == 0) {
+    // phase 1: fit.  state 0 = iterating, 1 = stopped (results not yet written), 2 = no more spots
+    int state = 2;
+    if (spot < a.n) {
+        start_spot(spot);
+        state = 0;
+    }
+    for (;;) {
+        const unsigned long long stopped = __ballot(lead && state == 1);
+        const unsigned long long running = __ballot(lead && state == 0);
+        if (!stopped && !running) break;
+        if (stopped && (!running || __popcll(stopped) >= (unsigned)refill)) {
+            if (state == 1) {
+                if (lead) {
                     float* pp = a.params + (size_t)spot * K;
 #pragma unroll
                     for (int k = 0; k < K; k++) pp[k] = th[k];
                     if (a.iterations) a.iterations[spot] = iter;
-                    if (a.crlb) {
-                        float* cp = a.crlb + (size_t)spot * K;
-#pragma unroll
-                        for (int k = 0; k < K; k++) cp[k] = crlb[k];
-                    }
-                    if (a.chisq) a.chisq[spot] = chi;
                 }
                 spot += stride;
-                active = spot < a.n;
-                if (active) load_and_init(spot);
+                if (spot < a.n) {
+                    start_spot(spot);
+                    state = 0;
+                } else {
+                    state = 2;
+                }
             }
+            continue;
+        }
+        if (state != 0) continue;
+        if (iter >= a.maxIter) {
+            // loop exhausted without a stop: the reference reports i + 1 = max + 1
+            iter = a.maxIter + 1;
+            state = 1;
+            continue;
+        }
+        if (a.trace && iter < a.traceLen && lead) {
+            float* tp = a.trace + ((size_t)spot * a.traceLen + iter) * K;
+#pragma unroll
+            for (int k = 0; k < K; k++) tp[k] = th[k];
+        }
+        float acc[NA + K];
+#pragma unroll
+        for (int i = 0; i < NA + K; i++) acc[i] = 0.0f;
+        model.template for_rows<R, RL>(th, y0, [&](int r, int x, float mu, const float (&J)[K]) {
+            const float xv = fmaxf(s[r][x], 1e-6f);
+            const float inv = rcpf(fmaxf(mu, 1e-6f));
+            const float t = xv * inv;
+            const float xf2 = t * inv;
+            const float bf = t - 1.0f;
+            int q = 0;
+#pragma unroll
+            for (int i = 0; i < K; i++) {
+                const float w = J[i] * xf2;
+#pragma unroll
+                for (int j = i; j < K; j++) acc[q++] += w * J[j];
+            }
+#pragma unroll
+            for (int i = 0; i < K; i++) acc[NA + i] += bf * J[i];
+        });
+        group_sum_n<G>(acc);
+
+        double step[K];
+        bool stop;
+        if (!lm_step_f64<K>(acc, acc + NA, a.lambda, step)) {
+            stop = true;  // zero pivot: the reference breaks before updating
+        } else {
+            float last[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                last[k] = th[k];
+                th[k] += (float)step[k];
+            }
+            model.check_limits(th, R);
+            bool small = true;
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                if (fabsf(last[k] - th[k]) > M::stop_limit(k)) small = false;
+            stop = (iter > 0 && small);
+        }
+        iter++;
+        if (stop) state = 1;
+    }
+
+    // phase 2: CRLB and chi^2 of the same spots, one spot per group per pass
+    if (!a.crlb && !a.chisq) return;
+    __threadfence();  // this wavefront's parameter stores are read back below
+    for (int sp = g_first; __any(sp < a.n); sp += stride) {
+        if (sp < a.n) {
+            float sx, sy, sum;
+            load_rows<R, RL>(a.sample + (size_t)sp * T::S, y0, s, sx, sy, sum);
+#pragma unroll
+            for (int k = 0; k < K; k++) th[k] = a.params[(size_t)sp * K + k];
+            crlb_spot<M, R, G>(model, a, s, th, y0, sp, lead);
         }
     }
 }

@@ -25,13 +25,13 @@ namespace ppa {
 // (id, model, R, G) of the variants compiled into the persistent kernel: the
 // batch workload's ROI sizes {7,9,11,13,16,20,25} for all four models.
 #define PPA_MIXED_VARIANTS(X)                                                                              \
-    X(0, kFixed, 7, 1) X(1, kFixed, 9, 2) X(2, kFixed, 11, 2) X(3, kFixed, 13, 4) X(4, kFixed, 16, 4)     \
+    X(0, kFixed, 7, 2) X(1, kFixed, 9, 2) X(2, kFixed, 11, 2) X(3, kFixed, 13, 4) X(4, kFixed, 16, 4)     \
     X(5, kFixed, 20, 8) X(6, kFixed, 25, 16)                                                               \
-    X(7, kSigma, 7, 1) X(8, kSigma, 9, 2) X(9, kSigma, 11, 2) X(10, kSigma, 13, 4) X(11, kSigma, 16, 4)   \
+    X(7, kSigma, 7, 2) X(8, kSigma, 9, 2) X(9, kSigma, 11, 2) X(10, kSigma, 13, 4) X(11, kSigma, 16, 4)   \
     X(12, kSigma, 20, 8) X(13, kSigma, 25, 16)                                                             \
-    X(14, kAstig, 7, 1) X(15, kAstig, 9, 2) X(16, kAstig, 11, 2) X(17, kAstig, 13, 4) X(18, kAstig, 16, 4) \
+    X(14, kAstig, 7, 2) X(15, kAstig, 9, 2) X(16, kAstig, 11, 2) X(17, kAstig, 13, 4) X(18, kAstig, 16, 4) \
     X(19, kAstig, 20, 8) X(20, kAstig, 25, 16)                                                             \
-    X(21, kSpline, 7, 1) X(22, kSpline, 9, 2) X(23, kSpline, 11, 2) X(24, kSpline, 13, 4)                  \
+    X(21, kSpline, 7, 2) X(22, kSpline, 9, 2) X(23, kSpline, 11, 2) X(24, kSpline, 13, 4)                  \
     X(25, kSpline, 16, 4) X(26, kSpline, 20, 8) X(27, kSpline, 25, 16)
 
 enum { kSpline = 3 };  // model tag for the variant list (Gaussian tags are GaussVariant)

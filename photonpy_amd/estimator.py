@@ -318,6 +318,11 @@ class Estimator:
         if self._lib.Estimator_SetGroupSize(self.inst, int(group)) != 0:
             raise ValueError(f"invalid group size {group}")
 
+    def SetRefill(self, refill):
+        """Stopped lane groups per batched refill in the fit kernel (0 = default)."""
+        if self._lib.Estimator_SetRefill(self.inst, int(refill)) != 0:
+            raise ValueError(f"invalid refill batch {refill}")
+
     def GetGroupSize(self):
         return self._lib.Estimator_GetGroupSize(self.inst)
 

@@ -101,6 +101,7 @@ def _bind(lib):
         "Estimator_GetLMParams": ([V, ct.POINTER(LevMarParams)], None),
         "Estimator_SetGroupSize": ([V, I], ct.c_int),
         "Estimator_GetGroupSize": ([V], ct.c_int),
+        "Estimator_SetRefill": ([V, I], ct.c_int),
         "Estimator_ComputeExpectedValue": ([V, I, FloatArray, FloatArray, IntArray, FloatArray], None),
         "Estimator_Estimate": ([V, I, FloatArray, FloatArray, IntArray, NullableFloatArrayType, FloatArray,
                                 FloatArray, IntArray, NullableFloatArrayType, I], None),

@@ -192,6 +192,21 @@ def test_group_sizes_agree(ctx, R, G):
     assert _compare("sigma", p2[conv], p1[conv]).all()
 
 
+def test_refill_batches_do_not_change_results(ctx):
+    # a spot's fit does not depend on when its group is refilled
+    th, smp = MD.samples("sigma", 3000, 23)
+    est, _, _ = MD.make("sigma", ctx)
+    outs = []
+    for rf in (1, 4, 32):
+        est.SetRefill(rf)
+        p, _, _ = est.Estimate(smp)
+        c, x = est.CRLBAndChiSquare(p, smp)
+        outs.append((p, est.last_iterations.copy()))
+    est.SetRefill(0)
+    for p, it in outs[1:]:
+        assert np.array_equal(p, outs[0][0], equal_nan=True) and np.array_equal(it, outs[0][1])
+
+
 def test_spline_coefficients_match_oracle(ctx):
     stack, _ = C.spline_zstack(nz=12, ny=15, nx=15)
     c_gpu = pa.CSplineMethods(ctx).ComputeSplineCoeff(stack)
