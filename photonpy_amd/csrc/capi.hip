@@ -61,6 +61,16 @@ struct Estimator {
     size_t wsBytes = 0;
 
     float* devShared = nullptr;  // device copy of the shared PSF parameters (Gaussian models)
+    // Work counters of the fit kernel: one slot per launch, cycling, so launches of this
+    // estimator on different streams do not share a counter unless 64 are in flight.
+    static constexpr int kCounterSlots = 64;
+    int* devCounters = nullptr;
+    unsigned counterSlot = 0;
+
+    int* next_counter() {
+        if (!devCounters) hip_check(hipMalloc(&devCounters, kCounterSlots * sizeof(int)), "hipMalloc(counters)");
+        return devCounters + (counterSlot++ % kCounterSlots);
+    }
 
     int K() const { return num_params(desc.kind); }
 
@@ -96,6 +106,7 @@ struct Estimator {
     ~Estimator() {
         if (devCoefs) (void)hipFree(devCoefs);
         if (devShared) (void)hipFree(devShared);
+        if (devCounters) (void)hipFree(devCounters);
         if (ws) (void)hipFree(ws);
     }
 
@@ -192,6 +203,7 @@ static void run_fit(Estimator* e, int n, const float* d_sample, const float* d_i
     a.maxIter = e->lm.iterations;
     a.lambda = e->lm.lambda;
     a.refill = e->refill;
+    a.counter = e->next_counter();
     hipError_t err = launch_fit_any(e->desc, R, e->G(), a, 0, st);
     if (err == hipErrorInvalidValue)
         throw std::runtime_error("photonpy_amd: no fit kernel compiled for roisize " + std::to_string(R) + ", group " +

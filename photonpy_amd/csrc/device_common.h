@@ -21,11 +21,22 @@ PP_DEV float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // All-reduce a value over the G lanes of an aligned lane group (G power of 2,
 // G <= 64).  Partners combine partial sums commutatively, so every lane of the
-// group ends with bit-identical totals.
+// group ends with bit-identical totals.  Steps inside a row of 16 lanes use DPP
+// moves (quad_perm xor 1 / xor 2, row_half_mirror, row_mirror: each pairs every
+// lane with one in the other half of its block); wider steps use ds_bpermute.
+template <int CTRL>
+PP_DEV float dpp_add(float v) {
+    return v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
 template <int G>
 PP_DEV float group_sum(float v) {
+    if constexpr (G >= 2) v = dpp_add<0xB1>(v);   // quad_perm [1,0,3,2]
+    if constexpr (G >= 4) v = dpp_add<0x4E>(v);   // quad_perm [2,3,0,1]
+    if constexpr (G >= 8) v = dpp_add<0x141>(v);  // row_half_mirror
+    if constexpr (G >= 16) v = dpp_add<0x140>(v); // row_mirror
 #pragma unroll
-    for (int m = 1; m < G; m <<= 1) v += __shfl_xor(v, m, kWave);
+    for (int m = 16; m < G; m <<= 1) v += __shfl_xor(v, m, kWave);
     return v;
 }
 
@@ -37,9 +48,38 @@ PP_DEV void group_sum_n(float (&v)[N]) {
     }
 }
 
+// Value of v held by lane SRC of this lane's aligned group of G lanes.
+// G <= 4 uses a quad_perm DPP move (a full-rate VALU op), larger groups ds_bpermute.
+template <int G, int SRC>
+PP_DEV float group_bcast(float v) {
+    static_assert(SRC >= 0 && SRC < G, "source lane outside the group");
+    if constexpr (G == 1) {
+        return v;
+    } else if constexpr (G == 2) {
+        constexpr int ctrl = SRC | (SRC << 2) | ((2 + SRC) << 4) | ((2 + SRC) << 6);
+        return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, false));
+    } else if constexpr (G == 4) {
+        constexpr int ctrl = SRC * 0x55;
+        return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, false));
+    } else {
+        const int lane = threadIdx.x & (kWave - 1);
+        return __shfl(v, (lane & ~(G - 1)) + SRC, kWave);
+    }
+}
+
 // Packed index of (a,b), a<=b, in the upper triangle of a KxK matrix.
 template <int K>
 constexpr int tri(int a, int b) { return a * K - a * (a - 1) / 2 + (b - a); }
+
+// 1/sqrt(v) in FP64: the hardware estimate refined by two Newton steps
+// y <- y (3 - v y^2) / 2 (each roughly doubles the correct bits).  For v < 0
+// the result is NaN, as 1/sqrt(v) is.
+PP_DEV double rsqrt_f64(double v) {
+    double y = __builtin_amdgcn_rsq(v);
+    y = y * (1.5 - 0.5 * v * y * y);
+    y = y * (1.5 - 0.5 * v * y * y);
+    return y;
+}
 
 // ---------------------------------------------------------------------------
 // Levenberg-Marquardt step in FP64 (reference: Estimation.h:213-228 and
@@ -85,8 +125,7 @@ PP_DEV bool lm_step_f64(const float* alphaTri, const float* beta, float lambda, 
 #pragma unroll
         for (int p = 0; p < k; p++) v -= P[ltri<K>(k, p)] * P[ltri<K>(k, p)];
         if (v == 0.0) return false;
-        const double d = sqrt(v);
-        invd[k] = 1.0 / d;
+        invd[k] = rsqrt_f64(v);
 #pragma unroll
         for (int i = k + 1; i < K; i++) {
             double w = P[ltri<K>(i, k)];

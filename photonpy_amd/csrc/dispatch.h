@@ -54,6 +54,8 @@ struct AdjArgs {
 };
 hipError_t launch_adjoint_any(const ModelDesc& d, const AdjArgs& a, hipStream_t st);
 hipError_t launch_spline_coefs(const int* shape, const float* d_data, float* d_out, int border, hipStream_t st);
+// Sets *counter = value on the stream (a one-thread kernel, so it can also be captured in a graph).
+hipError_t launch_set_counter(int* counter, int value, hipStream_t st);
 
 inline hipError_t launch_fit_any(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st) {
     switch (d.kind) {
@@ -85,7 +87,14 @@ hipError_t launch_fit_t(const M& m, const FitArgs& a, int maxBlocks, hipStream_t
     long cap = maxBlocks > 0 ? maxBlocks : resident;
     if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL((fit_kernel<M, R, G>), dim3((unsigned)blocks), dim3(kBlock), 0, st, m, a);
+    if (a.counter) {
+        // spots [0, groups) are the groups' first spots; the counter hands out the rest
+        const hipError_t e = launch_set_counter(a.counter, (int)(blocks * groupsPerBlock), st);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((fit_kernel<M, R, G>), dim3((unsigned)blocks), dim3(kBlock), 0, st, m, a, 0);
+    if (a.crlb || a.chisq)
+        hipLaunchKernelGGL((fit_kernel<M, R, G>), dim3((unsigned)blocks), dim3(kBlock), 0, st, m, a, 1);
     return hipGetLastError();
 }
 

@@ -112,6 +112,13 @@ __global__ void __launch_bounds__(256) spline_coef_kernel(int s0, int s1, int s2
             }
 }
 
+__global__ void set_counter_kernel(int* counter, int value) { *counter = value; }
+
+hipError_t launch_set_counter(int* counter, int value, hipStream_t st) {
+    hipLaunchKernelGGL(set_counter_kernel, dim3(1), dim3(1), 0, st, counter, value);
+    return hipGetLastError();
+}
+
 hipError_t launch_spline_coefs(const int* shape, const float* d_data, float* d_out, int border, hipStream_t st) {
     const int o0 = shape[0] - 3 + 2 * border, o1 = shape[1] - 3 + 2 * border, o2 = shape[2] - 3 + 2 * border;
     if (o0 <= 0 || o1 <= 0 || o2 <= 0) return hipErrorInvalidValue;

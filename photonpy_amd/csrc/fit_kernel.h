@@ -13,10 +13,12 @@
 // stopped waits until `refill` groups of its wavefront (or every still-running
 // one) have stopped; those groups then write their results and load their next
 // spot of a static stride together, so the wavefront pays the partly-masked
-// refill once per batch of groups instead of once per spot.  The CRLB and
-// chi^2 are computed afterwards in a second phase that walks the same spots in
-// lockstep (every group does exactly one spot per pass), reading the samples
-// and the fitted parameters back from HBM.
+// refill once per batch of groups instead of once per spot.  Next spots come
+// from a work counter in HBM (one atomic per batch), so every group stays busy
+// until the batch is exhausted and the launch ends within about one spot's fit
+// of each other.  The CRLB and chi^2 are computed by a second launch that walks
+// all spots in lockstep (every group does exactly one spot per pass), reading
+// the samples and the fitted parameters back from HBM.
 //
 // Semantics follow LevMarOptimize (Estimation.h:194-251): trace before the
 // step, damping, Cholesky break on a zero pivot, clamp, stop when i > 0 and
@@ -40,6 +42,7 @@ struct FitArgs {
     int maxIter;
     float lambda;
     int refill;             // stopped groups per batched refill (<= 0: one eighth of the wave's groups)
+    int* counter;           // next unclaimed spot (starts at the grid's group count), or null: static stride
 };
 
 template <class M, int R, int G>
@@ -50,8 +53,11 @@ struct FitTraits {
     static constexpr int NA = K * (K + 1) / 2;   // packed alpha
 };
 
-// Loads rows [y0, y0+RL) of spot sp into s and returns the lane's centre-of-mass partial sums.
-template <int R, int RL>
+// Loads rows [y0, y0+RL) of a spot into s and returns the lane's centre-of-mass
+// partial sums.  With CLAMP the stored samples are max(x, 1e-6) as the
+// Levenberg-Marquardt terms use them (`if (x < 1e-6) x = 1e-6`, which keeps a
+// NaN, Estimation.h:114); the sums use the raw values.
+template <int R, int RL, bool CLAMP>
 PP_DEV void load_rows(const float* src, int y0, float (&s)[RL][R], float& sx, float& sy, float& sum) {
     sx = sy = sum = 0.0f;
 #pragma unroll
@@ -60,7 +66,7 @@ PP_DEV void load_rows(const float* src, int y0, float (&s)[RL][R], float& sx, fl
 #pragma unroll
         for (int x = 0; x < R; x++) {
             const float v = (y < R) ? __builtin_nontemporal_load(src + y * R + x) : 0.0f;
-            s[r][x] = v;
+            s[r][x] = (CLAMP && v < 1e-6f) ? 1e-6f : v;
             sx += (float)x * v;
             sy += (float)y * v;
             sum += v;
@@ -77,7 +83,7 @@ PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTrai
     float fi[NA + 1];
 #pragma unroll
     for (int i = 0; i < NA + 1; i++) fi[i] = 0.0f;
-    model.template for_rows<R, T::RL>(th, y0, [&](int r, int x, float mu, const float (&J)[K]) {
+    model.template for_rows<R, T::RL, G>(th, y0, [&](int r, int x, float mu, const float (&J)[K]) {
         const float m = fmaxf(1e-8f, mu);
         const float inv = rcpf(m);
         const float e = s[r][x] - m;
@@ -126,7 +132,7 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
 
     auto start_spot = [&](int sp) {
         float sx, sy, sum;
-        load_rows<R, RL>(a.sample + (size_t)sp * T::S, y0, s, sx, sy, sum);
+        load_rows<R, RL, true>(a.sample + (size_t)sp * T::S, y0, s, sx, sy, sum);
         if (a.initial) {
 #pragma unroll
             for (int k = 0; k < K; k++) th[k] = a.initial[(size_t)sp * K + k];
@@ -150,6 +156,13 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
         const unsigned long long running = __ballot(lead && state == 0);
         if (!stopped && !running) break;
         if (stopped && (!running || __popcll(stopped) >= (unsigned)refill)) {
+            // next spots: claimed together from the work counter (one atomic per batch,
+            // handed out in lane order), or the static stride
+            int base = 0;
+            if (a.counter) {
+                if (lane == 0) base = atomicAdd(a.counter, (int)__popcll(stopped));
+                base = __shfl(base, 0, kWave);
+            }
             if (state == 1) {
                 if (lead) {
                     float* pp = a.params + (size_t)spot * K;
@@ -157,7 +170,12 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
                     for (int k = 0; k < K; k++) pp[k] = th[k];
                     if (a.iterations) a.iterations[spot] = iter;
                 }
-                spot += stride;
+                if (a.counter) {
+                    const int leadLane = lane & ~(G - 1);
+                    spot = base + (int)__popcll(stopped & ((1ull << leadLane) - 1ull));
+                } else {
+                    spot += stride;
+                }
                 if (spot < a.n) {
                     start_spot(spot);
                     state = 0;
@@ -182,8 +200,8 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
         float acc[NA + K];
 #pragma unroll
         for (int i = 0; i < NA + K; i++) acc[i] = 0.0f;
-        model.template for_rows<R, RL>(th, y0, [&](int r, int x, float mu, const float (&J)[K]) {
-            const float xv = fmaxf(s[r][x], 1e-6f);
+        model.template for_rows<R, RL, G>(th, y0, [&](int r, int x, float mu, const float (&J)[K]) {
+            const float xv = s[r][x];  // clamped at load
             const float inv = rcpf(fmaxf(mu, 1e-6f));
             const float t = xv * inv;
             const float xf2 = t * inv;
@@ -222,24 +240,40 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
         if (stop) state = 1;
     }
 
-    // phase 2: CRLB and chi^2 of the same spots, one spot per group per pass
-    if (!a.crlb && !a.chisq) return;
-    __threadfence();  // this wavefront's parameter stores are read back below
+}
+
+// CRLB and chi^2 of spots first, first+stride, ... < a.n from the samples and the
+// fitted parameters in HBM, one spot per lane group per pass (no divergence).
+template <class M, int R, int G>
+__device__ __forceinline__ void crlb_spots(const M& model, const FitArgs& a, int g_first, int stride) {
+    using T = FitTraits<M, R, G>;
+    constexpr int K = T::K, RL = T::RL;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int g = lane & (G - 1);
+    const int y0 = g * RL;
+    float s[RL][R];
+    float th[K];
     for (int sp = g_first; __any(sp < a.n); sp += stride) {
         if (sp < a.n) {
             float sx, sy, sum;
-            load_rows<R, RL>(a.sample + (size_t)sp * T::S, y0, s, sx, sy, sum);
+            load_rows<R, RL, false>(a.sample + (size_t)sp * T::S, y0, s, sx, sy, sum);
 #pragma unroll
             for (int k = 0; k < K; k++) th[k] = a.params[(size_t)sp * K + k];
-            crlb_spot<M, R, G>(model, a, s, th, y0, sp, lead);
+            crlb_spot<M, R, G>(model, a, s, th, y0, sp, g == 0);
         }
     }
 }
 
+// mode 0: fit (phase 1); mode 1: CRLB and chi^2 of the fitted spots (phase 2).
+// The phases are separate launches so that phase 2 sees every wavefront's results.
 template <class M, int R, int G>
-__global__ void __launch_bounds__(256) fit_kernel(M model, FitArgs a) {
+__global__ void __launch_bounds__(256) fit_kernel(M model, FitArgs a, int mode) {
     const int groupsPerBlock = blockDim.x / G;
-    fit_spots<M, R, G>(model, a, blockIdx.x * groupsPerBlock + threadIdx.x / G, gridDim.x * groupsPerBlock);
+    const int first = blockIdx.x * groupsPerBlock + threadIdx.x / G, stride = gridDim.x * groupsPerBlock;
+    if (mode == 0)
+        fit_spots<M, R, G>(model, a, first, stride);
+    else
+        crlb_spots<M, R, G>(model, a, first, stride);
 }
 
 }  // namespace ppa

@@ -17,6 +17,8 @@
 // expressions for a single pixel so both paths give bit-identical mu and J.
 #pragma once
 
+#include <type_traits>
+
 #include "device_common.h"
 
 namespace ppa {
@@ -46,6 +48,21 @@ PP_DEV AxisTerms axis_terms(const Edge& e0, const Edge& e1, float cp, float cs) 
     t.dE = cp * (e0.G - e1.G);
     t.S = (e0.H - e1.H) * cs;
     return t;
+}
+
+// Compile-time loop: calls f(std::integral_constant<int, I>) for I in [B, E).
+template <int B, int E, class F>
+PP_DEV void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// Edge E of a group-shared edge set: held by lane E % G in slot E / G.
+template <int G, int E, int PER>
+PP_DEV Edge fetch_edge(const float (&Ev)[PER], const float (&Gv)[PER], const float (&Hv)[PER]) {
+    return {group_bcast<G, E % G>(Ev[E / G]), group_bcast<G, E % G>(Gv[E / G]), group_bcast<G, E % G>(Hv[E / G])};
 }
 
 // Calibration of the Gaussian models.  ax/ay = {s0, gamma, d, A, B} per axis.
@@ -161,14 +178,17 @@ struct GaussModel {
         pixel(axis_terms(x0, x1, SX.cp, SX.cs), axis_terms(y0, y1, SY.cp, SY.cs), t, mu, J);
     }
 
-    // Fit-kernel evaluation over a block of RL rows starting at y0 of an RxR ROI.
-    // f(r, x, mu, J) is called for every pixel of rows y0+r < R.
-    template <int R, int RL, class F>
+    // Fit-kernel evaluation over a block of RL rows starting at y0 of an RxR ROI,
+    // called by every lane of an aligned group of G lanes.  The group shares the
+    // R+1 column edges: lane g evaluates edges g, g+G, ... and the others receive
+    // them by lane broadcast.  f(r, x, mu, J) is called for every pixel of rows
+    // y0+r < R.
+    template <int R, int RL, int G, class F>
     PP_DEV void for_rows(const float (&t)[K], int y0, F&& f) const {
         AxisShape SX, SY;
         shapes(t, SX, SY);
         AxisTerms col[R];
-        {
+        if constexpr (G == 1) {
             Edge prev = gauss_edge(0.0f, t[0], SX.c2);
 #pragma unroll
             for (int x = 0; x < R; x++) {
@@ -176,6 +196,24 @@ struct GaussModel {
                 col[x] = axis_terms(prev, nx, SX.cp, SX.cs);
                 prev = nx;
             }
+        } else {
+            constexpr int PER = (R + 1 + G - 1) / G;  // edges evaluated per lane
+            const int g = threadIdx.x & (G - 1);
+            float Ev[PER], Gv[PER], Hv[PER];
+#pragma unroll
+            for (int j = 0; j < PER; j++) {
+                const Edge ed = gauss_edge((float)(g + j * G), t[0], SX.c2);
+                Ev[j] = ed.E;
+                Gv[j] = ed.G;
+                Hv[j] = ed.H;
+            }
+            Edge prev = fetch_edge<G, 0>(Ev, Gv, Hv);
+            static_for<1, R + 1>([&](auto e) {
+                constexpr int E = decltype(e)::value;
+                const Edge nx = fetch_edge<G, E>(Ev, Gv, Hv);
+                col[E - 1] = axis_terms(prev, nx, SX.cp, SX.cs);
+                prev = nx;
+            });
         }
         Edge prev = gauss_edge((float)y0, t[1], SY.c2);
 #pragma unroll
@@ -303,7 +341,7 @@ struct SplineModel {
         finish(F, t, f, fx, fy, fz, mu, J);
     }
 
-    template <int R, int RL, class Fn>
+    template <int R, int RL, int G, class Fn>
     PP_DEV void for_rows(const float (&t)[K], int y0, Fn&& fn) const {
         Frame F;
         frame(t, F);
