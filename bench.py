@@ -35,19 +35,45 @@ PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 vector
 #   per pixel-iteration   K^2 + 4K + 4 (normal equations) + model terms
 #   per row/column-iter   row/column terms of the separable Gaussians
 #   per iteration         FP64 damping + Cholesky + solve: 4K^2 + 4K + 2K^3/3
-#   CRLB pass             S (model + K^2 + 2K + 3) + (W+H) rowcol + 8K^3/3
 #   initial estimate      3S (centre of mass)
+#   CRLB kernel           S (model + K^2 + 2K + 3) + (W+H) rowcol + 8K^3/3
 MODEL_TERMS = {"fixed": 5, "sigma": 8, "astig": 8, "spline": 422}
 ROWCOL_TERMS = {"fixed": 11, "sigma": 15, "astig": 16, "spline": 0}
 
 
-def flops_per_loc(kind, R, executed_iters):
+def fit_flops_per_loc(kind, R, executed_iters):
+    """Fit kernel: centre of mass + executed iterations."""
     K = 4 if kind == "fixed" else 5
     S = R * R
     per_iter = S * (K * K + 4 * K + 4 + MODEL_TERMS[kind]) + 2 * R * ROWCOL_TERMS[kind] + \
         (4 * K * K + 4 * K + 2 * K ** 3 / 3)
-    crlb = S * (MODEL_TERMS[kind] + K * K + 2 * K + 3) + 2 * R * ROWCOL_TERMS[kind] + 8 * K ** 3 / 3
-    return per_iter * executed_iters + crlb + 3 * S
+    return per_iter * executed_iters + 3 * S
+
+
+def crlb_flops_per_loc(kind, R):
+    """CRLB kernel: Fisher matrix + chi^2 + FP64 LU inversion."""
+    K = 4 if kind == "fixed" else 5
+    S = R * R
+    return S * (MODEL_TERMS[kind] + K * K + 2 * K + 3) + 2 * R * ROWCOL_TERMS[kind] + 8 * K ** 3 / 3
+
+
+def hbm_traffic(workload):
+    """HBM bytes per fit-kernel launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE scaled by a plain copy's readings), from profiles/traffic_<workload>.txt
+    (`key = value` lines), or None when that file is absent."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if "=" in line and not line.lstrip().startswith("#"):
+                k, v = line.split("=", 1)
+                vals[k.strip()] = v.strip()
+    try:
+        return float(vals["fit_kernel_hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
 
 
 def hbm_bytes_per_loc(kind, R):
@@ -206,6 +232,8 @@ def main():
         stream = torch.cuda.current_stream()
         ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
         ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        # HIP events on the launch stream around each kernel (fit, then CRLB)
+        est.set_phase_timing(True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(args.steps):
@@ -214,16 +242,21 @@ def main():
             ev1[k].record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev0, ev1)]))
+        launches, fit_ms, crlb_ms = est.phase_times()
+        est.set_phase_timing(False)
+        assert launches == args.steps, (launches, args.steps)
+        step_gpu_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev0, ev1)]))
         ms_per_step = wall * 1e3 / args.steps
 
         it = out["iterations"].cpu().numpy()
         p = out["params"].cpu().numpy()
         executed = np.minimum(it, cfg["iters"]).astype(np.float64)
-        per0 = flops_per_loc(cfg["kind"], cfg["R"], 0)
-        per1 = flops_per_loc(cfg["kind"], cfg["R"], 1) - per0
-        flops = n * per0 + per1 * executed.sum()
-        achieved = flops / (kernel_ms * 1e-3) / 1e12
+        per0 = fit_flops_per_loc(cfg["kind"], cfg["R"], 0)
+        per1 = fit_flops_per_loc(cfg["kind"], cfg["R"], 1) - per0
+        fit_flops = n * per0 + per1 * executed.sum()
+        crlb_flops = n * crlb_flops_per_loc(cfg["kind"], cfg["R"])
+        achieved = fit_flops / (fit_ms * 1e-3) / 1e12
+        traffic = hbm_traffic(args.config) if n == cfg["n"] else None
 
         # PCIe-inclusive rate through the host-pointer API (samples H2D, results D2H)
         m = min(n, 200_000)
@@ -249,16 +282,20 @@ def main():
         "data": "synthetic",
         "config": {"workload": cfg["workload"], "roi": cfg["R"], "spots": n, "iterations": cfg["iters"],
                    "lanes_per_spot": group},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": None},
+        "roofline": {"kernel": "fit_kernel", "bound": "valu", "achieved": achieved, "peak": PEAK_FP32,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic},
         "cpu_baseline": cpu,
         "detail": {
-            "kernel_ms": kernel_ms,
-            "algorithmic_flop_per_launch": flops,
+            "step_gpu_ms": step_gpu_ms,
+            "fit_kernel_ms": fit_ms,
+            "crlb_kernel_ms": crlb_ms,
+            "fit_kernel_algorithmic_flop": fit_flops,
+            "crlb_kernel_algorithmic_flop": crlb_flops,
+            "crlb_kernel_tflops": crlb_flops / (crlb_ms * 1e-3) / 1e12 if crlb_ms > 0 else None,
             "mean_iterations": float(executed.mean()),
             "stopped_fraction": float(np.mean(it <= cfg["iters"])),
             "hbm_bytes_per_launch": n * hbm_bytes_per_loc(cfg["kind"], cfg["R"]),
-            "hbm_GBps": n * hbm_bytes_per_loc(cfg["kind"], cfg["R"]) / (kernel_ms * 1e-3) / 1e9,
+            "hbm_GBps": n * hbm_bytes_per_loc(cfg["kind"], cfg["R"]) / (step_gpu_ms * 1e-3) / 1e9,
             "pcie_inclusive_loc_per_s": e2e,
             "finite_fraction": float(np.mean(np.all(np.isfinite(p), axis=1))),
         },

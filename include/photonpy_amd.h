@@ -133,6 +133,14 @@ int Estimator_GetGroupSize(Estimator* e);
  * (0 = one eighth of the wavefront's groups, at least 1). */
 int Estimator_SetRefill(Estimator* e, int refill);
 
+/* Kernel timing of the fit (no reference counterpart; for benchmarks).  While
+ * enabled, every fit launch records HIP events on its stream around the fit
+ * kernel and the CRLB kernel.  Estimator_GetPhaseTimes waits for the recorded
+ * launches, writes the mean duration in ms of each kernel and releases the
+ * events; it returns the number of launches averaged (-1 on error). */
+int Estimator_SetPhaseTiming(Estimator* e, int enable);
+int Estimator_GetPhaseTimes(Estimator* e, float* fitMs, float* crlbMs);
+
 /* Fused fit on device memory: Levenberg-Marquardt from the centre-of-mass start
  * (or d_initial), then CRLB and chi^2 when d_crlb / d_chisq are non-NULL.
  * d_sample [n,H,W], d_params [n,K], d_iterations [n], d_trace [n,traceLen,K]. */

@@ -67,6 +67,16 @@ struct Estimator {
     int* devCounters = nullptr;
     unsigned counterSlot = 0;
 
+    // Optional kernel timing: three events per fit launch (before the fit kernel,
+    // after it, after the CRLB kernel), read and released by Estimator_GetPhaseTimes.
+    bool phaseTiming = false;
+    std::vector<hipEvent_t> phaseEvents;
+
+    void release_phase_events() {
+        for (hipEvent_t ev : phaseEvents) (void)hipEventDestroy(ev);
+        phaseEvents.clear();
+    }
+
     int* next_counter() {
         if (!devCounters) hip_check(hipMalloc(&devCounters, kCounterSlots * sizeof(int)), "hipMalloc(counters)");
         return devCounters + (counterSlot++ % kCounterSlots);
@@ -108,6 +118,7 @@ struct Estimator {
         if (devShared) (void)hipFree(devShared);
         if (devCounters) (void)hipFree(devCounters);
         if (ws) (void)hipFree(ws);
+        release_phase_events();
     }
 
     void ensure_device() {
@@ -204,7 +215,16 @@ static void run_fit(Estimator* e, int n, const float* d_sample, const float* d_i
     a.lambda = e->lm.lambda;
     a.refill = e->refill;
     a.counter = e->next_counter();
-    hipError_t err = launch_fit_any(e->desc, R, e->G(), a, 0, st);
+    hipEvent_t* events = nullptr;
+    if (e->phaseTiming) {
+        for (int i = 0; i < 3; i++) {
+            hipEvent_t ev;
+            hip_check(hipEventCreate(&ev), "hipEventCreate");
+            e->phaseEvents.push_back(ev);
+        }
+        events = e->phaseEvents.data() + e->phaseEvents.size() - 3;
+    }
+    hipError_t err = launch_fit_any(e->desc, R, e->G(), a, events, st);
     if (err == hipErrorInvalidValue)
         throw std::runtime_error("photonpy_amd: no fit kernel compiled for roisize " + std::to_string(R) + ", group " +
                                  std::to_string(e->G()));
@@ -428,6 +448,38 @@ int Estimator_SetRefill(Estimator* e, int refill) {
     if (!e || refill < 0 || refill > 64) return 1;
     e->refill = refill;
     return 0;
+}
+
+int Estimator_SetPhaseTiming(Estimator* e, int enable) {
+    if (!e) return 1;
+    std::lock_guard<std::mutex> lk(e->mtx);
+    e->phaseTiming = enable != 0;
+    if (!e->phaseTiming) e->release_phase_events();
+    return 0;
+}
+
+int Estimator_GetPhaseTimes(Estimator* e, float* fitMs, float* crlbMs) {
+    if (!e) return -1;
+    return guarded(
+        [&]() -> int {
+            std::lock_guard<std::mutex> lk(e->mtx);
+            const int launches = (int)(e->phaseEvents.size() / 3);
+            double fit = 0.0, crlb = 0.0;
+            for (int i = 0; i < launches; i++) {
+                hipEvent_t* ev = e->phaseEvents.data() + 3 * i;
+                float a = 0.0f, b = 0.0f;
+                hip_check(hipEventSynchronize(ev[2]), "hipEventSynchronize");
+                hip_check(hipEventElapsedTime(&a, ev[0], ev[1]), "hipEventElapsedTime");
+                hip_check(hipEventElapsedTime(&b, ev[1], ev[2]), "hipEventElapsedTime");
+                fit += a;
+                crlb += b;
+            }
+            e->release_phase_events();
+            if (fitMs) *fitMs = launches ? (float)(fit / launches) : 0.0f;
+            if (crlbMs) *crlbMs = launches ? (float)(crlb / launches) : 0.0f;
+            return launches;
+        },
+        -1);
 }
 
 void Estimator_ComputeExpectedValue(Estimator* e, int n, const float* params, const float* constants,

@@ -30,12 +30,13 @@ inline int default_group(int R) {
 
 constexpr int kMinFitR = 3, kMaxFitR = 32;
 
-// Per-model launchers, defined in kernels_<model>.hip.  Return hipErrorInvalidValue
-// when no kernel is compiled for (R, G).
-hipError_t launch_fit_fixed(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st);
-hipError_t launch_fit_sigma(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st);
-hipError_t launch_fit_astig(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st);
-hipError_t launch_fit_spline(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st);
+// Per-model launchers, defined in fit_<model>_<part>.hip.  Return hipErrorInvalidValue
+// when no kernel is compiled for (R, G).  `events` is null or three events that are
+// recorded on the stream before the fit kernel, after it and after the CRLB kernel.
+hipError_t launch_fit_fixed(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st);
+hipError_t launch_fit_sigma(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st);
+hipError_t launch_fit_astig(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st);
+hipError_t launch_fit_spline(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st);
 
 hipError_t launch_ev_any(const ModelDesc& d, int R, int n, const float* params, float* ev, float* deriv, hipStream_t st);
 hipError_t launch_crlb_any(const ModelDesc& d, int R, int n, const float* params, const float* sample, float* crlb,
@@ -57,44 +58,51 @@ hipError_t launch_spline_coefs(const int* shape, const float* d_data, float* d_o
 // Sets *counter = value on the stream (a one-thread kernel, so it can also be captured in a graph).
 hipError_t launch_set_counter(int* counter, int value, hipStream_t st);
 
-inline hipError_t launch_fit_any(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st) {
+inline hipError_t launch_fit_any(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st) {
     switch (d.kind) {
-    case kKindFixed: return launch_fit_fixed(d, R, G, a, maxBlocks, st);
-    case kKindSigma: return launch_fit_sigma(d, R, G, a, maxBlocks, st);
-    case kKindAstig: return launch_fit_astig(d, R, G, a, maxBlocks, st);
-    case kKindSpline: return launch_fit_spline(d, R, G, a, maxBlocks, st);
+    case kKindFixed: return launch_fit_fixed(d, R, G, a, events, st);
+    case kKindSigma: return launch_fit_sigma(d, R, G, a, events, st);
+    case kKindAstig: return launch_fit_astig(d, R, G, a, events, st);
+    case kKindSpline: return launch_fit_spline(d, R, G, a, events, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-// Persistent grid for a fit kernel: enough blocks to cover every group once,
-// capped at the number of blocks the device keeps resident.
+// Blocks of a kernel that the whole device keeps resident (256 threads per block).
+inline int resident_blocks(const void* kernel) {
+    int dev = 0, perCU = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, 256, 0) != hipSuccess || perCU < 1) perCU = 1;
+    return perCU * prop.multiProcessorCount;
+}
+
+// Persistent grids: enough blocks to cover every group once, capped at what the
+// device keeps resident.  The fit kernel starts every group on spot
+// blockIdx * groupsPerBlock + group and hands out the rest from the counter.
 template <class M, int R, int G>
-hipError_t launch_fit_t(const M& m, const FitArgs& a, int maxBlocks, hipStream_t st) {
+hipError_t launch_fit_t(const M& m, const FitArgs& a, hipEvent_t* events, hipStream_t st) {
     constexpr int kBlock = 256;
-    static int resident = 0;  // blocks resident on the whole device
-    if (resident == 0) {
-        int dev = 0, perCU = 0;
-        hipDeviceProp_t prop;
-        if (hipGetDevice(&dev) != hipSuccess) return hipErrorNoDevice;
-        if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return hipErrorNoDevice;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fit_kernel<M, R, G>, kBlock, 0) != hipSuccess || perCU < 1)
-            perCU = 1;
-        resident = perCU * prop.multiProcessorCount;
+    static int residentFit = 0, residentCrlb = 0;  // per process; one device model (gfx950)
+    if (residentFit == 0) {
+        residentFit = resident_blocks(reinterpret_cast<const void*>(fit_kernel<M, R, G>));
+        residentCrlb = resident_blocks(reinterpret_cast<const void*>(crlb_phase_kernel<M, R, G>));
+        if (residentFit == 0 || residentCrlb == 0) return hipErrorNoDevice;
     }
     const long groupsPerBlock = kBlock / G;
-    long blocks = (a.n + groupsPerBlock - 1) / groupsPerBlock;
-    long cap = maxBlocks > 0 ? maxBlocks : resident;
-    if (blocks > cap) blocks = cap;
-    if (blocks < 1) blocks = 1;
+    const long need = (a.n + groupsPerBlock - 1) / groupsPerBlock;
+    const long fitBlocks = need < 1 ? 1 : (need > residentFit ? residentFit : need);
+    const long crlbBlocks = need < 1 ? 1 : (need > residentCrlb ? residentCrlb : need);
     if (a.counter) {
-        // spots [0, groups) are the groups' first spots; the counter hands out the rest
-        const hipError_t e = launch_set_counter(a.counter, (int)(blocks * groupsPerBlock), st);
+        const hipError_t e = launch_set_counter(a.counter, (int)(fitBlocks * groupsPerBlock), st);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((fit_kernel<M, R, G>), dim3((unsigned)blocks), dim3(kBlock), 0, st, m, a, 0);
+    if (events) (void)hipEventRecord(events[0], st);
+    hipLaunchKernelGGL((fit_kernel<M, R, G>), dim3((unsigned)fitBlocks), dim3(kBlock), 0, st, m, a);
+    if (events) (void)hipEventRecord(events[1], st);
     if (a.crlb || a.chisq)
-        hipLaunchKernelGGL((fit_kernel<M, R, G>), dim3((unsigned)blocks), dim3(kBlock), 0, st, m, a, 1);
+        hipLaunchKernelGGL((crlb_phase_kernel<M, R, G>), dim3((unsigned)crlbBlocks), dim3(kBlock), 0, st, m, a);
+    if (events) (void)hipEventRecord(events[2], st);
     return hipGetLastError();
 }
 

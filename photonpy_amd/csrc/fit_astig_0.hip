@@ -4,24 +4,24 @@
 
 namespace ppa {
 
-hipError_t launch_fit_astig_1(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st);
-hipError_t launch_fit_astig_2(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st);
-hipError_t launch_fit_astig_3(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st);
+hipError_t launch_fit_astig_1(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st);
+hipError_t launch_fit_astig_2(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st);
+hipError_t launch_fit_astig_3(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st);
 
-hipError_t launch_fit_astig(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st) {
+hipError_t launch_fit_astig(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st) {
     using Model = GaussModel<kAstig>;
     Model m{d.gc};
     PPA_FIT_SIZES_0(PPA_FIT_CASE)
     {
-        hipError_t e = launch_fit_astig_1(d, R, G, a, maxBlocks, st);
+        hipError_t e = launch_fit_astig_1(d, R, G, a, events, st);
         if (e != hipErrorInvalidValue) return e;
     }
     {
-        hipError_t e = launch_fit_astig_2(d, R, G, a, maxBlocks, st);
+        hipError_t e = launch_fit_astig_2(d, R, G, a, events, st);
         if (e != hipErrorInvalidValue) return e;
     }
     {
-        hipError_t e = launch_fit_astig_3(d, R, G, a, maxBlocks, st);
+        hipError_t e = launch_fit_astig_3(d, R, G, a, events, st);
         if (e != hipErrorInvalidValue) return e;
     }
     return hipErrorInvalidValue;

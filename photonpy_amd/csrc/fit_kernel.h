@@ -264,16 +264,19 @@ __device__ __forceinline__ void crlb_spots(const M& model, const FitArgs& a, int
     }
 }
 
-// mode 0: fit (phase 1); mode 1: CRLB and chi^2 of the fitted spots (phase 2).
-// The phases are separate launches so that phase 2 sees every wavefront's results.
+// Phase 1: fit.  Phase 2 (CRLB and chi^2 of the fitted spots) is a kernel of its
+// own, launched after it on the same stream, so that it sees every wavefront's
+// results and runs at the occupancy of its own, smaller register set.
 template <class M, int R, int G>
-__global__ void __launch_bounds__(256) fit_kernel(M model, FitArgs a, int mode) {
+__global__ void __launch_bounds__(256) fit_kernel(M model, FitArgs a) {
     const int groupsPerBlock = blockDim.x / G;
-    const int first = blockIdx.x * groupsPerBlock + threadIdx.x / G, stride = gridDim.x * groupsPerBlock;
-    if (mode == 0)
-        fit_spots<M, R, G>(model, a, first, stride);
-    else
-        crlb_spots<M, R, G>(model, a, first, stride);
+    fit_spots<M, R, G>(model, a, blockIdx.x * groupsPerBlock + threadIdx.x / G, gridDim.x * groupsPerBlock);
+}
+
+template <class M, int R, int G>
+__global__ void __launch_bounds__(256) crlb_phase_kernel(M model, FitArgs a) {
+    const int groupsPerBlock = blockDim.x / G;
+    crlb_spots<M, R, G>(model, a, blockIdx.x * groupsPerBlock + threadIdx.x / G, gridDim.x * groupsPerBlock);
 }
 
 }  // namespace ppa

@@ -5,7 +5,7 @@
 namespace ppa {
 
 
-hipError_t launch_fit_sigma_1(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st) {
+hipError_t launch_fit_sigma_1(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st) {
     using Model = GaussModel<kSigma>;
     Model m{d.gc};
     PPA_FIT_SIZES_1(PPA_FIT_CASE)

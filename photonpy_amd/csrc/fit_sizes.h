@@ -10,4 +10,4 @@
 #define PPA_FIT_SIZES_3(X) X(25, 16) X(25, 8) X(26, 16) X(27, 16) X(28, 16) X(29, 16) X(30, 16) X(31, 16) X(32, 16)
 
 #define PPA_FIT_CASE(R_, G_) \
-    if (R == R_ && G == G_) return launch_fit_t<Model, R_, G_>(m, a, maxBlocks, st);
+    if (R == R_ && G == G_) return launch_fit_t<Model, R_, G_>(m, a, events, st);

@@ -16,10 +16,10 @@ for name, ty, ctor in MODELS:
         tail = "    return hipErrorInvalidValue;\n"
         if part == 0:
             decls = "".join(
-                f"hipError_t launch_fit_{name}_{p}(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks,"
+                f"hipError_t launch_fit_{name}_{p}(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events,"
                 f" hipStream_t st);\n" for p in range(1, PARTS))
             tail = "".join(
-                f"    {{\n        hipError_t e = launch_fit_{name}_{p}(d, R, G, a, maxBlocks, st);\n"
+                f"    {{\n        hipError_t e = launch_fit_{name}_{p}(d, R, G, a, events, st);\n"
                 f"        if (e != hipErrorInvalidValue) return e;\n    }}\n" for p in range(1, PARTS)) + tail
             fname = f"launch_fit_{name}"
         else:
@@ -31,7 +31,7 @@ for name, ty, ctor in MODELS:
 namespace ppa {{
 
 {decls}
-hipError_t {fname}(const ModelDesc& d, int R, int G, const FitArgs& a, int maxBlocks, hipStream_t st) {{
+hipError_t {fname}(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st) {{
     using Model = {ty};
     {ctor}
     PPA_FIT_SIZES_{part}(PPA_FIT_CASE)

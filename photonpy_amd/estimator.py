@@ -8,6 +8,8 @@ reference counterpart.
 """
 from __future__ import annotations
 
+import ctypes as ct
+
 import numpy as np
 
 from .context import Context
@@ -325,6 +327,21 @@ class Estimator:
 
     def GetGroupSize(self):
         return self._lib.Estimator_GetGroupSize(self.inst)
+
+    def set_phase_timing(self, enable=True):
+        """Record HIP events around the fit and CRLB kernels of every fit launch."""
+        if self._lib.Estimator_SetPhaseTiming(self.inst, 1 if enable else 0) != 0:
+            raise RuntimeError("Estimator_SetPhaseTiming failed")
+
+    def phase_times(self):
+        """(launches, mean fit-kernel ms, mean CRLB-kernel ms) of the launches recorded
+        since timing was enabled or last read; waits for them and releases the events."""
+        fit, crlb = ct.c_float(), ct.c_float()
+        n = self._lib.Estimator_GetPhaseTimes(self.inst, ct.byref(fit), ct.byref(crlb))
+        if n < 0:
+            self._check()
+            raise RuntimeError("Estimator_GetPhaseTimes failed")
+        return n, fit.value, crlb.value
 
     # --------------------------------------------------------------- misc
     def Destroy(self):

@@ -102,6 +102,8 @@ def _bind(lib):
         "Estimator_SetGroupSize": ([V, I], ct.c_int),
         "Estimator_GetGroupSize": ([V], ct.c_int),
         "Estimator_SetRefill": ([V, I], ct.c_int),
+        "Estimator_SetPhaseTiming": ([V, I], ct.c_int),
+        "Estimator_GetPhaseTimes": ([V, ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)], ct.c_int),
         "Estimator_ComputeExpectedValue": ([V, I, FloatArray, FloatArray, IntArray, FloatArray], None),
         "Estimator_Estimate": ([V, I, FloatArray, FloatArray, IntArray, NullableFloatArrayType, FloatArray,
                                 FloatArray, IntArray, NullableFloatArrayType, I], None),
