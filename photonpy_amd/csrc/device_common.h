@@ -24,9 +24,12 @@ PP_DEV float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 // group ends with bit-identical totals.  Steps inside a row of 16 lanes use DPP
 // moves (quad_perm xor 1 / xor 2, row_half_mirror, row_mirror: each pairs every
 // lane with one in the other half of its block); wider steps use ds_bpermute.
+// The moved value is written with old = 0 and bound_ctrl set (every lane of a
+// group is active, so no lane reads a disabled source); that form lets the
+// compiler fold the move into the add as its DPP source operand.
 template <int CTRL>
 PP_DEV float dpp_add(float v) {
-    return v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
 template <int G>

@@ -31,15 +31,28 @@ struct Edge {
     float E, G, H;  // erf(a), exp(-a^2), a*exp(-a^2)
 };
 
+// exp(-a^2) as 2^(-a^2 log2 e) on the hardware exp2: relative error about
+// a^2 * 2^-24 from rounding the argument (the value itself is below 2^-24 once
+// a^2 > 16, so the absolute error stays under one FP32 ulp of 1).  Results
+// below FLT_MIN flush to zero.
+PP_DEV float exp_neg_sq(float a) { return __builtin_amdgcn_exp2f(-(a * a) * 1.44269504088896341f); }
+
 PP_DEV Edge gauss_edge(float e, float t, float c2) {
     float a = (e - t - 0.5f) * c2;
-    float g = expf(-a * a);
+    float g = exp_neg_sq(a);
     return {erff(a), g, a * g};
 }
 
-// Per-axis terms of one pixel from its left (e0) and right (e1) edges.
+// Per-axis terms of one pixel from its left (e0) and right (e1) edges.  dE and S
+// are scaled by the intensity I (scale_by) before pixel() uses them, once per
+// column and row instead of once per pixel.
 struct AxisTerms {
-    float D, dE, S;  // integrated Gaussian, d/dt (without I), shape-derivative term
+    float D, dE, S;  // integrated Gaussian, d/dt, shape-derivative term
+
+    PP_DEV void scale_by(float I) {
+        dE *= I;
+        S *= I;
+    }
 };
 
 PP_DEV AxisTerms axis_terms(const Edge& e0, const Edge& e1, float cp, float cs) {
@@ -157,16 +170,22 @@ struct GaussModel {
         }
     }
 
-    // mu and Jacobian of one pixel from its column (X) and row (Y) terms.
+    // mu and Jacobian of one pixel from its column (X) and row (Y) terms, whose
+    // dE and S are already multiplied by I.
     PP_DEV void pixel(const AxisTerms& X, const AxisTerms& Y, const float (&t)[K], float& mu, float (&J)[K]) const {
         const float I = t[iI], bg = t[iBg];
         const float p = X.D * Y.D;
         mu = bg + I * p;
-        J[0] = I * X.dE * Y.D;
-        J[1] = I * Y.dE * X.D;
+        J[0] = X.dE * Y.D;
+        J[1] = Y.dE * X.D;
         J[iI] = p;
         J[iBg] = 1.0f;
-        if (V != kFixed) J[iS] = I * (X.S * Y.D + Y.S * X.D);
+        if (V != kFixed) J[iS] = X.S * Y.D + Y.S * X.D;
+    }
+
+    PP_DEV static AxisTerms scaled(AxisTerms a, float I) {
+        a.scale_by(I);
+        return a;
     }
 
     // Single-pixel evaluation (expected value / derivatives kernels).
@@ -175,7 +194,8 @@ struct GaussModel {
         shapes(t, SX, SY);
         const Edge x0 = gauss_edge((float)x, t[0], SX.c2), x1 = gauss_edge((float)(x + 1), t[0], SX.c2);
         const Edge y0 = gauss_edge((float)y, t[1], SY.c2), y1 = gauss_edge((float)(y + 1), t[1], SY.c2);
-        pixel(axis_terms(x0, x1, SX.cp, SX.cs), axis_terms(y0, y1, SY.cp, SY.cs), t, mu, J);
+        pixel(scaled(axis_terms(x0, x1, SX.cp, SX.cs), t[iI]), scaled(axis_terms(y0, y1, SY.cp, SY.cs), t[iI]), t,
+              mu, J);
     }
 
     // Fit-kernel evaluation over a block of RL rows starting at y0 of an RxR ROI,
@@ -193,7 +213,7 @@ struct GaussModel {
 #pragma unroll
             for (int x = 0; x < R; x++) {
                 Edge nx = gauss_edge((float)(x + 1), t[0], SX.c2);
-                col[x] = axis_terms(prev, nx, SX.cp, SX.cs);
+                col[x] = scaled(axis_terms(prev, nx, SX.cp, SX.cs), t[iI]);
                 prev = nx;
             }
         } else {
@@ -211,7 +231,7 @@ struct GaussModel {
             static_for<1, R + 1>([&](auto e) {
                 constexpr int E = decltype(e)::value;
                 const Edge nx = fetch_edge<G, E>(Ev, Gv, Hv);
-                col[E - 1] = axis_terms(prev, nx, SX.cp, SX.cs);
+                col[E - 1] = scaled(axis_terms(prev, nx, SX.cp, SX.cs), t[iI]);
                 prev = nx;
             });
         }
@@ -220,7 +240,7 @@ struct GaussModel {
         for (int r = 0; r < RL; r++) {
             const int y = y0 + r;
             Edge nx = gauss_edge((float)(y + 1), t[1], SY.c2);
-            const AxisTerms row = axis_terms(prev, nx, SY.cp, SY.cs);
+            const AxisTerms row = scaled(axis_terms(prev, nx, SY.cp, SY.cs), t[iI]);
             prev = nx;
             if (y < R) {
 #pragma unroll
