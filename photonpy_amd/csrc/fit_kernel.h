@@ -56,8 +56,11 @@ struct FitTraits {
 // Loads rows [y0, y0+RL) of a spot into s and returns the lane's centre-of-mass
 // partial sums.  With CLAMP the stored samples are max(x, 1e-6) as the
 // Levenberg-Marquardt terms use them (`if (x < 1e-6) x = 1e-6`, which keeps a
-// NaN, Estimation.h:114); the sums use the raw values.
-template <int R, int RL, bool CLAMP>
+// NaN, Estimation.h:114); the sums use the raw values.  The fit kernel loads
+// with the non-temporal hint (a few groups refill at a time and each ROI is read
+// once); the CRLB pass, where every group loads at once, uses ordinary loads so
+// that the lines its lanes share stay in L2 between the per-column loads.
+template <int R, int RL, bool CLAMP, bool NT>
 PP_DEV void load_rows(const float* src, int y0, float (&s)[RL][R], float& sx, float& sy, float& sum) {
     sx = sy = sum = 0.0f;
 #pragma unroll
@@ -65,7 +68,8 @@ PP_DEV void load_rows(const float* src, int y0, float (&s)[RL][R], float& sx, fl
         const int y = y0 + r;
 #pragma unroll
         for (int x = 0; x < R; x++) {
-            const float v = (y < R) ? __builtin_nontemporal_load(src + y * R + x) : 0.0f;
+            float v = 0.0f;
+            if (y < R) v = NT ? __builtin_nontemporal_load(src + y * R + x) : src[y * R + x];
             s[r][x] = (CLAMP && v < 1e-6f) ? 1e-6f : v;
             sx += (float)x * v;
             sy += (float)y * v;
@@ -132,7 +136,7 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
 
     auto start_spot = [&](int sp) {
         float sx, sy, sum;
-        load_rows<R, RL, true>(a.sample + (size_t)sp * T::S, y0, s, sx, sy, sum);
+        load_rows<R, RL, true, true>(a.sample + (size_t)sp * T::S, y0, s, sx, sy, sum);
         if (a.initial) {
 #pragma unroll
             for (int k = 0; k < K; k++) th[k] = a.initial[(size_t)sp * K + k];
@@ -256,7 +260,7 @@ __device__ __forceinline__ void crlb_spots(const M& model, const FitArgs& a, int
     for (int sp = g_first; __any(sp < a.n); sp += stride) {
         if (sp < a.n) {
             float sx, sy, sum;
-            load_rows<R, RL, false>(a.sample + (size_t)sp * T::S, y0, s, sx, sy, sum);
+            load_rows<R, RL, false, false>(a.sample + (size_t)sp * T::S, y0, s, sx, sy, sum);
 #pragma unroll
             for (int k = 0; k < K; k++) th[k] = a.params[(size_t)sp * K + k];
             crlb_spot<M, R, G>(model, a, s, th, y0, sp, g == 0);

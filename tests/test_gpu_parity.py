@@ -278,3 +278,40 @@ def test_full_size_error_statistics_match_crlb(ctx):
         z = (p[good, k] - th[good, k]) / crlb[good, k]
         assert abs(np.std(z) - 1) < 0.1, (k, np.std(z))
         assert abs(np.mean(z)) < 0.05
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_error_against_fp64_fit(ctx, kind):
+    """Each model's error against an FP64 CPU fit (the oracle's estimator in FP64) on spots
+    that both fits converge on, away from the parameter bounds.  Prints one line per model
+    (`FP64-ERR ...`, shown with pytest -s) and asserts the 99th percentile is inside the
+    parity tolerance."""
+    n = {"fixed": 4000, "sigma": 4000, "astig": 1500, "spline": 600}[kind]
+    iters = 30 if kind == "fixed" else 50
+    lam = 1e-15 if kind == "fixed" else 1e-18
+    th, smp = MD.samples(kind, n, 21)
+    est, om, _ = MD.make(kind, ctx)
+    est.SetLevMarParams(lam, iters)
+    p_gpu, _, _ = est.Estimate(smp)
+    it_gpu = est.last_iterations
+    p64, it64 = om.estimate(smp, iterations=200, lambda_=lam, dtype=np.float64)
+    lo, hi = est.ParamLimits()
+    use = (it64 <= 200) & ~_on_bounds(p64, lo, hi) & ~_on_bounds(p_gpu, lo, hi)
+    use &= np.all(np.isfinite(p64), axis=1) & np.all(np.isfinite(p_gpu), axis=1)
+    if kind in ("fixed", "sigma"):
+        use &= it_gpu <= iters
+    assert use.mean() > 0.5
+    d = np.abs(p_gpu[use] - p64[use])
+    iI = 3 if _zcol(kind) else 2
+    rel = d.copy()
+    rel[:, iI] /= np.maximum(np.abs(p64[use, iI]), 1)
+    rel[:, iI + 1] /= np.maximum(np.abs(p64[use, iI + 1]), 1)
+    names = est.param_names
+    p99 = np.percentile(rel, 99, axis=0)
+    print(f"FP64-ERR {kind} spots={use.sum()} " + " ".join(
+        f"{nm}:p50={np.median(rel[:, k]):.2e},p99={p99[k]:.2e},max={rel[:, k].max():.2e}"
+        for k, nm in enumerate(names)))
+    assert p99[0] <= TOL["xy"] and p99[1] <= TOL["xy"]
+    assert p99[iI] <= TOL["rel"] and p99[iI + 1] <= TOL["rel"]
+    if _zcol(kind):
+        assert p99[2] <= TOL["z"]
