@@ -10,6 +10,8 @@ Workloads (the default is c2):
   c2  2D Gaussian free sigma, 11x11 px, 1,000,000 spots, 50 iterations + CRLB
   c3  astigmatic 3D Gaussian (quartic sigma(z)), 16x16 px, 2,000,000 spots, 50 iterations + CRLB
   c4  cubic-spline 3D PSF (61x26x26 cells), 20x20 px, 500,000 spots, 50 iterations + CRLB
+  c4adj  spline PSF adjoint (forward NLL + backward to params and all coefficients), 65,536 ROIs
+  c5  mixed batch: 4,000,000 ROIs over 7 sizes x 4 models, batch form vs plain per-bin launches
 Synthetic data (photonpy_amd/synthetic.py): true parameters from numpy
 PCG64(seed 1) as the workload states, expected images from the library's own
 expected-value kernel, Poisson noise from a seeded torch generator on the GPU.
@@ -129,6 +131,65 @@ def host_api_rate(est, smp_host):
     return m / (time.perf_counter() - t1)
 
 
+def bench_mixed(args):
+    """Mixed batch (4,000,000 ROIs over 28 model x size bins): the batch form (bins on
+    concurrent streams, one HIP graph launch per step) against the plain per-bin launches
+    one after another on one stream; both compute fit + CRLB + chi^2 for every spot."""
+    import torch
+    import photonpy_amd as pa
+    from photonpy_amd import autotune, synthetic
+    from photonpy_amd.batch import MixedBatch
+    with pa.Context() as ctx:
+        bins = synthetic.mixed_bins(ctx, n=args.n or None)
+        ests = [b["est"] for b in bins]
+        counts = [b["n"] for b in bins]
+        samples = [b["samples"] for b in bins]
+        n = sum(counts)
+        stream = torch.cuda.current_stream()
+        with MixedBatch(ests, counts, streams=autotune.batch_streams(), tune=True) as mb:
+            outs = mb.allocate_outputs()
+            ref = mb.allocate_outputs()
+
+            def timed(fn):
+                for _ in range(args.warmup):
+                    fn()
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                t0 = time.perf_counter()
+                e0.record(stream)
+                for _ in range(args.steps):
+                    fn()
+                e1.record(stream)
+                torch.cuda.synchronize()
+                return (time.perf_counter() - t0) * 1e3 / args.steps, e0.elapsed_time(e1) / args.steps
+
+            plain_ms, plain_gpu = timed(lambda: mb.run_plain(samples, ref))
+            batch_ms, batch_gpu = timed(lambda: mb.run(samples, outs, use_graph=True))
+            same = all(torch.equal(a["params"], b["params"]) for a, b in zip(outs, ref))
+            flops = 0.0
+            for b, o in zip(bins, outs):
+                it = np.minimum(o["iterations"].cpu().numpy(), b["iters"]).astype(np.float64)
+                per0 = fit_flops_per_loc(b["kind"], b["R"], 0)
+                per1 = fit_flops_per_loc(b["kind"], b["R"], 1) - per0
+                flops += b["n"] * (per0 + crlb_flops_per_loc(b["kind"], b["R"])) + per1 * it.sum()
+            streams = mb.num_streams()
+            forms = sorted({b["est"].spline_form() for b in bins if b["kind"] == "spline"})
+    achieved = flops / (batch_gpu * 1e-3) / 1e12
+    print(json.dumps({
+        "metric": METRIC, "value": n / (batch_ms * 1e-3), "unit": "localizations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": batch_ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 solve)", "data": "synthetic",
+        "config": {"workload": synthetic.MIXED["workload"], "spots": n, "bins": len(bins), "streams": streams,
+                   "spline_form": forms},
+        "roofline": {"kernel": "all fit and CRLB kernels of the batch", "bound": "valu", "achieved": achieved,
+                     "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None},
+        "cpu_baseline": None,
+        "detail": {"plain_ms_per_step": plain_ms, "plain_loc_per_s": n / (plain_ms * 1e-3),
+                   "batch_over_plain": plain_ms / batch_ms, "batch_gpu_ms": batch_gpu, "plain_gpu_ms": plain_gpu,
+                   "results_identical": bool(same),
+                   "bins": [{"kind": b["kind"], "R": b["R"], "n": b["n"]} for b in bins]}}))
+
+
 def bench_adjoint(args):
     """Spline PSF calibration step: forward Poisson NLL + hand-written adjoint on 65,536
     20x20 ROIs, gradients w.r.t. per-spot parameters and all 2,638,976 spline coefficients
@@ -181,7 +242,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c4adj"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c4adj", "c5"])
     ap.add_argument("--n", type=int, default=0, help="override the number of spots")
     ap.add_argument("--group", type=int, default=0, help="lane-group size override (0 = tuned/default)")
     ap.add_argument("--refill", type=int, default=0, help="refill batch override (0 = tuned/default)")
@@ -212,6 +273,8 @@ def main():
 
     if args.config == "c4adj":
         return bench_adjoint(args)
+    if args.config == "c5":
+        return bench_mixed(args)
     cfg = synthetic.WORKLOADS[args.config]
     n = args.n or cfg["n"]
     torch.cuda.set_device(0)

@@ -138,6 +138,14 @@ int Estimator_SetRefill(Estimator* e, int refill);
  * kernel and the CRLB kernel.  Estimator_GetPhaseTimes waits for the recorded
  * launches, writes the mean duration in ms of each kernel and releases the
  * events; it returns the number of launches averaged (-1 on error). */
+/* Spline evaluation form (no reference counterpart).  A coefficient table that CSpline_Compute
+ * made is the local tricubic interpolant of a sample grid; the library recovers that grid at
+ * creation (checked against every coefficient) and by default fits and evaluates from it (mode
+ * 0).  Mode 1 keeps the coefficient form.  Estimator_SplineForm: 1 grid, 0 coefficients, -1 not
+ * a spline estimator.  Results of the two forms agree to FP32 rounding of the model. */
+int Estimator_SetSplineMode(Estimator* e, int mode);
+int Estimator_SplineForm(Estimator* e);
+
 int Estimator_SetPhaseTiming(Estimator* e, int enable);
 int Estimator_GetPhaseTimes(Estimator* e, float* fitMs, float* crlbMs);
 
@@ -164,21 +172,23 @@ int Estimator_ChiSquareAndCRLBDevice(Estimator* e, int n, const float* d_params,
 int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const float* d_sample, const float* d_shared,
                             const float* d_gout, float* d_nll, float* d_grad_params, float* d_grad_shared,
                             void* stream);
-/* Persistent mixed batch: bins of spots, one estimator (model + ROI size) per bin, fitted by one
- * persistent launch whose wavefronts claim tiles of (64/G)*spotsPerGroup[b] spots from a work
- * counter, longest tiles first; with useGraph the reset + launch are replayed from a HIP graph
- * captured on the first run.  Compiled ROI sizes: 7, 9, 11, 13, 16, 20, 25 (all four models).
- * spotsPerGroup may be NULL (8 per group); blocks 0 = all resident blocks.  Run takes per-bin
- * device pointers (arrays of nbins pointers; d_iters/d_crlb/d_chisq may be NULL).  Two runs that
- * overlap on different streams race on the batch's work counter. */
+/* Mixed batch (no reference counterpart): bins of spots, one estimator (model + ROI size) per
+ * bin.  The bins are spread longest-first over nstreams internal HIP streams (0 = 4, one per
+ * hardware queue of the process) that run concurrently, so one bin's tail overlaps the next
+ * bins' work and different models share the CUs; each bin runs the estimator's own persistent
+ * fit + CRLB launches.  With useGraph the fork, every launch and the join are captured into one
+ * HIP graph on the first run with a given set of buffers and replayed with one hipGraphLaunch
+ * on `stream`; new buffers are captured again.  Run takes per-bin device pointers (arrays of
+ * nbins pointers; d_iters/d_crlb/d_chisq may be NULL).  A graph replay reuses the work-counter
+ * slots it was captured with: do not run it concurrently with other fits of the same
+ * estimators on other streams. */
 typedef struct MixedBatch MixedBatch;
-MixedBatch* MixedBatch_Create(int nbins, Estimator* const* estimators, const int* counts, const int* spotsPerGroup,
-                              int blocks);
+MixedBatch* MixedBatch_Create(int nbins, Estimator* const* estimators, const int* counts, int nstreams);
 void MixedBatch_Delete(MixedBatch* mb);
-int MixedBatch_NumTiles(MixedBatch* mb);
+int MixedBatch_NumStreams(MixedBatch* mb);
 int MixedBatch_Run(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
                    float* const* d_crlb, float* const* d_chisq, int useGraph, void* stream);
-/* The same bins as one plain per-model launch each (the comparison form). */
+/* The same bins as one plain launch per bin, in bin order on `stream` (the comparison form). */
 int MixedBatch_RunPlain(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
                         float* const* d_crlb, float* const* d_chisq, void* stream);
 

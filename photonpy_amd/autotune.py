@@ -1,14 +1,16 @@
-"""Autotuner for the fit kernels (no reference counterpart).
+"""Autotuner for the fit kernels and the mixed batch (no reference counterpart).
 
-Two knobs are tuned per (model, ROI size) and stored in ``tuning.json`` next
-to this file:
+Knobs tuned per (model, ROI size) and stored in ``tuning.json`` next to this file:
 
-* ``group``            lanes per spot of the fit kernel (a compiled power of two)
-* ``spots_per_group``  tile length of the persistent mixed batch, in spots per lane group
+* ``group``    lanes per spot of the fit kernel (a compiled power of two); with the
+               block size it sets the spots per block and, for the grid-form spline,
+               the LDS window tables per block
+* ``refill``   stopped lane groups of a wavefront that are refilled together
 
-``tune_group`` times the compiled group sizes of an estimator on device-resident
-samples and keeps the fastest; ``tune_batch`` times candidate tile lengths of a
-MixedBatch.  ``apply`` sets an estimator's tuned group size.
+and for a mixed batch ``streams``, the number of concurrent branches.
+``tune_group`` times the compiled group sizes x refill batches of an estimator on
+device-resident samples and keeps the fastest; ``tune_streams`` times branch counts
+of a MixedBatch.  ``apply`` sets an estimator's tuned values.
 """
 from __future__ import annotations
 
@@ -25,13 +27,25 @@ COMPILED_GROUPS = {3: [1], 4: [1], 5: [2], 6: [2], 7: [1, 2, 4], 8: [2], 9: [2, 
                    12: [4], 13: [4, 8], 14: [4], 15: [4], 16: [4, 8], 17: [8], 18: [8], 19: [8], 20: [8, 16],
                    21: [8], 22: [8], 23: [8], 24: [8], 25: [8, 16], 26: [16], 27: [16], 28: [16], 29: [16],
                    30: [16], 31: [16], 32: [16]}
+# grid-form spline (PPA_GRID_SIZES)
+GRID_GROUPS = {3: [4], 4: [4], 5: [4], 6: [4], 7: [4, 8], 8: [8], 9: [8, 16], 10: [8], 11: [8, 16], 12: [8],
+               13: [8, 16], 14: [16], 15: [16], 16: [16, 32], 17: [16], 18: [16], 19: [16], 20: [16, 32],
+               21: [16], 22: [16], 23: [16], 24: [16], 25: [16, 32], 26: [16], 27: [16], 28: [16], 29: [16],
+               30: [16], 31: [16], 32: [16]}
 
 
 def key(est):
     kind = _KIND.get(est.fmt)
     if kind is None:
-        kind = "spline" if est.num_shared() > 10 else "astig"
+        kind = "astig"
+        if est.num_shared() > 10:
+            kind = "splinegrid" if est.spline_form() == "grid" else "spline"
     return f"{kind}:{int(est.sampleshape[0])}"
+
+
+def compiled_groups(est):
+    R = int(est.sampleshape[0])
+    return (GRID_GROUPS if key(est).startswith("splinegrid") else COMPILED_GROUPS).get(R, [])
 
 
 def load():
@@ -44,10 +58,6 @@ def load():
 def save(table):
     with open(TABLE, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
-
-
-def lookup_spots_per_group(est, default=8):
-    return int(load().get(key(est), {}).get("spots_per_group", default))
 
 
 def apply(est):
@@ -76,9 +86,8 @@ def _time(fn, reps=3):
 def tune_group(est, samples, refills=(0, 1, 2, 4, 8), persist=True):
     """Time every compiled group size x refill batch for this estimator's ROI size on
     device samples [N,R,R]; keep the fastest pair."""
-    R = int(est.sampleshape[0])
     results = {}
-    for g in COMPILED_GROUPS.get(R, []):
+    for g in compiled_groups(est):
         est.SetGroupSize(g)
         for rf in refills:
             if rf > 64 // g:
@@ -97,26 +106,29 @@ def tune_group(est, samples, refills=(0, 1, 2, 4, 8), persist=True):
     return best, results
 
 
-def tune_batch(estimators, samples, candidates=(2, 4, 8, 16, 32), persist=True):
-    """Time tile lengths (spots per lane group) of the persistent batch; one length for all bins."""
+def tune_streams(estimators, samples, candidates=(1, 2, 4, 8), persist=True):
+    """Time branch counts of a MixedBatch over these bins (graph replay); keep the fastest."""
     from .batch import MixedBatch
     counts = [s.shape[0] for s in samples]
     results = {}
-    for spg in candidates:
-        with MixedBatch(estimators, counts, [spg] * len(estimators)) as mb:
+    for ns in candidates:
+        with MixedBatch(estimators, counts, streams=ns, tune=False) as mb:
             outs = mb.allocate_outputs()
-            results[spg] = _time(lambda: mb.run(samples, outs))
+            results[ns] = _time(lambda: mb.run(samples, outs))
     best = min(results, key=results.get)
     if persist:
         t = load()
-        for e in estimators:
-            t.setdefault(key(e), {})["spots_per_group"] = int(best)
+        t["mixed_batch"] = {"streams": int(best), "seconds": {str(k): v for k, v in results.items()}}
         save(t)
     return best, results
 
 
+def batch_streams(default=4):
+    return int(load().get("mixed_batch", {}).get("streams", default))
+
+
 def main():
-    """python -m photonpy_amd.autotune [--configs c1,c2,c3,c4] [--n 200000]"""
+    """python -m photonpy_amd.autotune [--configs c1,c2,c3,c4] [--n 200000] [--batch-n 400000]"""
     import argparse
     import torch
     from . import Context
@@ -124,9 +136,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--configs", default="c1,c2,c3,c4")
     ap.add_argument("--n", type=int, default=200_000)
+    ap.add_argument("--batch-n", type=int, default=0, help="also tune the mixed batch's stream count on this many ROIs")
     args = ap.parse_args()
     with Context() as ctx:
-        for c in args.configs.split(","):
+        for c in [c for c in args.configs.split(",") if c]:
             w = synthetic.WORKLOADS[c]
             est = synthetic.make_estimator(ctx, w["kind"], w["R"])
             est.SetLevMarParams(w["lam"], w["iters"])
@@ -136,6 +149,13 @@ def main():
                   flush=True)
             del smp
             torch.cuda.empty_cache()
+        if args.batch_n:
+            bins = synthetic.mixed_bins(ctx, n=args.batch_n)
+            for b in bins:
+                apply(b["est"])
+            best, res = tune_streams([b["est"] for b in bins], [b["samples"] for b in bins])
+            print("mixed batch streams best", best, {k: round(args.batch_n / v / 1e6, 2) for k, v in res.items()},
+                  flush=True)
 
 
 if __name__ == "__main__":

@@ -1,41 +1,40 @@
-"""Persistent mixed-model, mixed-size batch (no reference counterpart).
+"""Mixed-model, mixed-size batch (no reference counterpart).
 
 A ``MixedBatch`` fits several bins of spots, one bin per estimator (model and
-ROI size), in one persistent kernel launch replayed from a HIP graph.
-``run_plain`` fits the same bins with one ordinary launch per bin, which is
-the comparison form.  Tile sizes come from ``photonpy_amd.autotune`` when a
-tuning table exists for the bin's (model, ROI size).
+ROI size).  The bins run concurrently on a few HIP streams (longest first), and
+with ``use_graph`` the whole fork / launches / join sequence is one HIP graph,
+replayed with a single launch.  ``run_plain`` fits the same bins one launch
+after another on one stream, which is the comparison form.  Each estimator's
+lane-group size and refill batch come from ``photonpy_amd.autotune`` when a
+tuning table entry exists for its (model, ROI size).
 """
 from __future__ import annotations
 
 import ctypes as ct
 
-import numpy as np
-
 
 class MixedBatch:
-    def __init__(self, estimators, counts, spots_per_group=None, blocks=0):
+    def __init__(self, estimators, counts, streams=4, tune=True):
         if len(estimators) != len(counts) or not estimators:
             raise ValueError("one spot count per estimator is required")
         self.estimators = list(estimators)
         self.counts = [int(c) for c in counts]
         self._lib = estimators[0]._lib
         self._smlm = estimators[0]._smlm
+        if tune:
+            from .autotune import apply
+            for e in self.estimators:
+                apply(e)
         nb = len(estimators)
-        if spots_per_group is None:
-            from .autotune import lookup_spots_per_group
-            spots_per_group = [lookup_spots_per_group(e) for e in estimators]
-        self.spots_per_group = [int(s) for s in spots_per_group]
         arr = (ct.c_void_p * nb)(*[e.inst for e in estimators])
         cnt = (ct.c_int * nb)(*self.counts)
-        spg = (ct.c_int * nb)(*self.spots_per_group)
-        self.inst = self._lib.MixedBatch_Create(nb, arr, cnt, spg, int(blocks))
+        self.inst = self._lib.MixedBatch_Create(nb, arr, cnt, int(streams))
         if not self.inst:
             self._smlm.check()
             raise RuntimeError("MixedBatch_Create failed")
 
-    def num_tiles(self):
-        return self._lib.MixedBatch_NumTiles(self.inst)
+    def num_streams(self):
+        return self._lib.MixedBatch_NumStreams(self.inst)
 
     def allocate_outputs(self, device="cuda", crlb=True):
         """Per-bin output tensors (params, iterations, crlb, chisq)."""

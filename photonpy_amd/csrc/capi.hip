@@ -4,6 +4,7 @@
 // caller's stream.  Everything computes on the GPU: there is no CPU path.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <algorithm>
@@ -16,7 +17,6 @@
 
 #include "../../include/photonpy_amd.h"
 #include "dispatch.h"
-#include "mixed.h"
 
 namespace ppa {
 hipError_t launch_spline_coefs(const int* shape, const float* d_data, float* d_out, int border, hipStream_t st);
@@ -56,6 +56,12 @@ struct Estimator {
     // spline coefficients: host copy (owned) and lazily uploaded device copy
     std::vector<float> hostCoefs;
     float* devCoefs = nullptr;
+    // grid form of the spline (the samples the table interpolates), when the table
+    // is a CSpline_Compute interpolant; used unless the caller asks for coefficients
+    std::vector<float> hostGrid;
+    float* devGrid = nullptr;
+    bool gridForm = false;     // table verified to be a grid interpolant
+    int splineMode = 0;        // 0: grid form when available, 1: always coefficients
     // grow-only device workspace for the host-pointer API
     void* ws = nullptr;
     size_t wsBytes = 0;
@@ -77,8 +83,12 @@ struct Estimator {
         phaseEvents.clear();
     }
 
-    int* next_counter() {
+    void ensure_counters() {
         if (!devCounters) hip_check(hipMalloc(&devCounters, kCounterSlots * sizeof(int)), "hipMalloc(counters)");
+    }
+
+    int* next_counter() {
+        ensure_counters();
         return devCounters + (counterSlot++ % kCounterSlots);
     }
 
@@ -111,10 +121,11 @@ struct Estimator {
         return devShared;
     }
     int S() const { return roisize * roisize; }
-    int G() const { return group > 0 ? group : default_group(roisize); }
+    int G() const { return group > 0 ? group : default_group_for(desc, roisize); }
 
     ~Estimator() {
         if (devCoefs) (void)hipFree(devCoefs);
+        if (devGrid) (void)hipFree(devGrid);
         if (devShared) (void)hipFree(devShared);
         if (devCounters) (void)hipFree(devCounters);
         if (ws) (void)hipFree(ws);
@@ -127,6 +138,12 @@ struct Estimator {
             hip_check(hipMalloc(&devCoefs, bytes), "hipMalloc(spline coefficients)");
             hip_check(hipMemcpy(devCoefs, hostCoefs.data(), bytes, hipMemcpyHostToDevice), "upload spline coefficients");
             desc.sm.coefs = devCoefs;
+        }
+        if (desc.kind == kKindSpline && desc.useGrid && !devGrid) {
+            const size_t bytes = hostGrid.size() * sizeof(float);
+            hip_check(hipMalloc(&devGrid, bytes), "hipMalloc(spline grid)");
+            hip_check(hipMemcpy(devGrid, hostGrid.data(), bytes, hipMemcpyHostToDevice), "upload spline grid");
+            desc.sg.grid = devGrid;
         }
     }
 
@@ -171,6 +188,84 @@ static void require_gpu(bool cuda) {
         throw std::runtime_error("photonpy_amd: CPU estimators are not provided; create the estimator with cuda=True");
 }
 
+// Samples whose local tricubic interpolant a spline table is.  Along each axis
+// cell c of a CSpline_Compute table interpolates samples g[c..c+3] at the local
+// nodes t = -1, 0, 1, 2 (CubicSplinePSF.cu:151-204), so sample d is the value of
+// cell clamp(d-1, 0, N-1) at node d - c - 1.  The samples are read off the
+// polynomials in FP64; the table is accepted when every cell's coefficients,
+// recomputed from the samples with the exact inverse of the 4x4 Vandermonde
+// factor, agree with the given ones to 1e-5 of the largest coefficient (FP32
+// rounding of coefficients and samples stays below 1e-6).  Tables made any other
+// way (smoothing splines) fail the check and keep the coefficient form.
+static bool spline_grid_from_coefs(int Nz, int Ny, int Nx, const float* c, std::vector<float>& grid) {
+    const int D[3] = {Nz + 3, Ny + 3, Nx + 3}, N[3] = {Nz, Ny, Nx};
+    grid.assign((size_t)D[0] * D[1] * D[2], 0.0f);
+    auto cellOf = [&](int axis, int d, int& cell, double& t) {
+        cell = std::min(std::max(d - 1, 0), N[axis] - 1);
+        t = (double)(d - cell - 1);
+    };
+    for (int dz = 0; dz < D[0]; dz++)
+        for (int dy = 0; dy < D[1]; dy++)
+            for (int dx = 0; dx < D[2]; dx++) {
+                int cz, cy, cx;
+                double tz, ty, tx;
+                cellOf(0, dz, cz, tz);
+                cellOf(1, dy, cy, ty);
+                cellOf(2, dx, cx, tx);
+                const float* cc = c + 64 * (((size_t)cz * Ny + cy) * Nx + cx);
+                double v = 0.0, pz = 1.0;
+                for (int m = 0; m < 4; m++, pz *= tz) {
+                    double py = 1.0;
+                    for (int n = 0; n < 4; n++, py *= ty) {
+                        double px = 1.0;
+                        for (int o = 0; o < 4; o++, px *= tx) v += (double)cc[16 * m + 4 * n + o] * pz * py * px;
+                    }
+                }
+                grid[((size_t)dz * D[1] + dy) * D[2] + dx] = (float)v;
+            }
+    // inverse Vandermonde factor: coefficient p of the cubic through nodes -1, 0, 1, 2
+    static const double Vi[4][4] = {{0.0, 1.0, 0.0, 0.0},
+                                    {-1.0 / 3.0, -0.5, 1.0, -1.0 / 6.0},
+                                    {0.5, -1.0, 0.5, 0.0},
+                                    {-1.0 / 6.0, 0.5, -0.5, 1.0 / 6.0}};
+    double cmax = 0.0;
+    const size_t ncoef = (size_t)Nz * Ny * Nx * 64;
+    for (size_t i = 0; i < ncoef; i++) cmax = std::max(cmax, std::fabs((double)c[i]));
+    const double tol = 1e-5 * cmax;
+    for (int cz = 0; cz < Nz; cz++)
+        for (int cy = 0; cy < Ny; cy++)
+            for (int cx = 0; cx < Nx; cx++) {
+                double b[64], t1[64], t2[64];
+                for (int m = 0; m < 4; m++)
+                    for (int n = 0; n < 4; n++)
+                        for (int q = 0; q < 4; q++)
+                            b[16 * m + 4 * n + q] = grid[(((size_t)cz + m) * D[1] + cy + n) * D[2] + cx + q];
+                for (int m = 0; m < 4; m++)
+                    for (int n = 0; n < 4; n++)
+                        for (int p = 0; p < 4; p++) {
+                            double a = 0.0;
+                            for (int q = 0; q < 4; q++) a += Vi[p][q] * b[16 * m + 4 * n + q];
+                            t1[16 * m + 4 * n + p] = a;
+                        }
+                for (int m = 0; m < 4; m++)
+                    for (int p = 0; p < 4; p++)
+                        for (int q = 0; q < 4; q++) {
+                            double a = 0.0;
+                            for (int n = 0; n < 4; n++) a += Vi[p][n] * t1[16 * m + 4 * n + q];
+                            t2[16 * m + 4 * p + q] = a;
+                        }
+                const float* cc = c + 64 * (((size_t)cz * Ny + cy) * Nx + cx);
+                for (int p = 0; p < 4; p++)
+                    for (int n = 0; n < 4; n++)
+                        for (int q = 0; q < 4; q++) {
+                            double a = 0.0;
+                            for (int m = 0; m < 4; m++) a += Vi[p][m] * t2[16 * m + 4 * n + q];
+                            if (!(std::fabs(a - (double)cc[16 * p + 4 * n + q]) <= tol)) return false;
+                        }
+            }
+    return true;
+}
+
 static Estimator* make_gauss(int kind, int roisize, const GaussCalib& gc) {
     if (roisize < 1) throw std::runtime_error("photonpy_amd: roisize must be positive");
     auto* e = new Estimator();
@@ -195,7 +290,7 @@ static Estimator* make_gauss(int kind, int roisize, const GaussCalib& gc) {
 static hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 static void run_fit(Estimator* e, int n, const float* d_sample, const float* d_initial, float* d_params, int* d_iter,
-                    float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st) {
+                    float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st, bool timing = true) {
     e->ensure_device();
     const int R = e->roisize;
     if (R < kMinFitR || R > kMaxFitR)
@@ -216,7 +311,7 @@ static void run_fit(Estimator* e, int n, const float* d_sample, const float* d_i
     a.refill = e->refill;
     a.counter = e->next_counter();
     hipEvent_t* events = nullptr;
-    if (e->phaseTiming) {
+    if (e->phaseTiming && timing) {
         for (int i = 0; i < 3; i++) {
             hipEvent_t ev;
             hip_check(hipEventCreate(&ev), "hipEventCreate");
@@ -364,6 +459,15 @@ Estimator* CSpline_CreatePSF(int roisize, const CSpline_Calibration* calib, cons
             e->desc.sm.zmax = calib->z_max;
             e->desc.sm.coefs = nullptr;
             e->hostCoefs.assign(src, src + (size_t)Nz * Ny * Nx * 64);  // copied, as CubicSplinePSF.cu:48-52
+            e->desc.sg.Nz = Nz;
+            e->desc.sg.Ny = Ny;
+            e->desc.sg.Nx = Nx;
+            e->desc.sg.zmin = calib->z_min;
+            e->desc.sg.zmax = calib->z_max;
+            e->desc.sg.grid = nullptr;
+            e->gridForm = spline_grid_from_coefs(Nz, Ny, Nx, e->hostCoefs.data(), e->hostGrid);
+            if (!e->gridForm) e->hostGrid.clear();
+            e->desc.useGrid = e->gridForm;
             e->fmt = "x,y,z,I,bg";
             const float b = 1.5f, hi = (float)roisize - 1 - b;
             e->limits = {{b, hi}, {b, hi}, {calib->z_min, calib->z_max}, {10.0f, 1e9f}, {1e-6f, 1e9f}};
@@ -448,6 +552,19 @@ int Estimator_SetRefill(Estimator* e, int refill) {
     if (!e || refill < 0 || refill > 64) return 1;
     e->refill = refill;
     return 0;
+}
+
+int Estimator_SetSplineMode(Estimator* e, int mode) {
+    if (!e || e->desc.kind != kKindSpline || mode < 0 || mode > 1) return 1;
+    std::lock_guard<std::mutex> lk(e->mtx);
+    e->splineMode = mode;
+    e->desc.useGrid = e->gridForm && mode == 0;
+    return 0;
+}
+
+int Estimator_SplineForm(Estimator* e) {
+    if (!e || e->desc.kind != kKindSpline) return -1;
+    return e->desc.useGrid ? 1 : 0;
 }
 
 int Estimator_SetPhaseTiming(Estimator* e, int enable) {
@@ -679,86 +796,113 @@ int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const fl
 // ---------------------------------------------------------------- mixed batch
 struct MixedBatch {
     std::vector<Estimator*> est;
-    std::vector<int> counts, groups, variants;
-    std::vector<MixedTile> tiles;
-    MixedBin* hBins = nullptr;  // pinned staging of the bin table
-    MixedBin* dBins = nullptr;
-    MixedTile* dTiles = nullptr;
-    int* dCounter = nullptr;
-    int blocks = 0;
+    std::vector<int> counts;
+    std::vector<int> branch;              // stream of each bin
+    std::vector<std::vector<int>> order;  // bins of each stream, costliest first
+    std::vector<hipStream_t> streams;     // one per branch
+    std::vector<hipEvent_t> joins;        // one per branch
+    hipEvent_t fork = nullptr;
     hipStream_t capStream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipEvent_t copied = nullptr;
-    bool copyPending = false;
+    hipStream_t lastLaunch = nullptr;       // stream of the last graph launch
+    std::vector<const void*> capturedPtrs;  // buffers the graph was captured with
 
-    ~MixedBatch() {
+    void drop_graph() {
+        if (exec) (void)hipStreamSynchronize(lastLaunch);  // the last replay has finished
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        capturedPtrs.clear();
+    }
+
+    ~MixedBatch() {
+        drop_graph();
+        for (hipStream_t s : streams) (void)hipStreamSynchronize(s);
+        for (hipStream_t s : streams) (void)hipStreamDestroy(s);
+        for (hipEvent_t ev : joins) (void)hipEventDestroy(ev);
+        if (fork) (void)hipEventDestroy(fork);
         if (capStream) (void)hipStreamDestroy(capStream);
-        if (copied) (void)hipEventDestroy(copied);
-        if (hBins) (void)hipHostFree(hBins);
-        if (dBins) (void)hipFree(dBins);
-        if (dTiles) (void)hipFree(dTiles);
-        if (dCounter) (void)hipFree(dCounter);
     }
 };
 
-// relative cost of one spot-iteration per model (pixel work), for longest-first tile order
-static double model_cost(int kind) {
-    switch (kind) {
-    case kKindFixed: return 1.0;
-    case kKindSigma: return 1.3;
-    case kKindAstig: return 1.4;
-    default: return 5.0;
+// Relative cost of one pixel-iteration per model (grid spline included), for the
+// longest-first assignment of bins to branches.
+static double bin_cost(const Estimator* e, int count) {
+    double c = 1.0;
+    switch (e->desc.kind) {
+    case kKindFixed: c = 1.0; break;
+    case kKindSigma: c = 1.3; break;
+    case kKindAstig: c = 1.4; break;
+    default: c = e->desc.useGrid ? 4.0 : 40.0; break;
+    }
+    return c * (double)count * e->roisize * e->roisize;
+}
+
+// Launches every bin: each branch waits for `fork` on its stream, runs its bins in
+// order, and records its join event; the caller's stream waits for all joins.
+static void launch_bins(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
+                        float* const* d_crlb, float* const* d_chisq, hipStream_t origin) {
+    hip_check(hipEventRecord(mb->fork, origin), "hipEventRecord(fork)");
+    for (size_t b = 0; b < mb->streams.size(); b++) {
+        hipStream_t s = mb->streams[b];
+        hip_check(hipStreamWaitEvent(s, mb->fork, 0), "hipStreamWaitEvent(fork)");
+        for (int bin : mb->order[b]) {
+            Estimator* e = mb->est[bin];
+            std::lock_guard<std::mutex> l(e->mtx);
+            run_fit(e, mb->counts[bin], d_samples[bin], nullptr, d_params[bin], d_iters ? d_iters[bin] : nullptr,
+                    nullptr, 0, d_crlb ? d_crlb[bin] : nullptr, d_chisq ? d_chisq[bin] : nullptr, s, false);
+        }
+        hip_check(hipEventRecord(mb->joins[b], s), "hipEventRecord(join)");
+        hip_check(hipStreamWaitEvent(origin, mb->joins[b], 0), "hipStreamWaitEvent(join)");
     }
 }
 
 extern "C" {
 
-MixedBatch* MixedBatch_Create(int nbins, Estimator* const* ests, const int* counts, const int* spotsPerGroup,
-                              int blocks) {
+MixedBatch* MixedBatch_Create(int nbins, Estimator* const* ests, const int* counts, int nstreams) {
     return guarded(
         [&]() -> MixedBatch* {
             if (nbins <= 0 || !ests || !counts) throw std::runtime_error("photonpy_amd: empty mixed batch");
+            if (nstreams <= 0) nstreams = 4;
+            nstreams = std::min(nstreams, nbins);
             auto* mb = new MixedBatch();
             try {
-                struct Item { MixedTile t; double cost; };
-                std::vector<Item> items;
+                std::vector<std::pair<double, int>> cost;
                 for (int b = 0; b < nbins; b++) {
                     Estimator* e = ests[b];
                     if (!e) throw std::runtime_error("photonpy_amd: NULL estimator in mixed batch");
-                    const int kind = e->desc.kind, R = e->roisize;
-                    int G = e->group > 0 ? e->group : mixed_default_group(kind, R);
-                    const int v = G > 0 ? mixed_variant(kind, R, G) : -1;
-                    if (v < 0)
-                        throw std::runtime_error("photonpy_amd: the persistent batch kernel has no variant for roisize " +
-                                                 std::to_string(R) + " (compiled: 7, 9, 11, 13, 16, 20, 25)");
+                    if (counts[b] < 0) throw std::runtime_error("photonpy_amd: negative spot count");
+                    const int R = e->roisize;
+                    if (R < kMinFitR || R > kMaxFitR)
+                        throw std::runtime_error("photonpy_amd: fit kernels are compiled for roisize " +
+                                                 std::to_string(kMinFitR) + ".." + std::to_string(kMaxFitR) +
+                                                 ", got " + std::to_string(R));
                     mb->est.push_back(e);
                     mb->counts.push_back(counts[b]);
-                    mb->groups.push_back(G);
-                    mb->variants.push_back(v);
-                    const int spg = spotsPerGroup && spotsPerGroup[b] > 0 ? spotsPerGroup[b] : 8;
-                    const int tile = (kWave / G) * spg;
-                    const double c = model_cost(kind) * R * R;
-                    for (int s0 = 0; s0 < counts[b]; s0 += tile) {
-                        const int s1 = std::min(counts[b], s0 + tile);
-                        items.push_back({{b, s0, s1}, c * (s1 - s0) / (kWave / G)});
-                    }
+                    cost.push_back({bin_cost(e, counts[b]), b});
                 }
-                std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
-                for (auto& it : items) mb->tiles.push_back(it.t);
-                hip_check(hipHostMalloc(&mb->hBins, sizeof(MixedBin) * nbins, hipHostMallocDefault), "hipHostMalloc");
-                hip_check(hipMalloc(&mb->dBins, sizeof(MixedBin) * nbins), "hipMalloc");
-                hip_check(hipMalloc(&mb->dCounter, sizeof(int)), "hipMalloc");
-                if (!mb->tiles.empty()) {
-                    hip_check(hipMalloc(&mb->dTiles, sizeof(MixedTile) * mb->tiles.size()), "hipMalloc");
-                    hip_check(hipMemcpy(mb->dTiles, mb->tiles.data(), sizeof(MixedTile) * mb->tiles.size(),
-                                        hipMemcpyHostToDevice), "upload tiles");
+                // longest-processing-time-first onto the least-loaded branch
+                std::stable_sort(cost.begin(), cost.end(), [](auto& x, auto& y) { return x.first > y.first; });
+                std::vector<double> load(nstreams, 0.0);
+                mb->branch.assign(nbins, 0);
+                mb->order.assign(nstreams, {});
+                for (auto& c : cost) {
+                    const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                    load[s] += c.first;
+                    mb->branch[c.second] = s;
+                    mb->order[s].push_back(c.second);
                 }
-                hip_check(hipEventCreateWithFlags(&mb->copied, hipEventDisableTiming), "hipEventCreate");
-                mb->blocks = blocks > 0 ? blocks : mixed_resident_blocks();
-                if (mb->blocks <= 0) throw std::runtime_error("photonpy_amd: no HIP device");
+                for (int i = 0; i < nstreams; i++) {
+                    hipStream_t st;
+                    hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+                    mb->streams.push_back(st);
+                    hipEvent_t ev;
+                    hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+                    mb->joins.push_back(ev);
+                }
+                hip_check(hipEventCreateWithFlags(&mb->fork, hipEventDisableTiming), "hipEventCreate");
             } catch (...) {
                 delete mb;
                 throw;
@@ -770,75 +914,62 @@ MixedBatch* MixedBatch_Create(int nbins, Estimator* const* ests, const int* coun
 
 void MixedBatch_Delete(MixedBatch* mb) { delete mb; }
 
-int MixedBatch_NumTiles(MixedBatch* mb) { return mb ? (int)mb->tiles.size() : -1; }
-
-static void fill_bins(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
-                      float* const* d_crlb, float* const* d_chisq) {
-    if (mb->copyPending) {
-        hip_check(hipEventSynchronize(mb->copied), "wait for bin table copy");
-        mb->copyPending = false;
-    }
-    for (size_t b = 0; b < mb->est.size(); b++) {
-        Estimator* e = mb->est[b];
-        e->ensure_device();
-        MixedBin& B = mb->hBins[b];
-        std::memset(&B, 0, sizeof(B));
-        B.variant = mb->variants[b];
-        B.kind = e->desc.kind;
-        B.gc = e->desc.gc;
-        B.sm = e->desc.sm;
-        B.a.n = mb->counts[b];
-        B.a.sample = d_samples[b];
-        B.a.params = d_params[b];
-        B.a.iterations = d_iters ? d_iters[b] : nullptr;
-        B.a.crlb = d_crlb ? d_crlb[b] : nullptr;
-        B.a.chisq = d_chisq ? d_chisq[b] : nullptr;
-        B.a.maxIter = e->lm.iterations;
-        B.a.lambda = e->lm.lambda;
-        B.a.refill = e->refill;
-        if (!B.a.sample || !B.a.params) throw std::runtime_error("photonpy_amd: NULL sample or params buffer");
-    }
-}
+int MixedBatch_NumStreams(MixedBatch* mb) { return mb ? (int)mb->streams.size() : -1; }
 
 int MixedBatch_Run(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
                    float* const* d_crlb, float* const* d_chisq, int useGraph, void* stream) {
     if (!mb) return 1;
     return guarded(
         [&]() -> int {
-            // Switched off: one GPU run of this launch (an earlier build of the kernel) never
-            // finished and its cause has not been shown; MixedBatch_RunPlain is the working form.
-            throw std::runtime_error("photonpy_amd: the persistent mixed-batch launch is switched off "
-                                         "(it hung once on the GPU; cause not yet shown); use MixedBatch_RunPlain");
-#if 0
             hipStream_t st = as_stream(stream);
-            fill_bins(mb, d_samples, d_params, d_iters, d_crlb, d_chisq);
-            hip_check(hipMemcpyAsync(mb->dBins, mb->hBins, sizeof(MixedBin) * mb->est.size(), hipMemcpyHostToDevice, st),
-                      "upload bin table");
-            hip_check(hipEventRecord(mb->copied, st), "hipEventRecord");
-            mb->copyPending = true;
-            const int ntiles = (int)mb->tiles.size();
+            const size_t nb = mb->est.size();
+            for (size_t b = 0; b < nb; b++)
+                if (mb->counts[b] > 0 && (!d_samples[b] || !d_params[b]))
+                    throw std::runtime_error("photonpy_amd: NULL sample or params buffer");
+            for (Estimator* e : mb->est) {
+                // device tables and counters exist before any capture (no allocation inside it)
+                std::lock_guard<std::mutex> l(e->mtx);
+                e->ensure_device();
+                e->ensure_counters();
+            }
             if (!useGraph) {
-                hip_check(launch_mixed(mb->dBins, mb->dTiles, ntiles, mb->dCounter, mb->blocks, st), "mixed kernel");
+                launch_bins(mb, d_samples, d_params, d_iters, d_crlb, d_chisq, st);
                 return 0;
             }
-            if (!mb->exec) {
-                // capture once on a private stream (the caller's may be the legacy default stream)
-                hip_check(hipStreamCreateWithFlags(&mb->capStream, hipStreamNonBlocking), "hipStreamCreate");
-                hip_check(hipStreamBeginCapture(mb->capStream, hipStreamCaptureModeThreadLocal), "begin capture");
-                hipError_t le = launch_mixed(mb->dBins, mb->dTiles, ntiles, mb->dCounter, mb->blocks, mb->capStream);
-                hipError_t ce = hipStreamEndCapture(mb->capStream, &mb->graph);
-                hip_check(le, "mixed kernel (capture)");
-                hip_check(ce, "end capture");
+            std::vector<const void*> ptrs;
+            for (size_t b = 0; b < nb; b++) {
+                ptrs.push_back(d_samples[b]);
+                ptrs.push_back(d_params[b]);
+                ptrs.push_back(d_iters ? d_iters[b] : nullptr);
+                ptrs.push_back(d_crlb ? d_crlb[b] : nullptr);
+                ptrs.push_back(d_chisq ? d_chisq[b] : nullptr);
+            }
+            if (!mb->exec || ptrs != mb->capturedPtrs) {
+                // the kernels' buffers are baked into the graph: capture again for new ones
+                mb->drop_graph();
+                if (!mb->capStream) hip_check(hipStreamCreateWithFlags(&mb->capStream, hipStreamNonBlocking),
+                                              "hipStreamCreate");
+                hip_check(hipStreamBeginCapture(mb->capStream, hipStreamCaptureModeRelaxed), "begin capture");
+                try {
+                    launch_bins(mb, d_samples, d_params, d_iters, d_crlb, d_chisq, mb->capStream);
+                } catch (...) {
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(mb->capStream, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                hip_check(hipStreamEndCapture(mb->capStream, &mb->graph), "end capture");
                 hip_check(hipGraphInstantiate(&mb->exec, mb->graph, nullptr, nullptr, 0), "graph instantiate");
+                mb->capturedPtrs = ptrs;
             }
             hip_check(hipGraphLaunch(mb->exec, st), "graph launch");
+            mb->lastLaunch = st;
             return 0;
-#endif
         },
         2);
 }
 
-// The same batch as one plain launch per bin (Estimator_EstimateDevice per model and size).
+// The same bins as one plain launch per bin, one after another on the caller's stream.
 int MixedBatch_RunPlain(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
                         float* const* d_crlb, float* const* d_chisq, void* stream) {
     if (!mb) return 1;
@@ -848,7 +979,7 @@ int MixedBatch_RunPlain(MixedBatch* mb, const float* const* d_samples, float* co
                 Estimator* e = mb->est[b];
                 std::lock_guard<std::mutex> l(e->mtx);
                 run_fit(e, mb->counts[b], d_samples[b], nullptr, d_params[b], d_iters ? d_iters[b] : nullptr, nullptr,
-                        0, d_crlb ? d_crlb[b] : nullptr, d_chisq ? d_chisq[b] : nullptr, as_stream(stream));
+                        0, d_crlb ? d_crlb[b] : nullptr, d_chisq ? d_chisq[b] : nullptr, as_stream(stream), false);
             }
             return 0;
         },

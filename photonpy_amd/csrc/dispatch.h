@@ -13,7 +13,9 @@ enum ModelKind { kKindFixed = 0, kKindSigma = 1, kKindAstig = 2, kKindSpline = 3
 struct ModelDesc {
     int kind;
     GaussCalib gc;
-    SplineModel sm;  // coefs must point to device memory when launched
+    SplineModel sm;      // coefs must point to device memory when launched
+    SplineGridModel sg;  // grid form of the same spline (when grid != null)
+    bool useGrid;        // spline: fit / evaluate from sg instead of sm
 };
 
 inline int num_params(int kind) { return kind == kKindFixed ? 4 : 5; }
@@ -28,6 +30,18 @@ inline int default_group(int R) {
     return 16;
 }
 
+// Lane-group size of the grid-form spline fit (FlatLayout: any G divides the
+// pixels evenly enough; larger groups keep the per-group LDS window per lane small).
+inline int default_group_grid(int R) {
+    if (R <= 6) return 4;
+    if (R <= 12) return 8;
+    return 16;
+}
+
+inline int default_group_for(const ModelDesc& d, int R) {
+    return (d.kind == kKindSpline && d.useGrid) ? default_group_grid(R) : default_group(R);
+}
+
 constexpr int kMinFitR = 3, kMaxFitR = 32;
 
 // Per-model launchers, defined in fit_<model>_<part>.hip.  Return hipErrorInvalidValue
@@ -37,6 +51,8 @@ hipError_t launch_fit_fixed(const ModelDesc& d, int R, int G, const FitArgs& a, 
 hipError_t launch_fit_sigma(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st);
 hipError_t launch_fit_astig(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st);
 hipError_t launch_fit_spline(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st);
+hipError_t launch_fit_splinegrid(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events,
+                                 hipStream_t st);
 
 hipError_t launch_ev_any(const ModelDesc& d, int R, int n, const float* params, float* ev, float* deriv, hipStream_t st);
 hipError_t launch_crlb_any(const ModelDesc& d, int R, int n, const float* params, const float* sample, float* crlb,
@@ -63,18 +79,29 @@ inline hipError_t launch_fit_any(const ModelDesc& d, int R, int G, const FitArgs
     case kKindFixed: return launch_fit_fixed(d, R, G, a, events, st);
     case kKindSigma: return launch_fit_sigma(d, R, G, a, events, st);
     case kKindAstig: return launch_fit_astig(d, R, G, a, events, st);
-    case kKindSpline: return launch_fit_spline(d, R, G, a, events, st);
+    case kKindSpline:
+        return d.useGrid ? launch_fit_splinegrid(d, R, G, a, events, st) : launch_fit_spline(d, R, G, a, events, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-// Blocks of a kernel that the whole device keeps resident (256 threads per block).
-inline int resident_blocks(const void* kernel) {
+// Blocks of a kernel that the whole device keeps resident (block threads per
+// block, ldsBytes of dynamic LDS each).
+inline int resident_blocks(const void* kernel, int block, size_t ldsBytes) {
     int dev = 0, perCU = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, 256, 0) != hipSuccess || perCU < 1) perCU = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, block, ldsBytes) != hipSuccess || perCU < 1)
+        perCU = 1;
     return perCU * prop.multiProcessorCount;
+}
+
+// Threads per block of a fit kernel: 256, or fewer for models with per-group LDS
+// tables so that one block's tables stay within 64 KiB.
+constexpr int fit_block(int G, int ldsFloatsPerGroup) {
+    return (ldsFloatsPerGroup == 0 || (256 / G) * ldsFloatsPerGroup * 4 <= 65536)   ? 256
+           : ((128 / G) * ldsFloatsPerGroup * 4 <= 65536 && 128 >= G)               ? 128
+                                                                                     : 64;
 }
 
 // Persistent grids: enough blocks to cover every group once, capped at what the
@@ -82,11 +109,13 @@ inline int resident_blocks(const void* kernel) {
 // blockIdx * groupsPerBlock + group and hands out the rest from the counter.
 template <class M, int R, int G>
 hipError_t launch_fit_t(const M& m, const FitArgs& a, hipEvent_t* events, hipStream_t st) {
-    constexpr int kBlock = 256;
+    constexpr int kBlock = fit_block(G, M::lds_floats(R));
+    constexpr size_t kLds = (size_t)(kBlock / G) * M::lds_floats(R) * sizeof(float);
+    static_assert(kLds <= 65536, "per-group LDS table too large for one lane group per 64 lanes");
     static int residentFit = 0, residentCrlb = 0;  // per process; one device model (gfx950)
     if (residentFit == 0) {
-        residentFit = resident_blocks(reinterpret_cast<const void*>(fit_kernel<M, R, G>));
-        residentCrlb = resident_blocks(reinterpret_cast<const void*>(crlb_phase_kernel<M, R, G>));
+        residentFit = resident_blocks(reinterpret_cast<const void*>(fit_kernel<M, R, G>), kBlock, kLds);
+        residentCrlb = resident_blocks(reinterpret_cast<const void*>(crlb_phase_kernel<M, R, G>), kBlock, kLds);
         if (residentFit == 0 || residentCrlb == 0) return hipErrorNoDevice;
     }
     const long groupsPerBlock = kBlock / G;
@@ -98,10 +127,10 @@ hipError_t launch_fit_t(const M& m, const FitArgs& a, hipEvent_t* events, hipStr
         if (e != hipSuccess) return e;
     }
     if (events) (void)hipEventRecord(events[0], st);
-    hipLaunchKernelGGL((fit_kernel<M, R, G>), dim3((unsigned)fitBlocks), dim3(kBlock), 0, st, m, a);
+    hipLaunchKernelGGL((fit_kernel<M, R, G>), dim3((unsigned)fitBlocks), dim3(kBlock), kLds, st, m, a);
     if (events) (void)hipEventRecord(events[1], st);
     if (a.crlb || a.chisq)
-        hipLaunchKernelGGL((crlb_phase_kernel<M, R, G>), dim3((unsigned)crlbBlocks), dim3(kBlock), 0, st, m, a);
+        hipLaunchKernelGGL((crlb_phase_kernel<M, R, G>), dim3((unsigned)crlbBlocks), dim3(kBlock), kLds, st, m, a);
     if (events) (void)hipEventRecord(events[2], st);
     return hipGetLastError();
 }

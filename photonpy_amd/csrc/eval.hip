@@ -25,7 +25,7 @@ hipError_t launch_ev_any(const ModelDesc& d, int R, int n, const float* params, 
     case kKindFixed: return ev_t(GaussModel<kFixed>{d.gc}, R, n, params, ev, deriv, st);
     case kKindSigma: return ev_t(GaussModel<kSigma>{d.gc}, R, n, params, ev, deriv, st);
     case kKindAstig: return ev_t(GaussModel<kAstig>{d.gc}, R, n, params, ev, deriv, st);
-    case kKindSpline: return ev_t(d.sm, R, n, params, ev, deriv, st);
+    case kKindSpline: return d.useGrid ? ev_t(d.sg, R, n, params, ev, deriv, st) : ev_t(d.sm, R, n, params, ev, deriv, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -36,7 +36,9 @@ hipError_t launch_crlb_any(const ModelDesc& d, int R, int n, const float* params
     case kKindFixed: return crlb_t(GaussModel<kFixed>{d.gc}, R, n, params, sample, crlb, chisq, st);
     case kKindSigma: return crlb_t(GaussModel<kSigma>{d.gc}, R, n, params, sample, crlb, chisq, st);
     case kKindAstig: return crlb_t(GaussModel<kAstig>{d.gc}, R, n, params, sample, crlb, chisq, st);
-    case kKindSpline: return crlb_t(d.sm, R, n, params, sample, crlb, chisq, st);
+    case kKindSpline:
+        return d.useGrid ? crlb_t(d.sg, R, n, params, sample, crlb, chisq, st)
+                         : crlb_t(d.sm, R, n, params, sample, crlb, chisq, st);
     default: return hipErrorInvalidValue;
     }
 }

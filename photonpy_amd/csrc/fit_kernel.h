@@ -1,24 +1,26 @@
 // Batched Levenberg-Marquardt MLE fit, followed by CRLB / chi-square.
 //
 // Mapping (gfx950, wave64): a spot is fitted by an aligned group of G lanes of
-// one wavefront (G = 1..16, a power of two chosen per ROI size).  Lane g of a
-// group owns the rows [g*RL, g*RL+RL) of the R x R ROI and keeps those samples
-// in VGPRs for the whole fit, so HBM is read once per spot.  Per iteration
-// each lane evaluates its pixels, accumulates the packed normal equations
-// alpha (K(K+1)/2) and beta (K) in FP32, the group all-reduces them with lane
-// shuffles, and every lane solves the FP64 damped Cholesky system redundantly
-// (identical inputs give identical steps, so the group stays convergent).
+// one wavefront (G = 1..32, a power of two chosen per ROI size).  Each lane
+// keeps its share of the R x R samples in VGPRs for the whole fit (the model's
+// pixel layout says which: whole rows, or every G-th pixel), so HBM is read once
+// per spot.  Per iteration each lane evaluates its pixels, accumulates the packed
+// normal equations alpha (K(K+1)/2) and beta (K) in FP32, the group all-reduces
+// them with lane shuffles, and every lane solves the FP64 damped Cholesky system
+// redundantly (identical inputs give identical steps, so the group stays
+// convergent).  Models that stage per-spot tables in LDS get a region of
+// M::lds_floats(R) floats per group.
 //
 // Spots stop after different numbers of iterations.  A group whose spot has
 // stopped waits until `refill` groups of its wavefront (or every still-running
 // one) have stopped; those groups then write their results and load their next
-// spot of a static stride together, so the wavefront pays the partly-masked
-// refill once per batch of groups instead of once per spot.  Next spots come
-// from a work counter in HBM (one atomic per batch), so every group stays busy
-// until the batch is exhausted and the launch ends within about one spot's fit
-// of each other.  The CRLB and chi^2 are computed by a second launch that walks
-// all spots in lockstep (every group does exactly one spot per pass), reading
-// the samples and the fitted parameters back from HBM.
+// spots together, so the wavefront pays the partly-masked refill once per batch
+// of groups instead of once per spot.  Next spots come from a work counter in
+// HBM (one atomic per batch), so every group stays busy until the batch is
+// exhausted and the launch ends within about one spot's fit of each other.  The
+// CRLB and chi^2 are computed by a second kernel that walks all spots in
+// lockstep (every group does exactly one spot per pass), reading the samples and
+// the fitted parameters back from HBM.
 //
 // Semantics follow LevMarOptimize (Estimation.h:194-251): trace before the
 // step, damping, Cholesky break on a zero pivot, clamp, stop when i > 0 and
@@ -47,50 +49,53 @@ struct FitArgs {
 
 template <class M, int R, int G>
 struct FitTraits {
+    using L = typename M::template Layout<R, G>;
     static constexpr int K = M::K;
     static constexpr int S = R * R;
-    static constexpr int RL = (R + G - 1) / G;   // rows per lane
+    static constexpr int NS = L::NS;             // samples per lane
     static constexpr int NA = K * (K + 1) / 2;   // packed alpha
+    static constexpr int LDS = M::lds_floats(R); // LDS floats per group
 };
 
-// Loads rows [y0, y0+RL) of a spot into s and returns the lane's centre-of-mass
-// partial sums.  With CLAMP the stored samples are max(x, 1e-6) as the
-// Levenberg-Marquardt terms use them (`if (x < 1e-6) x = 1e-6`, which keeps a
-// NaN, Estimation.h:114); the sums use the raw values.  The fit kernel loads
-// with the non-temporal hint (a few groups refill at a time and each ROI is read
-// once); the CRLB pass, where every group loads at once, uses ordinary loads so
-// that the lines its lanes share stay in L2 between the per-column loads.
-template <int R, int RL, bool CLAMP, bool NT>
-PP_DEV void load_rows(const float* src, int y0, float (&s)[RL][R], float& sx, float& sy, float& sum) {
+// Loads lane g's samples of a spot (layout L) and returns the lane's
+// centre-of-mass partial sums.  With CLAMP the stored samples are max(x, 1e-6)
+// as the Levenberg-Marquardt terms use them (`if (x < 1e-6) x = 1e-6`, which
+// keeps a NaN, Estimation.h:114); the sums use the raw values.  The fit kernel
+// loads with the non-temporal hint (a few groups refill at a time and each ROI
+// is read once); the CRLB pass, where every group loads at once, uses ordinary
+// loads so that the lines its lanes share stay in L2 between loads.
+template <int R, class L, int NS, bool CLAMP, bool NT>
+PP_DEV void load_samples(const float* src, int g, float (&s)[NS], float& sx, float& sy, float& sum) {
+    // The per-slot offsets and coordinates depend only on g; for a strided layout
+    // the opaque copy keeps the compiler from hoisting all of them out of the fit
+    // loop into VGPRs (an empty asm: it emits no instruction).
+    if constexpr (L::kStrided) asm volatile("" : "+v"(g));
     sx = sy = sum = 0.0f;
 #pragma unroll
-    for (int r = 0; r < RL; r++) {
-        const int y = y0 + r;
-#pragma unroll
-        for (int x = 0; x < R; x++) {
-            float v = 0.0f;
-            if (y < R) v = NT ? __builtin_nontemporal_load(src + y * R + x) : src[y * R + x];
-            s[r][x] = (CLAMP && v < 1e-6f) ? 1e-6f : v;
-            sx += (float)x * v;
-            sy += (float)y * v;
-            sum += v;
-        }
+    for (int k = 0; k < NS; k++) {
+        const int idx = L::index(g, k);
+        float v = 0.0f;
+        if (idx < R * R) v = NT ? __builtin_nontemporal_load(src + idx) : src[idx];
+        s[k] = (CLAMP && v < 1e-6f) ? 1e-6f : v;
+        sx += (float)L::x(g, k) * v;
+        sy += (float)L::y(g, k) * v;
+        sum += v;
     }
 }
 
 // Fisher matrix (packed) and chi^2 of one spot, written by lane g == 0 of its group.
 template <class M, int R, int G>
-PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTraits<M, R, G>::RL][R],
-                      const float (&th)[M::K], int y0, int spot, bool lead) {
+PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTraits<M, R, G>::NS],
+                      const float (&th)[M::K], int g, float* lds, int spot) {
     using T = FitTraits<M, R, G>;
     constexpr int K = T::K, NA = T::NA;
     float fi[NA + 1];
 #pragma unroll
     for (int i = 0; i < NA + 1; i++) fi[i] = 0.0f;
-    model.template for_rows<R, T::RL, G>(th, y0, [&](int r, int x, float mu, const float (&J)[K]) {
+    model.template for_pixels<R, G>(th, g, lds, [&](int k, float mu, const float (&J)[K]) {
         const float m = fmaxf(1e-8f, mu);
         const float inv = rcpf(m);
-        const float e = s[r][x] - m;
+        const float e = s[k] - m;
         fi[NA] += e * e * inv;
         int q = 0;
 #pragma unroll
@@ -101,7 +106,7 @@ PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTrai
         }
     });
     group_sum_n<G>(fi);
-    if (lead) {
+    if (g == 0) {
         if (a.crlb) {
             float c[K];
             crlb_from_fisher<K>(fi, c);
@@ -114,29 +119,30 @@ PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTrai
 }
 
 // Fits spots first, first+stride, ... < a.n with the lane groups of the calling
-// wavefront; g_first is this lane group's first spot.  Every lane of the
-// wavefront must call it (the loops run until all groups of the wave are done).
+// wavefront (or, with a work counter, the spots it hands out); g_first is this
+// lane group's first spot and lds its LDS region.  Every lane of the wavefront
+// must call it (the loops run until all groups of the wave are done).
 template <class M, int R, int G>
-__device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int g_first, int stride) {
+__device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int g_first, int stride, float* lds) {
     using T = FitTraits<M, R, G>;
-    constexpr int K = T::K, RL = T::RL, NA = T::NA;
+    using L = typename T::L;
+    constexpr int K = T::K, NS = T::NS, NA = T::NA;
     constexpr int P = kWave / G;  // groups per wavefront
     static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "G must be a power of two <= 64");
 
     const int lane = threadIdx.x & (kWave - 1);
     const int g = lane & (G - 1);
     const bool lead = (g == 0);
-    const int y0 = g * RL;
     const int refill = a.refill > 0 ? a.refill : (P >= 8 ? P / 8 : 1);
 
-    float s[RL][R];
+    float s[NS];
     float th[K];
     int iter = 0;
     int spot = g_first;
 
     auto start_spot = [&](int sp) {
         float sx, sy, sum;
-        load_rows<R, RL, true, true>(a.sample + (size_t)sp * T::S, y0, s, sx, sy, sum);
+        load_samples<R, L, NS, true, true>(a.sample + (size_t)sp * T::S, g, s, sx, sy, sum);
         if (a.initial) {
 #pragma unroll
             for (int k = 0; k < K; k++) th[k] = a.initial[(size_t)sp * K + k];
@@ -149,7 +155,7 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
         iter = 0;
     };
 
-    // phase 1: fit.  state 0 = iterating, 1 = stopped (results not yet written), 2 = no more spots
+    // state 0 = iterating, 1 = stopped (results not yet written), 2 = no more spots
     int state = 2;
     if (spot < a.n) {
         start_spot(spot);
@@ -204,8 +210,8 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
         float acc[NA + K];
 #pragma unroll
         for (int i = 0; i < NA + K; i++) acc[i] = 0.0f;
-        model.template for_rows<R, RL, G>(th, y0, [&](int r, int x, float mu, const float (&J)[K]) {
-            const float xv = s[r][x];  // clamped at load
+        model.template for_pixels<R, G>(th, g, lds, [&](int k, float mu, const float (&J)[K]) {
+            const float xv = s[k];  // clamped at load
             const float inv = rcpf(fmaxf(mu, 1e-6f));
             const float t = xv * inv;
             const float xf2 = t * inv;
@@ -243,29 +249,34 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
         iter++;
         if (stop) state = 1;
     }
-
 }
 
 // CRLB and chi^2 of spots first, first+stride, ... < a.n from the samples and the
 // fitted parameters in HBM, one spot per lane group per pass (no divergence).
 template <class M, int R, int G>
-__device__ __forceinline__ void crlb_spots(const M& model, const FitArgs& a, int g_first, int stride) {
+__device__ __forceinline__ void crlb_spots(const M& model, const FitArgs& a, int g_first, int stride, float* lds) {
     using T = FitTraits<M, R, G>;
-    constexpr int K = T::K, RL = T::RL;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int g = lane & (G - 1);
-    const int y0 = g * RL;
-    float s[RL][R];
+    constexpr int K = T::K, NS = T::NS;
+    const int g = threadIdx.x & (G - 1);
+    float s[NS];
     float th[K];
     for (int sp = g_first; __any(sp < a.n); sp += stride) {
         if (sp < a.n) {
             float sx, sy, sum;
-            load_rows<R, RL, false, false>(a.sample + (size_t)sp * T::S, y0, s, sx, sy, sum);
+            load_samples<R, typename T::L, NS, false, false>(a.sample + (size_t)sp * T::S, g, s, sx, sy, sum);
 #pragma unroll
             for (int k = 0; k < K; k++) th[k] = a.params[(size_t)sp * K + k];
-            crlb_spot<M, R, G>(model, a, s, th, y0, sp, g == 0);
+            crlb_spot<M, R, G>(model, a, s, th, g, lds, sp);
         }
     }
+}
+
+// LDS region of this thread's lane group (models with per-group tables).
+template <class M, int R, int G>
+PP_DEV float* group_lds() {
+    extern __shared__ float ppa_lds[];
+    constexpr int per = M::lds_floats(R);
+    return per > 0 ? ppa_lds + (size_t)(threadIdx.x / G) * per : nullptr;
 }
 
 // Phase 1: fit.  Phase 2 (CRLB and chi^2 of the fitted spots) is a kernel of its
@@ -274,13 +285,15 @@ __device__ __forceinline__ void crlb_spots(const M& model, const FitArgs& a, int
 template <class M, int R, int G>
 __global__ void __launch_bounds__(256) fit_kernel(M model, FitArgs a) {
     const int groupsPerBlock = blockDim.x / G;
-    fit_spots<M, R, G>(model, a, blockIdx.x * groupsPerBlock + threadIdx.x / G, gridDim.x * groupsPerBlock);
+    fit_spots<M, R, G>(model, a, blockIdx.x * groupsPerBlock + threadIdx.x / G, gridDim.x * groupsPerBlock,
+                       group_lds<M, R, G>());
 }
 
 template <class M, int R, int G>
 __global__ void __launch_bounds__(256) crlb_phase_kernel(M model, FitArgs a) {
     const int groupsPerBlock = blockDim.x / G;
-    crlb_spots<M, R, G>(model, a, blockIdx.x * groupsPerBlock + threadIdx.x / G, gridDim.x * groupsPerBlock);
+    crlb_spots<M, R, G>(model, a, blockIdx.x * groupsPerBlock + threadIdx.x / G, gridDim.x * groupsPerBlock,
+                        group_lds<M, R, G>());
 }
 
 }  // namespace ppa

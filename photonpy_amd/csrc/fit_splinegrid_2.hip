@@ -1,0 +1,15 @@
+// Fit-kernel instantiations: splinegrid model, size list part 2 (written by gen_fit_sources.py).
+#include "dispatch.h"
+#include "fit_sizes.h"
+
+namespace ppa {
+
+
+hipError_t launch_fit_splinegrid_2(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st) {
+    using Model = SplineGridModel;
+    Model m = d.sg;
+    PPA_GRID_SIZES_2(PPA_FIT_CASE)
+    return hipErrorInvalidValue;
+}
+
+}  // namespace ppa

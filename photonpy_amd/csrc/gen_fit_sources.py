@@ -4,13 +4,14 @@ Run from this directory after changing the model list; the outputs are
 committed.  Each file instantiates one part of PPA_FIT_SIZES for one model so
 that `make -j` compiles them in parallel.
 """
-MODELS = [("fixed", "GaussModel<kFixed>", "Model m{d.gc};"),
-          ("sigma", "GaussModel<kSigma>", "Model m{d.gc};"),
-          ("astig", "GaussModel<kAstig>", "Model m{d.gc};"),
-          ("spline", "SplineModel", "Model m = d.sm;")]
+MODELS = [("fixed", "GaussModel<kFixed>", "Model m{d.gc};", "PPA_FIT_SIZES"),
+          ("sigma", "GaussModel<kSigma>", "Model m{d.gc};", "PPA_FIT_SIZES"),
+          ("astig", "GaussModel<kAstig>", "Model m{d.gc};", "PPA_FIT_SIZES"),
+          ("spline", "SplineModel", "Model m = d.sm;", "PPA_FIT_SIZES"),
+          ("splinegrid", "SplineGridModel", "Model m = d.sg;", "PPA_GRID_SIZES")]
 PARTS = 4
 
-for name, ty, ctor in MODELS:
+for name, ty, ctor, sizes in MODELS:
     for part in range(PARTS):
         decls = ""
         tail = "    return hipErrorInvalidValue;\n"
@@ -34,7 +35,7 @@ namespace ppa {{
 hipError_t {fname}(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st) {{
     using Model = {ty};
     {ctor}
-    PPA_FIT_SIZES_{part}(PPA_FIT_CASE)
+    {sizes}_{part}(PPA_FIT_CASE)
 {tail}}}
 
 }}  // namespace ppa

@@ -23,6 +23,31 @@
 
 namespace ppa {
 
+// Pixel ownership inside a lane group of G lanes fitting one R x R spot.  Lane g
+// keeps NS samples ("slots"); slot s of lane g is pixel index(g, s) = y*R + x.
+// RowLayout: lane g owns the rows [g*RL, g*RL+RL) (the Gaussian and coefficient
+// spline models, whose per-row and per-column terms are shared along rows).
+// FlatLayout: lane g owns pixels g, g+G, g+2G, ... (models that read their
+// per-pixel terms from LDS, where any pixel order costs the same).
+template <int R, int G>
+struct RowLayout {
+    static constexpr int RL = (R + G - 1) / G;
+    static constexpr int NS = RL * R;
+    static constexpr bool kStrided = false;
+    PP_DEV static int index(int g, int s) { return g * RL * R + s; }
+    PP_DEV static int x(int, int s) { return s % R; }
+    PP_DEV static int y(int g, int s) { return g * RL + s / R; }
+};
+
+template <int R, int G>
+struct FlatLayout {
+    static constexpr int NS = (R * R + G - 1) / G;
+    static constexpr bool kStrided = true;
+    PP_DEV static int index(int g, int s) { return g + G * s; }
+    PP_DEV static int x(int g, int s) { return index(g, s) % R; }
+    PP_DEV static int y(int g, int s) { return index(g, s) / R; }
+};
+
 enum GaussVariant { kFixed = 0, kSigma = 1, kAstig = 2 };
 
 constexpr float kPiF = 3.141592653589793f;
@@ -198,6 +223,18 @@ struct GaussModel {
               mu, J);
     }
 
+    template <int R, int G>
+    using Layout = RowLayout<R, G>;
+    static constexpr int lds_floats(int) { return 0; }
+
+    // Fit-kernel evaluation of lane g's pixels: fn(slot, mu, J) for every slot of
+    // RowLayout<R, G> that lies inside the ROI.
+    template <int R, int G, class Fn>
+    PP_DEV void for_pixels(const float (&t)[K], int g, float*, Fn&& fn) const {
+        constexpr int RL = RowLayout<R, G>::RL;
+        for_rows<R, RL, G>(t, g * RL, [&](int r, int x, float mu, const float (&J)[K]) { fn(r * R + x, mu, J); });
+    }
+
     // Fit-kernel evaluation over a block of RL rows starting at y0 of an RxR ROI,
     // called by every lane of an aligned group of G lanes.  The group shares the
     // R+1 column edges: lane g evaluates edges g, g+G, ... and the others receive
@@ -257,12 +294,12 @@ struct GaussModel {
 // ---------------------------------------------------------------------------
 // Cubic-spline PSF: coefs[Nz][Ny][Nx][64], coefficient q = 16*m + 4*n + o
 // multiplies z^m y^n x^o of the sub-voxel position.
-struct SplineModel {
+// Grid, clamps, stop limits and initial values shared by both spline forms.
+struct SplineCommon {
     static constexpr int K = 5;
     static constexpr int iI = 3, iBg = 4;
-    int Nz, Ny, Nx;
+    int Nz, Ny, Nx;  // cells per axis
     float zmin, zmax;
-    const float* coefs;
 
     PP_DEV static float stop_limit(int k) { return (k == 2) ? 1e-7f : 1e-4f; }
 
@@ -278,6 +315,10 @@ struct SplineModel {
     PP_DEV void initial(float cx, float cy, float sum, float (&t)[K]) const {
         t[0] = cx; t[1] = cy; t[2] = 0.0f; t[3] = sum; t[4] = 0.0f;
     }
+};
+
+struct SplineModel : SplineCommon {
+    const float* coefs;
 
     struct Frame {
         float tx, ty;              // position relative to the spline grid centre
@@ -361,6 +402,16 @@ struct SplineModel {
         finish(F, t, f, fx, fy, fz, mu, J);
     }
 
+    template <int R, int G>
+    using Layout = RowLayout<R, G>;
+    static constexpr int lds_floats(int) { return 0; }
+
+    template <int R, int G, class Fn>
+    PP_DEV void for_pixels(const float (&t)[K], int g, float*, Fn&& fn) const {
+        constexpr int RL = RowLayout<R, G>::RL;
+        for_rows<R, RL, G>(t, g * RL, [&](int r, int x, float mu, const float (&J)[K]) { fn(r * R + x, mu, J); });
+    }
+
     template <int R, int RL, int G, class Fn>
     PP_DEV void for_rows(const float (&t)[K], int y0, Fn&& fn) const {
         Frame F;
@@ -380,6 +431,190 @@ struct SplineModel {
                 }
             }
         }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Cubic-spline PSF evaluated from the samples it interpolates.  A table made by
+// CSpline_Compute (CubicSplinePSF.cu:151-204) holds, per cell, the tricubic
+// polynomial through a 4x4x4 block of samples: along each axis, cell c takes the
+// samples grid[c..c+3] at the local nodes t = -1, 0, 1, 2.  With the Lagrange
+// basis L_m of those nodes the cell's value at sub-voxel (w, v, u) is
+//   f = sum_{m,n,q} grid[k+m][j+n][i+q] L_m(w) L_n(v) L_q(u),
+// the same polynomial as the coefficient form sum_p c_p w^pz v^py u^px.  The
+// grid is [Nz+3][Ny+3][Nx+3] floats (215 KB for a 61x26x26-cell table, which
+// stays in L2) against 64 floats per cell (10.5 MB).  Per spot and iteration the
+// lane group contracts the grid's four z-slices over the spot's window into two
+// LDS tables (value and z-derivative weights, I / du folded in); each pixel then
+// reads its 4x4 entries of both.  Cell indices, the truncating sub-pixel offset
+// and the psf >= 0 clamp follow CubicSplinePSFModels.h:64-153 as in SplineModel.
+struct SplineGridModel : SplineCommon {
+    const float* grid;
+
+    template <int R, int G>
+    using Layout = FlatLayout<R, G>;
+    static constexpr int lds_floats(int R) { return 2 * (R + 3) * (R + 3); }
+
+    // Lagrange weights on the nodes -1, 0, 1, 2 at t, and their derivatives.
+    PP_DEV static void lagrange(float t, float (&w)[4], float (&dw)[4]) {
+        const float t2 = t * t, t3 = t2 * t;
+        w[0] = (-1.0f / 3.0f) * t + 0.5f * t2 - (1.0f / 6.0f) * t3;
+        w[1] = 1.0f - 0.5f * t - t2 + 0.5f * t3;
+        w[2] = t + 0.5f * t2 - 0.5f * t3;
+        w[3] = (-1.0f / 6.0f) * t + (1.0f / 6.0f) * t3;
+        dw[0] = (-1.0f / 3.0f) + t - 0.5f * t2;
+        dw[1] = -0.5f - 2.0f * t + 1.5f * t2;
+        dw[2] = 1.0f + t - 1.5f * t2;
+        dw[3] = (-1.0f / 6.0f) + 0.5f * t2;
+    }
+
+    struct Frame {
+        float tx, ty;          // position relative to the grid centre
+        int k;                 // z-slab
+        float wx[4], wdx[4];   // x weights; derivative weights times -I
+        float wy[4], wdy[4];   // y weights; derivative weights times -I
+        float wz[4], wdz[4];   // z weights; derivative weights times I / du
+    };
+
+    PP_DEV void frame(const float (&t)[K], Frame& F) const {
+        const float du = (zmax - zmin) / (float)Nz;
+        F.tx = t[0] - (float)Nx * 0.5f;
+        F.ty = t[1] - (float)Ny * 0.5f;
+        float spx = -F.tx; spx -= (float)(int)spx;   // truncation toward zero (reference quirk)
+        float spy = -F.ty; spy -= (float)(int)spy;
+        float spz = (t[2] - zmin) / du;
+        F.k = clampi((int)spz, 0, Nz - 1);
+        spz = clampf(spz - (float)F.k, 0.0f, 1.0f);
+        const float I = t[iI], mI = -I, Idu = I / du;
+        lagrange(spx, F.wx, F.wdx);
+        lagrange(spy, F.wy, F.wdy);
+        lagrange(spz, F.wz, F.wdz);
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            F.wdx[m] *= mI;
+            F.wdy[m] *= mI;
+            F.wdz[m] *= Idu;
+        }
+    }
+
+    PP_DEV int cell_x(const Frame& F, int x) const { return clampi((int)((float)x - F.tx), 0, Nx - 1); }
+    PP_DEV int cell_y(const Frame& F, int y) const { return clampi((int)((float)y - F.ty), 0, Ny - 1); }
+
+    PP_DEV void finish(const float (&t)[K], float f, float fx, float fy, float fz, float& mu, float (&J)[K]) const {
+        J[0] = fx;  // -I folded into the x/y derivative weights, I / du into the z ones
+        J[1] = fy;
+        J[2] = fz;
+        const float psf = f < 0.0f ? 0.0f : f;
+        J[3] = psf;
+        J[4] = 1.0f;
+        mu = t[iI] * psf + t[iBg];
+    }
+
+    // Single-pixel evaluation straight from the grid (expected value / derivatives).
+    PP_DEV void eval_pixel(const float (&t)[K], int x, int y, float& mu, float (&J)[K]) const {
+        Frame F;
+        frame(t, F);
+        const int i = cell_x(F, x), j = cell_y(F, y);
+        const int Dy = Ny + 3, Dx = Nx + 3;
+        float f = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            float a = 0.0f, ad = 0.0f, az = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                float e = 0.0f, ez = 0.0f;
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    const float v = grid[((size_t)(F.k + m) * Dy + (j + n)) * Dx + (i + q)];
+                    e += v * F.wz[m];
+                    ez += v * F.wdz[m];
+                }
+                a += F.wx[q] * e;
+                ad += F.wdx[q] * e;
+                az += F.wx[q] * ez;
+            }
+            f += F.wy[n] * a;
+            fx += F.wy[n] * ad;
+            fy += F.wdy[n] * a;
+            fz += F.wy[n] * az;
+        }
+        finish(t, f, fx, fy, fz, mu, J);
+    }
+
+    // Fit-kernel evaluation of lane g's pixels (FlatLayout<R, G>), called by every
+    // lane of the group with the group's LDS region (lds_floats(R) floats).
+    template <int R, int G, class Fn>
+    PP_DEV void for_pixels(const float (&t)[K], int g, float* lds, Fn&& fn) const {
+        constexpr int W = R + 3;          // window width in grid samples
+        constexpr int NE = W * W;         // window entries
+        constexpr int PE = (NE + G - 1) / G;
+        using L = FlatLayout<R, G>;
+        // Pixel coordinates depend only on g and the slot; without this opaque copy
+        // the compiler hoists all of them out of the iteration loop and keeps them
+        // in VGPRs for the whole fit.  (An empty asm: it emits no instruction.)
+        asm volatile("" : "+v"(g));
+        Frame F;
+        frame(t, F);
+        const int i0 = cell_x(F, 0), j0 = cell_y(F, 0);
+        const int Dy = Ny + 3, Dx = Nx + 3;
+        const float* slab = grid + (size_t)F.k * Dy * Dx;
+        const size_t zs = (size_t)Dy * Dx;
+        float* E = lds;
+        float* Ez = lds + NE;
+        // z-contraction of the window: rows j0.., columns i0.. (clamped to the grid;
+        // entries past the last cell are never read by a pixel).  Two entries per
+        // trip keep 8 loads in flight per lane without holding the whole window.
+#pragma unroll 2
+        for (int u = 0; u < PE; u++) {
+            const int e = g + G * u;
+            if (NE % G == 0 || e < NE) {
+                const int r = e / W, c = e - (e / W) * W;
+                const int gy = min(j0 + r, Dy - 1), gx = min(i0 + c, Dx - 1);
+                const float* src = slab + (size_t)gy * Dx + gx;
+                const float v0 = src[0], v1 = src[zs], v2 = src[2 * zs], v3 = src[3 * zs];
+                E[e] = v0 * F.wz[0] + v1 * F.wz[1] + v2 * F.wz[2] + v3 * F.wz[3];
+                Ez[e] = v0 * F.wdz[0] + v1 * F.wdz[1] + v2 * F.wdz[2] + v3 * F.wdz[3];
+            }
+        }
+        // the group is inside one wavefront, whose LDS operations complete in order:
+        // only the compiler has to be kept from moving the reads above the writes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int s = 0; s < L::NS; s++) {
+            const int p = L::index(g, s);
+            if (L::NS * G == R * R || p < R * R) {
+                const int y = p / R, x = p - (p / R) * R;
+                const int o = (cell_y(F, y) - j0) * W + (cell_x(F, x) - i0);
+                float f = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll
+                for (int n = 0; n < 4; n++) {
+                    float a = 0.0f, ad = 0.0f, az = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const float e = E[o + n * W + q], ez = Ez[o + n * W + q];
+                        a += F.wx[q] * e;
+                        ad += F.wdx[q] * e;
+                        az += F.wx[q] * ez;
+                    }
+                    f += F.wy[n] * a;
+                    fx += F.wy[n] * ad;
+                    fy += F.wdy[n] * a;
+                    fz += F.wy[n] * az;
+                }
+                float mu, J[K];
+                finish(t, f, fx, fy, fz, mu, J);
+                fn(s, mu, J);
+            }
+            // one pixel at a time: without this the scheduler hoists the LDS reads of
+            // many pixels and runs out of VGPRs
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // the next call overwrites the tables: every lane's reads come first
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 };
 

@@ -328,6 +328,19 @@ class Estimator:
     def GetGroupSize(self):
         return self._lib.Estimator_GetGroupSize(self.inst)
 
+    def spline_form(self):
+        """'grid' or 'coefficients' for a spline estimator (None otherwise): the form its
+        fit and expected-value kernels evaluate the PSF in."""
+        f = self._lib.Estimator_SplineForm(self.inst)
+        return None if f < 0 else ("grid" if f == 1 else "coefficients")
+
+    def set_spline_mode(self, mode):
+        """'auto' (the grid form when the table is a CSpline_Compute interpolant) or
+        'coefficients'."""
+        m = {"auto": 0, "coefficients": 1}[mode]
+        if self._lib.Estimator_SetSplineMode(self.inst, m) != 0:
+            raise ValueError("set_spline_mode applies to spline estimators only")
+
     def set_phase_timing(self, enable=True):
         """Record HIP events around the fit and CRLB kernels of every fit launch."""
         if self._lib.Estimator_SetPhaseTiming(self.inst, 1 if enable else 0) != 0:

@@ -103,6 +103,8 @@ def _bind(lib):
         "Estimator_GetGroupSize": ([V], ct.c_int),
         "Estimator_SetRefill": ([V, I], ct.c_int),
         "Estimator_SetPhaseTiming": ([V, I], ct.c_int),
+        "Estimator_SetSplineMode": ([V, I], ct.c_int),
+        "Estimator_SplineForm": ([V], ct.c_int),
         "Estimator_GetPhaseTimes": ([V, ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)], ct.c_int),
         "Estimator_ComputeExpectedValue": ([V, I, FloatArray, FloatArray, IntArray, FloatArray], None),
         "Estimator_Estimate": ([V, I, FloatArray, FloatArray, IntArray, NullableFloatArrayType, FloatArray,
@@ -116,9 +118,9 @@ def _bind(lib):
         "Estimator_ChiSquareAndCRLBDevice": ([V, I, V, V, V, V, V], ct.c_int),
         "Estimator_NLLGradDevice": ([V, I, V, V, V, V, V, V, V, V], ct.c_int),
         "Estimator_NumShared": ([V], ct.c_int),
-        "MixedBatch_Create": ([I, ct.POINTER(V), ct.POINTER(ct.c_int), ct.POINTER(ct.c_int), I], V),
+        "MixedBatch_Create": ([I, ct.POINTER(V), ct.POINTER(ct.c_int), I], V),
         "MixedBatch_Delete": ([V], None),
-        "MixedBatch_NumTiles": ([V], ct.c_int),
+        "MixedBatch_NumStreams": ([V], ct.c_int),
         "MixedBatch_Run": ([V, ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), I, V],
                            ct.c_int),
         "MixedBatch_RunPlain": ([V, ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), V],

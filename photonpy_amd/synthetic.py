@@ -87,3 +87,37 @@ def make_samples(est, kind, n, seed=1, device="cuda"):
     smp = torch.poisson(mu, generator=gen)
     del mu
     return th, smp
+
+
+MIXED = dict(n=4_000_000, sizes=[7, 9, 11, 13, 16, 20, 25], weights=[0.1, 0.2, 0.25, 0.15, 0.15, 0.1, 0.05],
+             kinds=["fixed", "sigma", "astig", "spline"],
+             workload="mixed batch: 4,000,000 ROIs, sizes {7,9,11,13,16,20,25} weighted "
+                      "{.1,.2,.25,.15,.15,.1,.05}, models uniform over fixed/free-sigma/astig/spline, + CRLB")
+# per-model LM settings of the single-model workloads
+MODEL_LM = {"fixed": (1e-15, 30), "sigma": (1e-18, 50), "astig": (1e-18, 50), "spline": (1e-18, 50)}
+
+
+def mixed_bins(ctx, n=None, seed=1):
+    """The mixed workload as (model, ROI size) bins: each ROI draws its size from the
+    weighted list and its model uniformly; bins are filled with that many ROIs of the
+    model's own parameter distribution.  Returns a list of dicts (kind, R, n, est,
+    samples [n,R,R] on the device, true params)."""
+    n = n or MIXED["n"]
+    rng = np.random.default_rng(seed)
+    size_idx = rng.choice(len(MIXED["sizes"]), size=n, p=MIXED["weights"])
+    kind_idx = rng.integers(0, len(MIXED["kinds"]), size=n)
+    counts = np.zeros((len(MIXED["kinds"]), len(MIXED["sizes"])), np.int64)
+    np.add.at(counts, (kind_idx, size_idx), 1)
+    table = spline_table(ctx)
+    bins = []
+    for ki, kind in enumerate(MIXED["kinds"]):
+        for si, R in enumerate(MIXED["sizes"]):
+            m = int(counts[ki, si])
+            if m == 0:
+                continue
+            est = make_estimator(ctx, kind, R, spline=table)
+            lam, iters = MODEL_LM[kind]
+            est.SetLevMarParams(lam, iters)
+            th, smp = make_samples(est, kind, m, seed=seed + 1000 * ki + R)
+            bins.append(dict(kind=kind, R=R, n=m, est=est, samples=smp, params=th, iters=iters))
+    return bins

@@ -1,8 +1,9 @@
-"""Persistent mixed-model / mixed-size batch vs the plain per-bin launches.
+"""Mixed-model, mixed-size batch vs the plain per-bin launches.
 
-Every spot's fit is independent of scheduling, so the persistent launch (eager or
-replayed from its HIP graph) must reproduce the plain launches bit for bit; the
-plain launches themselves are parity-tested against the oracle in test_gpu_parity.
+Every spot's fit is independent of scheduling (which stream, which wavefront, when), so
+the batch (eager or replayed from its HIP graph) must reproduce the plain launches bit for
+bit; the plain launches themselves are parity-tested against the oracle in
+test_gpu_parity.
 """
 import numpy as np
 import pytest
@@ -11,13 +12,10 @@ import photonpy_amd as pa
 from photonpy_amd.batch import MixedBatch
 from tests import _models as MD
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="the persistent mixed-batch launch hung once on the GPU (build with per-variant "
-                            "__noinline__ functions); its cause has not been shown, so the path is switched off"),
-]
+pytestmark = pytest.mark.gpu
 
 SIZES = [7, 9, 11, 13, 16, 20, 25]
+KEYS = ("params", "iterations", "crlb", "chisq")
 
 
 @pytest.fixture(scope="module")
@@ -46,45 +44,55 @@ def _cat(outs, key):
     return [o[key].cpu().numpy() for o in outs]
 
 
-def test_persistent_matches_plain(ctx):
+def _same(ref, outs, tag):
+    for key in KEYS:
+        for b, (a, c) in enumerate(zip(_cat(ref, key), _cat(outs, key))):
+            assert np.array_equal(a, c, equal_nan=True), (tag, key, b)
+
+
+def test_batch_matches_plain(ctx):
     import torch
     ests, samples, counts = _bins(ctx, ["fixed", "sigma", "astig", "spline"], SIZES, 300)
-    with MixedBatch(ests, counts, [4] * len(ests)) as mb:
+    with MixedBatch(ests, counts, streams=4, tune=False) as mb:
+        assert mb.num_streams() == 4
         ref = mb.allocate_outputs()
         mb.run_plain(samples, ref)
         eager = mb.allocate_outputs()
         mb.run(samples, eager, use_graph=False)
         torch.cuda.synchronize()
-        for key in ("params", "iterations", "crlb", "chisq"):
-            for a, b in zip(_cat(ref, key), _cat(eager, key)):
-                assert np.array_equal(a, b, equal_nan=True), key
+        _same(ref, eager, "eager")
         # graph replays, with eager work and synchronisation between them, each checked
+        outs = mb.allocate_outputs()
         for rep in range(3):
-            outs = mb.allocate_outputs()
+            for o in outs:
+                for v in o.values():
+                    v.fill_(-1)
             mb.run(samples, outs, use_graph=True)
             torch.cuda.synchronize()
             _ = (samples[0] * 2).sum().item()
-            for key in ("params", "iterations", "crlb", "chisq"):
-                for a, b in zip(_cat(ref, key), _cat(outs, key)):
-                    assert np.array_equal(a, b, equal_nan=True), (rep, key)
+            _same(ref, outs, f"replay {rep}")
+        # new output buffers: captured again
+        outs2 = mb.allocate_outputs()
+        mb.run(samples, outs2, use_graph=True)
+        torch.cuda.synchronize()
+        _same(ref, outs2, "recaptured")
 
 
-def test_unsupported_size_rejected(ctx):
-    est = pa.GaussianPSFMethods(ctx).CreatePSF_XYIBg(12, 1.5, True)
-    with pytest.raises(pa.NativeError, match="roisize 12"):
-        MixedBatch([est], [10], [8])
-
-
-def test_tile_lengths_do_not_change_results(ctx):
+def test_stream_count_does_not_change_results(ctx):
     import torch
     ests, samples, counts = _bins(ctx, ["sigma", "astig"], [9, 16], 500, seed=4)
     results = []
-    for spg in (1, 8, 32):
-        with MixedBatch(ests, counts, [spg] * len(ests)) as mb:
+    for ns in (1, 2, 8):
+        with MixedBatch(ests, counts, streams=ns, tune=False) as mb:
             outs = mb.allocate_outputs()
             mb.run(samples, outs)
             torch.cuda.synchronize()
-            results.append(_cat(outs, "params"))
+            results.append(outs)
     for r in results[1:]:
-        for a, b in zip(results[0], r):
-            assert np.array_equal(a, b, equal_nan=True)
+        _same(results[0], r, "streams")
+
+
+def test_unsupported_size_rejected(ctx):
+    est = pa.GaussianPSFMethods(ctx).CreatePSF_XYIBg(40, 1.5, True)
+    with pytest.raises(pa.NativeError, match="roisize"):
+        MixedBatch([est], [10], tune=False)

@@ -217,6 +217,55 @@ def test_refill_batches_do_not_change_results(ctx):
         assert np.array_equal(p, outs[0][0], equal_nan=True) and np.array_equal(it, outs[0][1])
 
 
+def test_spline_grid_form_matches_coefficient_form(ctx):
+    """The spline's grid form (default for CSpline_Compute tables) and its coefficient form
+    agree on expected values, derivatives, CRLB and fits."""
+    th, smp = MD.samples("spline", 600, 31)
+    est, om, _ = MD.make("spline", ctx)
+    assert est.spline_form() == "grid"
+    ev_g = est.ExpectedValue(th)
+    dv_g, _ = est.Derivatives(th)
+    crlb_g, chi_g = est.CRLBAndChiSquare(th, smp)
+    est.SetLevMarParams(1e-18, 50)
+    p_g, _, _ = est.Estimate(smp)
+    it_g = est.last_iterations.copy()
+    est.set_spline_mode("coefficients")
+    assert est.spline_form() == "coefficients"
+    ev_c = est.ExpectedValue(th)
+    dv_c, _ = est.Derivatives(th)
+    crlb_c, chi_c = est.CRLBAndChiSquare(th, smp)
+    p_c, _, _ = est.Estimate(smp)
+    it_c = est.last_iterations.copy()
+    est.set_spline_mode("auto")
+    scale = np.abs(ev_c).max(axis=(1, 2))[:, None, None]
+    assert np.max(np.abs(ev_g - ev_c) / scale) < 2e-6
+    dscale = np.maximum(np.abs(dv_c).max(axis=(2, 3), keepdims=True), 1e-6)
+    assert np.max(np.abs(dv_g - dv_c) / dscale) < 5e-5
+    ok = np.all(np.isfinite(crlb_c), axis=1) & (crlb_c[:, 0] < 0.5)
+    assert np.allclose(crlb_g[ok], crlb_c[ok], rtol=1e-3)
+    assert np.allclose(chi_g, chi_c, rtol=1e-4, atol=1e-3)
+    lo, hi = est.ParamLimits()
+    last_ok = ~_on_bounds(p_c, lo, hi) & ~_on_bounds(p_g, lo, hi)
+    agree = _compare("spline", p_g[last_ok], p_c[last_ok])
+    assert agree.mean() > 0.97, agree.mean()
+    # the grid form also matches the oracle (coefficient form) through test_estimate_parity
+
+
+@pytest.mark.parametrize("R,G", [(20, 16), (20, 32), (16, 32), (11, 16), (7, 4)])
+def test_spline_grid_group_sizes_agree(ctx, R, G):
+    th, smp = MD.samples("spline", 400, 33, R=R)
+    est, _, _ = MD.make("spline", ctx, R=R)
+    est.SetLevMarParams(1e-18, 50)
+    p1, _, _ = est.Estimate(smp)
+    c1, x1 = est.CRLBAndChiSquare(p1, smp)
+    est.SetGroupSize(G)
+    p2, _, _ = est.Estimate(smp)
+    # a spot's arithmetic does not depend on G except the order of the group sums
+    lo, hi = est.ParamLimits()
+    use = ~_on_bounds(p1, lo, hi) & ~_on_bounds(p2, lo, hi)
+    assert _compare("spline", p2[use], p1[use]).mean() > 0.97
+
+
 def test_spline_coefficients_match_oracle(ctx):
     stack, _ = C.spline_zstack(nz=12, ny=15, nx=15)
     c_gpu = pa.CSplineMethods(ctx).ComputeSplineCoeff(stack)
@@ -250,7 +299,8 @@ def _full_size_fixed_point(ctx, kind, n, seed):
     return ok.mean(), out
 
 
-@pytest.mark.parametrize("kind,n", [("sigma", 1_000_000), ("astig", 2_000_000), ("fixed", 10_000)])
+@pytest.mark.parametrize("kind,n", [("sigma", 1_000_000), ("astig", 2_000_000), ("fixed", 10_000),
+                                    ("spline", 500_000)])
 def test_full_size_noise_free_fixed_point(ctx, kind, n):
     frac, out = _full_size_fixed_point(ctx, kind, n, 99)
     assert frac > 0.999, frac
