@@ -12,6 +12,7 @@ Workloads (the default is c2):
   c4  cubic-spline 3D PSF (61x26x26 cells), 20x20 px, 500,000 spots, 50 iterations + CRLB
   c4adj  spline PSF adjoint (forward NLL + backward to params and all coefficients), 65,536 ROIs
   c5  mixed batch: 4,000,000 ROIs over 7 sizes x 4 models, batch form vs plain per-bin launches
+  queue  C2's ROIs from host memory through the EstimQueue (the reference's streaming path)
 Synthetic data (photonpy_amd/synthetic.py): true parameters from numpy
 PCG64(seed 1) as the workload states, expected images from the library's own
 expected-value kernel, Poisson noise from a seeded torch generator on the GPU.
@@ -197,6 +198,57 @@ def bench_mixed(args):
                    "bins": [{"kind": b["kind"], "R": b["R"], "n": b["n"]} for b in bins]}}))
 
 
+def bench_queue(args):
+    """The reference's own throughput path (photonpy/tests/psf_queue.py): C2's ROIs in
+    host memory streamed through an EstimQueue (batches of 4096 on 4 streams, scheduled
+    10,000 at a time, results collected as they finish).  Host-to-host rate, PCIe
+    included; the queue's batches use the same fused kernels as the device path."""
+    import torch
+    import photonpy_amd as pa
+    from photonpy_amd import autotune, synthetic
+    cfg = synthetic.WORKLOADS["c2"]
+    n = args.n or cfg["n"]
+    with pa.Context() as ctx:
+        est = synthetic.make_estimator(ctx, cfg["kind"], cfg["R"])
+        est.SetLevMarParams(cfg["lam"], cfg["iters"])
+        autotune.apply(est)
+        _, smp = synthetic.make_samples(est, cfg["kind"], n, seed=1)
+        host = smp.cpu().numpy()
+        del smp
+        torch.cuda.empty_cache()
+        ids = np.arange(n, dtype=np.int32)
+        chunk = 10_000
+
+        def run_once():
+            got = 0
+            with pa.EstimQueue(est, batchSize=4096, numStreams=4) as q:
+                for s0 in range(0, n, chunk):
+                    q.Schedule(host[s0:s0 + chunk], ids=ids[s0:s0 + chunk])
+                    if q.GetResultCount() > 0:
+                        got += len(q.GetResults().ids)
+                q.Flush()
+                while not q.IsIdle():
+                    time.sleep(0.001)
+                got += len(q.GetResults().ids)
+            return got
+
+        for _ in range(args.warmup):
+            run_once()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            got = run_once()
+        dt = (time.perf_counter() - t0) / args.steps
+    assert got == n, (got, n)
+    print(json.dumps({
+        "metric": METRIC, "value": n / dt, "unit": "localizations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (f64 solve)", "data": "synthetic",
+        "config": {"workload": "EstimQueue from host memory: " + cfg["workload"], "spots": n, "batch": 4096,
+                   "streams": 4, "schedule_chunk": chunk},
+        "roofline": None, "cpu_baseline": None,
+        "detail": {"note": "host-to-host including PCIe transfers; compare detail.pcie_inclusive_loc_per_s of c2"}}))
+
+
 def bench_adjoint(args):
     """Spline PSF calibration step: forward Poisson NLL + hand-written adjoint on 65,536
     20x20 ROIs, gradients w.r.t. per-spot parameters and all 2,638,976 spline coefficients
@@ -249,7 +301,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c4adj", "c5"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c4adj", "c5", "queue"])
     ap.add_argument("--n", type=int, default=0, help="override the number of spots")
     ap.add_argument("--group", type=int, default=0, help="lane-group size override (0 = tuned/default)")
     ap.add_argument("--refill", type=int, default=0, help="refill batch override (0 = tuned/default)")
@@ -282,6 +334,8 @@ def main():
         return bench_adjoint(args)
     if args.config == "c5":
         return bench_mixed(args)
+    if args.config == "queue":
+        return bench_queue(args)
     cfg = synthetic.WORKLOADS[args.config]
     n = args.n or cfg["n"]
     torch.cuda.set_device(0)

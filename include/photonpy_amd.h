@@ -25,6 +25,7 @@ extern "C" {
 
 typedef struct Context Context;
 typedef struct Estimator Estimator;
+typedef struct EstimationQueue EstimationQueue;
 
 /* Estimators/Estimator.h:18-26 */
 typedef struct LMParams {
@@ -59,6 +60,13 @@ typedef struct Gauss3D_CalibrationQ {
     float minz, maxz;
     float Bx, By;
 } Gauss3D_CalibrationQ;
+
+/* Estimators/EstimationQueue.h:21-25 */
+typedef struct EstimatorResult {
+    int id;
+    float chisq;
+    int iterations;
+} EstimatorResult;
 
 /* Estimators/CSpline/CubicSplinePSF.h:12-23; shape = {Nz, Ny, Nx} */
 typedef struct CSpline_Calibration {
@@ -111,6 +119,27 @@ void Estimator_ComputeDerivatives(Estimator* e, int numspots, const float* param
 /* Estimators/Estimator.cpp:146-157 — sample may be NULL (chisq then 0) */
 void Estimator_ChiSquareAndCRLB(Estimator* e, const float* params, const float* sample, const float* constants,
                                 const int* spotpos, int numspots, float* crlb, float* chisq);
+
+/* Estimators/EstimationQueue.h:111-127, EstimationQueue.cpp:329-412.  Asynchronous
+ * batches from host memory: Schedule copies spots into pinned batches of batchSize and
+ * blocks while GetQueueLength() (spots in queued batches + batches in flight) is at
+ * least maxQueueLen; a worker thread fits each batch (+ CRLB, chi^2) on one of
+ * numStreams streams (< 0: 3); GetResults copies whole finished batches, newest first,
+ * while they fit in maxresults and returns the number of spots copied (estim [n,K],
+ * crlb [n,K], roipos [n,2], samples [n,H,W] when keepSamples, result [n]; each may be
+ * NULL; diag is unused, numDiag = 0).  constants must be NULL or empty (no model has
+ * any).  A queue created in a Context is deleted by Context_Destroy. */
+EstimationQueue* EstimQueue_Create(Estimator* psf, int batchSize, int maxQueueLen, bool keepSamples, int numStreams,
+                                   Context* ctx);
+void EstimQueue_Delete(EstimationQueue* q);
+void EstimQueue_Schedule(EstimationQueue* q, int numspots, const int* ids, const float* h_samples,
+                         const float* h_initial, const float* h_constants, const int* h_roipos);
+void EstimQueue_Flush(EstimationQueue* q);
+bool EstimQueue_IsIdle(EstimationQueue* q);
+int EstimQueue_GetResultCount(EstimationQueue* q);
+int EstimQueue_GetQueueLength(EstimationQueue* q);
+int EstimQueue_GetResults(EstimationQueue* q, int maxresults, float* estim, float* diag, float* crlb, int* roipos,
+                          float* samples, EstimatorResult* result);
 
 /* ---------------------------------------------------------------------- */
 /* MI355X additions (no reference counterpart)                              */

@@ -9,6 +9,8 @@ from .context import Context
 from .estimator import Estimator
 from .gaussian import Gauss3D_Calibration, GaussianPSFMethods, Gaussian
 from .cspline import CSplineCalibration, CSplineMethods, CSplinePSF
+from .estim_queue import EstimQueue, EstimQueue_Results
 
 __all__ = ["SMLM", "NativeError", "LIB_PATH", "Context", "Estimator", "Gauss3D_Calibration",
-           "GaussianPSFMethods", "Gaussian", "CSplineCalibration", "CSplineMethods", "CSplinePSF"]
+           "GaussianPSFMethods", "Gaussian", "CSplineCalibration", "CSplineMethods", "CSplinePSF", "EstimQueue",
+           "EstimQueue_Results"]

@@ -2,21 +2,7 @@
 // include/photonpy_amd.h).  Host-pointer calls stage through a per-estimator
 // device workspace and run synchronously; device-pointer calls launch on the
 // caller's stream.  Everything computes on the GPU: there is no CPU path.
-#include <hip/hip_runtime.h>
-
-#include <cmath>
-#include <cstdio>
-#include <cstdlib>
-#include <algorithm>
-#include <cstring>
-#include <mutex>
-#include <stdexcept>
-#include <string>
-#include <unordered_set>
-#include <vector>
-
-#include "../../include/photonpy_amd.h"
-#include "dispatch.h"
+#include "capi_internal.h"
 
 namespace ppa {
 hipError_t launch_spline_coefs(const int* shape, const float* d_data, float* d_out, int border, hipStream_t st);
@@ -24,169 +10,6 @@ hipError_t launch_spline_coefs(const int* shape, const float* d_data, float* d_o
 
 using namespace ppa;
 
-// ---------------------------------------------------------------- errors
-static thread_local std::string g_lastError;
-
-static void set_error(const std::string& msg) { g_lastError = msg; }
-
-struct HipError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-static void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-// ---------------------------------------------------------------- objects
-struct Context {
-    std::mutex mtx;
-    std::unordered_set<Estimator*> objects;
-};
-
-struct Estimator {
-    Context* ctx = nullptr;
-    std::mutex mtx;
-    int roisize = 0;
-    ModelDesc desc{};
-    std::string fmt;
-    std::vector<ParamLimit> limits;
-    LMParams lm{30, 1e-15f, 0.0f};  // Estimator.cpp:24
-    int group = 0;                  // 0: default_group(roisize)
-    int refill = 0;                 // stopped groups per batched refill (0: kernel default)
-    // spline coefficients: host copy (owned) and lazily uploaded device copy
-    std::vector<float> hostCoefs;
-    float* devCoefs = nullptr;
-    // grid form of the spline (the samples the table interpolates), when the table
-    // is a CSpline_Compute interpolant; used unless the caller asks for coefficients
-    std::vector<float> hostGrid;
-    float* devGrid = nullptr;
-    bool gridForm = false;     // table verified to be a grid interpolant
-    int splineMode = 0;        // 0: grid form when available, 1: always coefficients
-    // grow-only device workspace for the host-pointer API
-    void* ws = nullptr;
-    size_t wsBytes = 0;
-
-    float* devShared = nullptr;  // device copy of the shared PSF parameters (Gaussian models)
-    // Work counters of the fit kernel: one slot per launch, cycling, so launches of this
-    // estimator on different streams do not share a counter unless 64 are in flight.
-    static constexpr int kCounterSlots = 64;
-    int* devCounters = nullptr;
-    unsigned counterSlot = 0;
-
-    // Optional kernel timing: three events per fit launch (before the fit kernel,
-    // after it, after the CRLB kernel), read and released by Estimator_GetPhaseTimes.
-    bool phaseTiming = false;
-    std::vector<hipEvent_t> phaseEvents;
-
-    void release_phase_events() {
-        for (hipEvent_t ev : phaseEvents) (void)hipEventDestroy(ev);
-        phaseEvents.clear();
-    }
-
-    void ensure_counters() {
-        if (!devCounters) hip_check(hipMalloc(&devCounters, kCounterSlots * sizeof(int)), "hipMalloc(counters)");
-    }
-
-    int* next_counter() {
-        ensure_counters();
-        return devCounters + (counterSlot++ % kCounterSlots);
-    }
-
-    int K() const { return num_params(desc.kind); }
-
-    // Shared PSF parameters in the adjoint's layout
-    std::vector<float> shared_values() const {
-        if (desc.kind == kKindFixed) return {desc.gc.sx, desc.gc.sy};
-        if (desc.kind == kKindAstig) {
-            std::vector<float> v(desc.gc.ax, desc.gc.ax + 5);
-            v.insert(v.end(), desc.gc.ay, desc.gc.ay + 5);
-            return v;
-        }
-        if (desc.kind == kKindSpline) return hostCoefs;
-        return {};
-    }
-    size_t num_shared() const {
-        if (desc.kind == kKindSpline) return hostCoefs.size();
-        return shared_values().size();
-    }
-    const float* device_shared() {
-        ensure_device();
-        if (desc.kind == kKindSpline) return devCoefs;
-        std::vector<float> v = shared_values();
-        if (v.empty()) return nullptr;
-        if (!devShared) {
-            hip_check(hipMalloc(&devShared, v.size() * sizeof(float)), "hipMalloc(shared)");
-            hip_check(hipMemcpy(devShared, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice), "upload shared");
-        }
-        return devShared;
-    }
-    int S() const { return roisize * roisize; }
-    int G() const { return group > 0 ? group : default_group_for(desc, roisize); }
-
-    ~Estimator() {
-        if (devCoefs) (void)hipFree(devCoefs);
-        if (devGrid) (void)hipFree(devGrid);
-        if (devShared) (void)hipFree(devShared);
-        if (devCounters) (void)hipFree(devCounters);
-        if (ws) (void)hipFree(ws);
-        release_phase_events();
-    }
-
-    void ensure_device() {
-        if (desc.kind == kKindSpline && !devCoefs) {
-            const size_t bytes = hostCoefs.size() * sizeof(float);
-            hip_check(hipMalloc(&devCoefs, bytes), "hipMalloc(spline coefficients)");
-            hip_check(hipMemcpy(devCoefs, hostCoefs.data(), bytes, hipMemcpyHostToDevice), "upload spline coefficients");
-            desc.sm.coefs = devCoefs;
-        }
-        if (desc.kind == kKindSpline && desc.useGrid && !devGrid) {
-            const size_t bytes = hostGrid.size() * sizeof(float);
-            hip_check(hipMalloc(&devGrid, bytes), "hipMalloc(spline grid)");
-            hip_check(hipMemcpy(devGrid, hostGrid.data(), bytes, hipMemcpyHostToDevice), "upload spline grid");
-            desc.sg.grid = devGrid;
-        }
-    }
-
-    void* workspace(size_t bytes) {
-        if (bytes > wsBytes) {
-            if (ws) hip_check(hipFree(ws), "hipFree(workspace)");
-            ws = nullptr;
-            wsBytes = 0;
-            hip_check(hipMalloc(&ws, bytes), "hipMalloc(workspace)");
-            wsBytes = bytes;
-        }
-        return ws;
-    }
-};
-
-static void attach(Estimator* e, Context* ctx) {
-    if (!ctx) return;
-    std::lock_guard<std::mutex> l(ctx->mtx);
-    ctx->objects.insert(e);
-    e->ctx = ctx;
-}
-
-static void detach(Estimator* e) {
-    if (!e->ctx) return;
-    std::lock_guard<std::mutex> l(e->ctx->mtx);
-    e->ctx->objects.erase(e);
-    e->ctx = nullptr;
-}
-
-template <class F>
-static auto guarded(F fn, decltype(fn()) fallback) -> decltype(fn()) {
-    try {
-        return fn();
-    } catch (const std::exception& ex) {
-        set_error(ex.what());
-        return fallback;
-    }
-}
-
-static void require_gpu(bool cuda) {
-    if (!cuda)
-        throw std::runtime_error("photonpy_amd: CPU estimators are not provided; create the estimator with cuda=True");
-}
 
 // Samples whose local tricubic interpolant a spline table is.  Along each axis
 // cell c of a CSpline_Compute table interpolates samples g[c..c+3] at the local
@@ -287,10 +110,8 @@ static Estimator* make_gauss(int kind, int roisize, const GaussCalib& gc) {
 }
 
 // ---------------------------------------------------------------- launch helpers
-static hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
-
-static void run_fit(Estimator* e, int n, const float* d_sample, const float* d_initial, float* d_params, int* d_iter,
-                    float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st, bool timing = true) {
+void run_fit(Estimator* e, int n, const float* d_sample, const float* d_initial, float* d_params, int* d_iter,
+             float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st, bool timing) {
     e->ensure_device();
     const int R = e->roisize;
     if (R < kMinFitR || R > kMaxFitR)
@@ -360,11 +181,15 @@ Context* Context_Create(void) { return new Context(); }
 void Context_Destroy(Context* ctx) {
     if (!ctx) return;
     std::vector<Estimator*> objs;
+    std::vector<EstimationQueue*> queues;
     {
         std::lock_guard<std::mutex> l(ctx->mtx);
         objs.assign(ctx->objects.begin(), ctx->objects.end());
+        queues.assign(ctx->queues.begin(), ctx->queues.end());
         ctx->objects.clear();
+        ctx->queues.clear();
     }
+    for (EstimationQueue* q : queues) queue_destroy(q);
     for (Estimator* e : objs) {
         e->ctx = nullptr;
         delete e;
@@ -793,197 +618,3 @@ int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const fl
 
 }  // extern "C"
 
-// ---------------------------------------------------------------- mixed batch
-struct MixedBatch {
-    std::vector<Estimator*> est;
-    std::vector<int> counts;
-    std::vector<int> branch;              // stream of each bin
-    std::vector<std::vector<int>> order;  // bins of each stream, costliest first
-    std::vector<hipStream_t> streams;     // one per branch
-    std::vector<hipEvent_t> joins;        // one per branch
-    hipEvent_t fork = nullptr;
-    hipStream_t capStream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipStream_t lastLaunch = nullptr;       // stream of the last graph launch
-    std::vector<const void*> capturedPtrs;  // buffers the graph was captured with
-
-    void drop_graph() {
-        if (exec) (void)hipStreamSynchronize(lastLaunch);  // the last replay has finished
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
-        capturedPtrs.clear();
-    }
-
-    ~MixedBatch() {
-        drop_graph();
-        for (hipStream_t s : streams) (void)hipStreamSynchronize(s);
-        for (hipStream_t s : streams) (void)hipStreamDestroy(s);
-        for (hipEvent_t ev : joins) (void)hipEventDestroy(ev);
-        if (fork) (void)hipEventDestroy(fork);
-        if (capStream) (void)hipStreamDestroy(capStream);
-    }
-};
-
-// Relative cost of one pixel-iteration per model (grid spline included), for the
-// longest-first assignment of bins to branches.
-static double bin_cost(const Estimator* e, int count) {
-    double c = 1.0;
-    switch (e->desc.kind) {
-    case kKindFixed: c = 1.0; break;
-    case kKindSigma: c = 1.3; break;
-    case kKindAstig: c = 1.4; break;
-    default: c = e->desc.useGrid ? 4.0 : 40.0; break;
-    }
-    return c * (double)count * e->roisize * e->roisize;
-}
-
-// Launches every bin: each branch waits for `fork` on its stream, runs its bins in
-// order, and records its join event; the caller's stream waits for all joins.
-static void launch_bins(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
-                        float* const* d_crlb, float* const* d_chisq, hipStream_t origin) {
-    hip_check(hipEventRecord(mb->fork, origin), "hipEventRecord(fork)");
-    for (size_t b = 0; b < mb->streams.size(); b++) {
-        hipStream_t s = mb->streams[b];
-        hip_check(hipStreamWaitEvent(s, mb->fork, 0), "hipStreamWaitEvent(fork)");
-        for (int bin : mb->order[b]) {
-            Estimator* e = mb->est[bin];
-            std::lock_guard<std::mutex> l(e->mtx);
-            run_fit(e, mb->counts[bin], d_samples[bin], nullptr, d_params[bin], d_iters ? d_iters[bin] : nullptr,
-                    nullptr, 0, d_crlb ? d_crlb[bin] : nullptr, d_chisq ? d_chisq[bin] : nullptr, s, false);
-        }
-        hip_check(hipEventRecord(mb->joins[b], s), "hipEventRecord(join)");
-        hip_check(hipStreamWaitEvent(origin, mb->joins[b], 0), "hipStreamWaitEvent(join)");
-    }
-}
-
-extern "C" {
-
-MixedBatch* MixedBatch_Create(int nbins, Estimator* const* ests, const int* counts, int nstreams) {
-    return guarded(
-        [&]() -> MixedBatch* {
-            if (nbins <= 0 || !ests || !counts) throw std::runtime_error("photonpy_amd: empty mixed batch");
-            if (nstreams <= 0) nstreams = 4;
-            nstreams = std::min(nstreams, nbins);
-            auto* mb = new MixedBatch();
-            try {
-                std::vector<std::pair<double, int>> cost;
-                for (int b = 0; b < nbins; b++) {
-                    Estimator* e = ests[b];
-                    if (!e) throw std::runtime_error("photonpy_amd: NULL estimator in mixed batch");
-                    if (counts[b] < 0) throw std::runtime_error("photonpy_amd: negative spot count");
-                    const int R = e->roisize;
-                    if (R < kMinFitR || R > kMaxFitR)
-                        throw std::runtime_error("photonpy_amd: fit kernels are compiled for roisize " +
-                                                 std::to_string(kMinFitR) + ".." + std::to_string(kMaxFitR) +
-                                                 ", got " + std::to_string(R));
-                    mb->est.push_back(e);
-                    mb->counts.push_back(counts[b]);
-                    cost.push_back({bin_cost(e, counts[b]), b});
-                }
-                // longest-processing-time-first onto the least-loaded branch
-                std::stable_sort(cost.begin(), cost.end(), [](auto& x, auto& y) { return x.first > y.first; });
-                std::vector<double> load(nstreams, 0.0);
-                mb->branch.assign(nbins, 0);
-                mb->order.assign(nstreams, {});
-                for (auto& c : cost) {
-                    const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-                    load[s] += c.first;
-                    mb->branch[c.second] = s;
-                    mb->order[s].push_back(c.second);
-                }
-                for (int i = 0; i < nstreams; i++) {
-                    hipStream_t st;
-                    hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
-                    mb->streams.push_back(st);
-                    hipEvent_t ev;
-                    hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
-                    mb->joins.push_back(ev);
-                }
-                hip_check(hipEventCreateWithFlags(&mb->fork, hipEventDisableTiming), "hipEventCreate");
-            } catch (...) {
-                delete mb;
-                throw;
-            }
-            return mb;
-        },
-        nullptr);
-}
-
-void MixedBatch_Delete(MixedBatch* mb) { delete mb; }
-
-int MixedBatch_NumStreams(MixedBatch* mb) { return mb ? (int)mb->streams.size() : -1; }
-
-int MixedBatch_Run(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
-                   float* const* d_crlb, float* const* d_chisq, int useGraph, void* stream) {
-    if (!mb) return 1;
-    return guarded(
-        [&]() -> int {
-            hipStream_t st = as_stream(stream);
-            const size_t nb = mb->est.size();
-            for (size_t b = 0; b < nb; b++)
-                if (mb->counts[b] > 0 && (!d_samples[b] || !d_params[b]))
-                    throw std::runtime_error("photonpy_amd: NULL sample or params buffer");
-            for (Estimator* e : mb->est) {
-                // device tables and counters exist before any capture (no allocation inside it)
-                std::lock_guard<std::mutex> l(e->mtx);
-                e->ensure_device();
-                e->ensure_counters();
-            }
-            if (!useGraph) {
-                launch_bins(mb, d_samples, d_params, d_iters, d_crlb, d_chisq, st);
-                return 0;
-            }
-            std::vector<const void*> ptrs;
-            for (size_t b = 0; b < nb; b++) {
-                ptrs.push_back(d_samples[b]);
-                ptrs.push_back(d_params[b]);
-                ptrs.push_back(d_iters ? d_iters[b] : nullptr);
-                ptrs.push_back(d_crlb ? d_crlb[b] : nullptr);
-                ptrs.push_back(d_chisq ? d_chisq[b] : nullptr);
-            }
-            if (!mb->exec || ptrs != mb->capturedPtrs) {
-                // the kernels' buffers are baked into the graph: capture again for new ones
-                mb->drop_graph();
-                if (!mb->capStream) hip_check(hipStreamCreateWithFlags(&mb->capStream, hipStreamNonBlocking),
-                                              "hipStreamCreate");
-                hip_check(hipStreamBeginCapture(mb->capStream, hipStreamCaptureModeRelaxed), "begin capture");
-                try {
-                    launch_bins(mb, d_samples, d_params, d_iters, d_crlb, d_chisq, mb->capStream);
-                } catch (...) {
-                    hipGraph_t g = nullptr;
-                    (void)hipStreamEndCapture(mb->capStream, &g);
-                    if (g) (void)hipGraphDestroy(g);
-                    throw;
-                }
-                hip_check(hipStreamEndCapture(mb->capStream, &mb->graph), "end capture");
-                hip_check(hipGraphInstantiate(&mb->exec, mb->graph, nullptr, nullptr, 0), "graph instantiate");
-                mb->capturedPtrs = ptrs;
-            }
-            hip_check(hipGraphLaunch(mb->exec, st), "graph launch");
-            mb->lastLaunch = st;
-            return 0;
-        },
-        2);
-}
-
-// The same bins as one plain launch per bin, one after another on the caller's stream.
-int MixedBatch_RunPlain(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
-                        float* const* d_crlb, float* const* d_chisq, void* stream) {
-    if (!mb) return 1;
-    return guarded(
-        [&]() -> int {
-            for (size_t b = 0; b < mb->est.size(); b++) {
-                Estimator* e = mb->est[b];
-                std::lock_guard<std::mutex> l(e->mtx);
-                run_fit(e, mb->counts[b], d_samples[b], nullptr, d_params[b], d_iters ? d_iters[b] : nullptr, nullptr,
-                        0, d_crlb ? d_crlb[b] : nullptr, d_chisq ? d_chisq[b] : nullptr, as_stream(stream), false);
-            }
-            return 0;
-        },
-        2);
-}
-
-}  // extern "C"

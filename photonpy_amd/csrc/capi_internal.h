@@ -1,0 +1,201 @@
+// Objects and helpers shared by the C-ABI sources (capi.hip, batch.hip, queue.hip):
+// the last-error string, HIP error checks, Context and Estimator, and the fused
+// fit launch every entry point goes through.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/photonpy_amd.h"
+#include "dispatch.h"
+
+// Context and Estimator are the opaque types of include/photonpy_amd.h, so they and
+// these helpers live in the global namespace.
+using namespace ppa;
+
+// ---------------------------------------------------------------- errors
+inline thread_local std::string g_lastError;
+
+inline void set_error(const std::string& msg) { g_lastError = msg; }
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------- objects
+struct EstimationQueue;
+void queue_destroy(EstimationQueue* q);  // queue.hip
+
+struct Context {
+    std::mutex mtx;
+    std::unordered_set<Estimator*> objects;
+    std::unordered_set<EstimationQueue*> queues;  // deleted before the estimators they use
+};
+
+struct Estimator {
+    Context* ctx = nullptr;
+    std::mutex mtx;
+    int roisize = 0;
+    ModelDesc desc{};
+    std::string fmt;
+    std::vector<ParamLimit> limits;
+    LMParams lm{30, 1e-15f, 0.0f};  // Estimator.cpp:24
+    int group = 0;                  // 0: default_group(roisize)
+    int refill = 0;                 // stopped groups per batched refill (0: kernel default)
+    // spline coefficients: host copy (owned) and lazily uploaded device copy
+    std::vector<float> hostCoefs;
+    float* devCoefs = nullptr;
+    // grid form of the spline (the samples the table interpolates), when the table
+    // is a CSpline_Compute interpolant; used unless the caller asks for coefficients
+    std::vector<float> hostGrid;
+    float* devGrid = nullptr;
+    bool gridForm = false;     // table verified to be a grid interpolant
+    int splineMode = 0;        // 0: grid form when available, 1: always coefficients
+    // grow-only device workspace for the host-pointer API
+    void* ws = nullptr;
+    size_t wsBytes = 0;
+
+    float* devShared = nullptr;  // device copy of the shared PSF parameters (Gaussian models)
+    // Work counters of the fit kernel: one slot per launch, cycling, so launches of this
+    // estimator on different streams do not share a counter unless 64 are in flight.
+    static constexpr int kCounterSlots = 64;
+    int* devCounters = nullptr;
+    unsigned counterSlot = 0;
+
+    // Optional kernel timing: three events per fit launch (before the fit kernel,
+    // after it, after the CRLB kernel), read and released by Estimator_GetPhaseTimes.
+    bool phaseTiming = false;
+    std::vector<hipEvent_t> phaseEvents;
+
+    void release_phase_events() {
+        for (hipEvent_t ev : phaseEvents) (void)hipEventDestroy(ev);
+        phaseEvents.clear();
+    }
+
+    void ensure_counters() {
+        if (!devCounters) hip_check(hipMalloc(&devCounters, kCounterSlots * sizeof(int)), "hipMalloc(counters)");
+    }
+
+    int* next_counter() {
+        ensure_counters();
+        return devCounters + (counterSlot++ % kCounterSlots);
+    }
+
+    int K() const { return num_params(desc.kind); }
+
+    // Shared PSF parameters in the adjoint's layout
+    std::vector<float> shared_values() const {
+        if (desc.kind == kKindFixed) return {desc.gc.sx, desc.gc.sy};
+        if (desc.kind == kKindAstig) {
+            std::vector<float> v(desc.gc.ax, desc.gc.ax + 5);
+            v.insert(v.end(), desc.gc.ay, desc.gc.ay + 5);
+            return v;
+        }
+        if (desc.kind == kKindSpline) return hostCoefs;
+        return {};
+    }
+    size_t num_shared() const {
+        if (desc.kind == kKindSpline) return hostCoefs.size();
+        return shared_values().size();
+    }
+    const float* device_shared() {
+        ensure_device();
+        if (desc.kind == kKindSpline) return devCoefs;
+        std::vector<float> v = shared_values();
+        if (v.empty()) return nullptr;
+        if (!devShared) {
+            hip_check(hipMalloc(&devShared, v.size() * sizeof(float)), "hipMalloc(shared)");
+            hip_check(hipMemcpy(devShared, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice), "upload shared");
+        }
+        return devShared;
+    }
+    int S() const { return roisize * roisize; }
+    int G() const { return group > 0 ? group : default_group_for(desc, roisize); }
+
+    ~Estimator() {
+        if (devCoefs) (void)hipFree(devCoefs);
+        if (devGrid) (void)hipFree(devGrid);
+        if (devShared) (void)hipFree(devShared);
+        if (devCounters) (void)hipFree(devCounters);
+        if (ws) (void)hipFree(ws);
+        release_phase_events();
+    }
+
+    void ensure_device() {
+        if (desc.kind == kKindSpline && !devCoefs) {
+            const size_t bytes = hostCoefs.size() * sizeof(float);
+            hip_check(hipMalloc(&devCoefs, bytes), "hipMalloc(spline coefficients)");
+            hip_check(hipMemcpy(devCoefs, hostCoefs.data(), bytes, hipMemcpyHostToDevice), "upload spline coefficients");
+            desc.sm.coefs = devCoefs;
+        }
+        if (desc.kind == kKindSpline && desc.useGrid && !devGrid) {
+            const size_t bytes = hostGrid.size() * sizeof(float);
+            hip_check(hipMalloc(&devGrid, bytes), "hipMalloc(spline grid)");
+            hip_check(hipMemcpy(devGrid, hostGrid.data(), bytes, hipMemcpyHostToDevice), "upload spline grid");
+            desc.sg.grid = devGrid;
+        }
+    }
+
+    void* workspace(size_t bytes) {
+        if (bytes > wsBytes) {
+            if (ws) hip_check(hipFree(ws), "hipFree(workspace)");
+            ws = nullptr;
+            wsBytes = 0;
+            hip_check(hipMalloc(&ws, bytes), "hipMalloc(workspace)");
+            wsBytes = bytes;
+        }
+        return ws;
+    }
+};
+
+inline void attach(Estimator* e, Context* ctx) {
+    if (!ctx) return;
+    std::lock_guard<std::mutex> l(ctx->mtx);
+    ctx->objects.insert(e);
+    e->ctx = ctx;
+}
+
+inline void detach(Estimator* e) {
+    if (!e->ctx) return;
+    std::lock_guard<std::mutex> l(e->ctx->mtx);
+    e->ctx->objects.erase(e);
+    e->ctx = nullptr;
+}
+
+template <class F>
+inline auto guarded(F fn, decltype(fn()) fallback) -> decltype(fn()) {
+    try {
+        return fn();
+    } catch (const std::exception& ex) {
+        set_error(ex.what());
+        return fallback;
+    }
+}
+
+inline void require_gpu(bool cuda) {
+    if (!cuda)
+        throw std::runtime_error("photonpy_amd: CPU estimators are not provided; create the estimator with cuda=True");
+}
+
+// ---------------------------------------------------------------- launch helpers
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Fused fit (+ CRLB / chi^2 when d_crlb / d_chisq are given) of n spots on st with the
+// estimator's model, LM settings, lane-group size, refill batch and next work counter;
+// timing = false keeps phase-timing events out (graph capture).
+void run_fit(Estimator* e, int n, const float* d_sample, const float* d_initial, float* d_params, int* d_iter,
+             float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st, bool timing = true);

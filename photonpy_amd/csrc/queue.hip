@@ -1,0 +1,353 @@
+// EstimationQueue (EstimationQueue.h:27-108, EstimationQueue.cpp:11-412): spots are
+// scheduled from host memory into pinned batches; a worker thread launches full
+// (or flushed) batches round-robin on a few HIP streams -- copy in, fused fit + CRLB
+// + chi^2, copy out -- and moves completed batches to a result list that
+// GetResults drains.  Same entry points, argument meaning and queue semantics as
+// the reference:
+//   - Schedule blocks (5 ms polls) while GetQueueLength() >= maxQueueLen, where the
+//     queue length is the spots in scheduled-but-unlaunched batches plus the number
+//     of batches in flight (EstimationQueue.cpp:219-226, 160-171);
+//   - a call with initial values switches the queue to initial values for every
+//     later launch (:223);
+//   - GetResults copies whole finished batches, newest first, while they fit in the
+//     space left, and returns the number of spots copied (:117-158);
+//   - IsIdle ignores the batch still being filled; Flush queues it (:175-205).
+// Differences: batches are fitted by this library's fused kernels (CRLB from the
+// fit's own second kernel), diagnostics are empty (numDiag = 0, as the reference's
+// GPU path), and maxQueueLen < 1 is treated as 1 (the reference never returns).
+#include <atomic>
+#include <chrono>
+#include <deque>
+#include <memory>
+#include <thread>
+
+#include "capi_internal.h"
+
+namespace {
+
+template <class T>
+T* pinned(size_t n) {
+    void* p = nullptr;
+    if (n == 0) return nullptr;
+    hip_check(hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault), "hipHostMalloc(queue batch)");
+    return static_cast<T*>(p);
+}
+
+template <class T>
+T* device_alloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) return nullptr;
+    hip_check(hipMalloc(&p, n * sizeof(T)), "hipMalloc(queue stream buffers)");
+    return static_cast<T*>(p);
+}
+
+}  // namespace
+
+struct EstimationQueue {
+    struct Batch {
+        std::vector<int> ids;
+        float* samples = nullptr;  // pinned [batch, S]
+        float* params = nullptr;   // pinned [batch, K]: initial values in, estimates out
+        float* crlb = nullptr;     // pinned [batch, K]
+        float* chisq = nullptr;    // pinned [batch]
+        int* iterations = nullptr; // pinned [batch]
+        std::vector<int> roipos;   // [batch, 2], returned with the results
+        int numspots = 0;
+
+        Batch(int n, int S, int K) : ids(n), roipos((size_t)n * 2) {
+            samples = pinned<float>((size_t)n * S);
+            params = pinned<float>((size_t)n * K);
+            crlb = pinned<float>((size_t)n * K);
+            chisq = pinned<float>(n);
+            iterations = pinned<int>(n);
+        }
+        ~Batch() {
+            for (void* p : {(void*)samples, (void*)params, (void*)crlb, (void*)chisq, (void*)iterations})
+                if (p) (void)hipHostFree(p);
+        }
+    };
+
+    struct Lane {
+        hipStream_t stream = nullptr;
+        hipEvent_t done = nullptr;
+        std::unique_ptr<Batch> current;
+        float *samples = nullptr, *params = nullptr, *crlb = nullptr, *chisq = nullptr;
+        int* iterations = nullptr;
+    };
+
+    Estimator* est;
+    Context* ctx;
+    int device = 0;
+    int batchSize, maxQueueLen, K, S;
+    bool keepSamples;
+    std::atomic<bool> useInitial{false};
+    std::vector<Lane> lanes;
+
+    std::deque<std::unique_ptr<Batch>> todo, finished, recycle;
+    int numActive = 0;
+    std::mutex todoMtx, finishedMtx, recycleMtx, scheduleMtx;
+    std::unique_ptr<Batch> next;
+
+    std::thread worker;
+    std::atomic<bool> stop{false};
+    std::string workerError;  // first launch failure, reported by later calls
+
+    EstimationQueue(Estimator* e, int batch, int maxLen, bool keep, int streams, Context* c)
+        : est(e), ctx(c), batchSize(batch), maxQueueLen(std::max(1, maxLen)), K(e->K()), S(e->S()),
+          keepSamples(keep) {
+        if (streams < 0) streams = 3;  // EstimationQueue.cpp:16-17
+        if (streams == 0) streams = 1;
+        hip_check(hipGetDevice(&device), "hipGetDevice");
+        {
+            std::lock_guard<std::mutex> l(est->mtx);
+            est->ensure_device();
+            est->ensure_counters();
+        }
+        lanes.resize(streams);
+        for (Lane& ln : lanes) {
+            hip_check(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking), "hipStreamCreate");
+            hip_check(hipEventCreateWithFlags(&ln.done, hipEventDisableTiming), "hipEventCreate");
+            ln.samples = device_alloc<float>((size_t)batchSize * S);
+            ln.params = device_alloc<float>((size_t)batchSize * K);
+            ln.crlb = device_alloc<float>((size_t)batchSize * K);
+            ln.chisq = device_alloc<float>(batchSize);
+            ln.iterations = device_alloc<int>(batchSize);
+        }
+        next = std::make_unique<Batch>(batchSize, S, K);
+        worker = std::thread([this]() { run(); });
+    }
+
+    ~EstimationQueue() {
+        stop = true;
+        if (worker.joinable()) worker.join();
+        for (Lane& ln : lanes) {
+            if (ln.stream) (void)hipStreamSynchronize(ln.stream);
+            for (void* p : {(void*)ln.samples, (void*)ln.params, (void*)ln.crlb, (void*)ln.chisq, (void*)ln.iterations})
+                if (p) (void)hipFree(p);
+            if (ln.done) (void)hipEventDestroy(ln.done);
+            if (ln.stream) (void)hipStreamDestroy(ln.stream);
+        }
+    }
+
+    void launch(Lane& ln, std::unique_ptr<Batch> b) {
+        const int n = b->numspots;
+        hip_check(hipMemcpyAsync(ln.samples, b->samples, sizeof(float) * n * S, hipMemcpyHostToDevice, ln.stream),
+                  "queue: samples to device");
+        const bool init = useInitial;
+        if (init)
+            hip_check(hipMemcpyAsync(ln.params, b->params, sizeof(float) * n * K, hipMemcpyHostToDevice, ln.stream),
+                      "queue: initial values to device");
+        {
+            std::lock_guard<std::mutex> l(est->mtx);
+            run_fit(est, n, ln.samples, init ? ln.params : nullptr, ln.params, ln.iterations, nullptr, 0, ln.crlb,
+                    ln.chisq, ln.stream, false);
+        }
+        hip_check(hipMemcpyAsync(b->params, ln.params, sizeof(float) * n * K, hipMemcpyDeviceToHost, ln.stream),
+                  "queue: estimates to host");
+        hip_check(hipMemcpyAsync(b->crlb, ln.crlb, sizeof(float) * n * K, hipMemcpyDeviceToHost, ln.stream),
+                  "queue: CRLB to host");
+        hip_check(hipMemcpyAsync(b->chisq, ln.chisq, sizeof(float) * n, hipMemcpyDeviceToHost, ln.stream),
+                  "queue: chi^2 to host");
+        hip_check(hipMemcpyAsync(b->iterations, ln.iterations, sizeof(int) * n, hipMemcpyDeviceToHost, ln.stream),
+                  "queue: iterations to host");
+        hip_check(hipEventRecord(ln.done, ln.stream), "hipEventRecord");
+        ln.current = std::move(b);
+    }
+
+    // Worker thread (EstimationQueue.cpp:54-99): start queued batches on idle
+    // streams, collect completed ones; sleep 1 ms when nothing changed.
+    void run() {
+        (void)hipSetDevice(device);
+        while (!stop) {
+            bool idle = true;
+            for (Lane& ln : lanes) {
+                if (!ln.current) {
+                    std::unique_ptr<Batch> b;
+                    {
+                        std::lock_guard<std::mutex> l(todoMtx);
+                        if (!todo.empty()) {
+                            b = std::move(todo.front());
+                            todo.pop_front();
+                            numActive++;
+                        }
+                    }
+                    if (b) {
+                        try {
+                            launch(ln, std::move(b));
+                        } catch (const std::exception& ex) {
+                            std::lock_guard<std::mutex> l(todoMtx);
+                            if (workerError.empty()) workerError = ex.what();
+                            numActive--;
+                        }
+                        idle = false;
+                    }
+                }
+                if (ln.current && hipEventQuery(ln.done) == hipSuccess) {
+                    {
+                        std::lock_guard<std::mutex> l(finishedMtx);
+                        finished.push_back(std::move(ln.current));
+                    }
+                    {
+                        std::lock_guard<std::mutex> l(todoMtx);
+                        numActive--;
+                    }
+                    idle = false;
+                }
+            }
+            if (idle) std::this_thread::sleep_for(std::chrono::milliseconds(1));
+        }
+    }
+
+    void check_worker() {
+        std::lock_guard<std::mutex> l(todoMtx);
+        if (!workerError.empty()) throw std::runtime_error(workerError);
+    }
+
+    int queue_length() {
+        std::lock_guard<std::mutex> l(todoMtx);
+        int sum = 0;
+        for (auto& b : todo) sum += b->numspots;
+        return sum + numActive;
+    }
+
+    bool is_idle() {
+        std::lock_guard<std::mutex> l(todoMtx);
+        return todo.empty() && numActive == 0;
+    }
+
+    // moves the batch being filled to the queue (scheduleMtx held by the caller)
+    void flush_locked() {
+        if (next->numspots == 0) return;
+        {
+            std::lock_guard<std::mutex> l(todoMtx);
+            todo.push_back(std::move(next));
+        }
+        {
+            std::lock_guard<std::mutex> l(recycleMtx);
+            if (!recycle.empty()) {
+                next = std::move(recycle.front());
+                recycle.pop_front();
+            }
+        }
+        if (!next) next = std::make_unique<Batch>(batchSize, S, K);
+    }
+
+    void schedule(int count, const int* ids, const float* samples, const float* initial, const int* roipos) {
+        check_worker();
+        while (queue_length() >= maxQueueLen) std::this_thread::sleep_for(std::chrono::milliseconds(5));
+        useInitial = initial != nullptr;
+        std::lock_guard<std::mutex> l(scheduleMtx);
+        for (int i = 0; i < count; i++) {
+            Batch& b = *next;
+            const int j = b.numspots++;
+            b.ids[j] = ids ? ids[i] : 0;
+            std::memcpy(b.samples + (size_t)j * S, samples + (size_t)i * S, sizeof(float) * S);
+            b.roipos[2 * j] = roipos ? roipos[2 * i] : 0;
+            b.roipos[2 * j + 1] = roipos ? roipos[2 * i + 1] : 0;
+            if (initial) std::memcpy(b.params + (size_t)j * K, initial + (size_t)i * K, sizeof(float) * K);
+            if (b.numspots == batchSize) flush_locked();
+        }
+    }
+
+    int result_count() {
+        std::lock_guard<std::mutex> l(finishedMtx);
+        int total = 0;
+        for (auto& b : finished) total += b->numspots;
+        return total;
+    }
+
+    int results(int count, float* estim, float* diag, float* crlb, int* roipos, float* samples,
+                EstimatorResult* res) {
+        (void)diag;  // numDiag = 0
+        int copied = 0;
+        while (copied < count) {
+            const int space = count - copied;
+            std::unique_ptr<Batch> b;
+            {
+                std::lock_guard<std::mutex> l(finishedMtx);
+                if (finished.empty() || finished.back()->numspots > space) return copied;
+                b = std::move(finished.back());
+                finished.pop_back();
+            }
+            const int n = b->numspots;
+            for (int i = 0; i < n; i++) {
+                if (res) {
+                    res[copied + i].id = b->ids[i];
+                    res[copied + i].chisq = b->chisq[i];
+                    res[copied + i].iterations = b->iterations[i];
+                }
+            }
+            if (estim) std::memcpy(estim + (size_t)copied * K, b->params, sizeof(float) * n * K);
+            if (crlb) std::memcpy(crlb + (size_t)copied * K, b->crlb, sizeof(float) * n * K);
+            if (roipos) std::memcpy(roipos + (size_t)copied * 2, b->roipos.data(), sizeof(int) * n * 2);
+            if (samples && keepSamples) std::memcpy(samples + (size_t)copied * S, b->samples, sizeof(float) * n * S);
+            copied += n;
+            b->numspots = 0;
+            std::lock_guard<std::mutex> l(recycleMtx);
+            recycle.push_back(std::move(b));
+        }
+        return copied;
+    }
+};
+
+void queue_destroy(EstimationQueue* q) { delete q; }
+
+extern "C" {
+
+EstimationQueue* EstimQueue_Create(Estimator* psf, int batchSize, int maxQueueLen, bool keepSamples, int numStreams,
+                                   Context* ctx) {
+    return guarded(
+        [&]() -> EstimationQueue* {
+            if (!psf) throw std::runtime_error("photonpy_amd: EstimationQueue needs an estimator");
+            if (batchSize < 1) throw std::runtime_error("photonpy_amd: batchSize must be positive");
+            auto* q = new EstimationQueue(psf, batchSize, maxQueueLen, keepSamples, numStreams, ctx);
+            if (ctx) {
+                std::lock_guard<std::mutex> l(ctx->mtx);
+                ctx->queues.insert(q);
+            }
+            return q;
+        },
+        nullptr);
+}
+
+void EstimQueue_Delete(EstimationQueue* q) {
+    if (!q) return;
+    if (q->ctx) {
+        std::lock_guard<std::mutex> l(q->ctx->mtx);
+        q->ctx->queues.erase(q);
+    }
+    delete q;
+}
+
+void EstimQueue_Schedule(EstimationQueue* q, int numspots, const int* ids, const float* h_samples,
+                         const float* h_initial, const float* h_constants, const int* h_roipos) {
+    (void)h_constants;  // none of the models has per-spot constants
+    if (!q || numspots <= 0) return;
+    guarded(
+        [&]() -> int {
+            if (!h_samples) throw std::runtime_error("photonpy_amd: NULL samples");
+            q->schedule(numspots, ids, h_samples, h_initial, h_roipos);
+            return 0;
+        },
+        0);
+}
+
+void EstimQueue_Flush(EstimationQueue* q) {
+    if (!q) return;
+    std::lock_guard<std::mutex> l(q->scheduleMtx);
+    q->flush_locked();
+}
+
+bool EstimQueue_IsIdle(EstimationQueue* q) { return q ? q->is_idle() : true; }
+
+int EstimQueue_GetResultCount(EstimationQueue* q) { return q ? q->result_count() : 0; }
+
+int EstimQueue_GetQueueLength(EstimationQueue* q) { return q ? q->queue_length() : 0; }
+
+int EstimQueue_GetResults(EstimationQueue* q, int maxresults, float* estim, float* diag, float* crlb, int* roipos,
+                          float* samples, EstimatorResult* result) {
+    if (!q) return 0;
+    return guarded([&]() -> int { return q->results(maxresults, estim, diag, crlb, roipos, samples, result); }, 0);
+}
+
+}  // extern "C"

@@ -43,6 +43,10 @@ class LevMarParams(ct.Structure):
     _fields_ = [("iterations", ct.c_int32), ("lambda_", ct.c_float), ("factor", ct.c_float)]
 
 
+# EstimatorResult (Estimators/EstimationQueue.h:21-25)
+EstimResultDType = np.dtype([("id", "<i4"), ("chisq", "<f4"), ("iterations", "<i4")])
+
+
 class NativeError(RuntimeError):
     """Raised when a call into libphotonpy_amd reports an error."""
 
@@ -118,6 +122,17 @@ def _bind(lib):
         "Estimator_ChiSquareAndCRLBDevice": ([V, I, V, V, V, V, V], ct.c_int),
         "Estimator_NLLGradDevice": ([V, I, V, V, V, V, V, V, V, V], ct.c_int),
         "Estimator_NumShared": ([V], ct.c_int),
+        "EstimQueue_Create": ([V, I, I, ct.c_bool, I, V], V),
+        "EstimQueue_Delete": ([V], None),
+        "EstimQueue_Schedule": ([V, I, IntArray, FloatArray, NullableFloatArrayType, NullableFloatArrayType,
+                                 IntArray], None),
+        "EstimQueue_Flush": ([V], None),
+        "EstimQueue_IsIdle": ([V], ct.c_bool),
+        "EstimQueue_GetResultCount": ([V], ct.c_int),
+        "EstimQueue_GetQueueLength": ([V], ct.c_int),
+        "EstimQueue_GetResults": ([V, I, FloatArray, NullableFloatArrayType, NullableFloatArrayType, IntArray,
+                                   NullableFloatArrayType, ctl.ndpointer(EstimResultDType, flags="aligned, c_contiguous")],
+                                  ct.c_int),
         "MixedBatch_Create": ([I, ct.POINTER(V), ct.POINTER(ct.c_int), I], V),
         "MixedBatch_Delete": ([V], None),
         "MixedBatch_NumStreams": ([V], ct.c_int),

@@ -122,3 +122,24 @@ def test_context_destroys_estimators():
     for _ in range(3):
         g.CreatePSF_XYIBg(10, 1.5, True)
     c.Destroy()
+
+
+def test_estimator_result_layout():
+    from photonpy_amd.lib import EstimResultDType
+    assert EstimResultDType.itemsize == 12
+    assert [EstimResultDType.fields[n][1] for n in ("id", "chisq", "iterations")] == [0, 4, 8]
+
+
+def test_queue_results_filtering():
+    from photonpy_amd.estim_queue import EstimQueue_Results
+    n = 6
+    ids = np.array([3, 1, 5, 0, 4, 2], np.int32)
+    est = np.arange(n * 4, dtype=np.float32).reshape(n, 4)
+    r = EstimQueue_Results(["x", "y", "I", "bg"], (10, 10), est.copy(), np.zeros((n, 0), np.float32),
+                           est.copy() * 2, np.arange(n), np.arange(n) * 0.5, np.zeros((n, 2), np.int32), None, ids)
+    r.SortByID(isUnique=True)
+    assert np.array_equal(r.ids, np.arange(n))
+    assert np.array_equal(r.estim, est[np.argsort(ids)])
+    r.Filter(r.estim[:, 0] > 8)
+    assert np.all(r.estim[:, 0] > 8) and len(r.ids) == len(r.chisq) == len(r.crlb)
+    assert r.ColIdx("I") == 2
