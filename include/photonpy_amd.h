@@ -95,6 +95,13 @@ Estimator* CSpline_CreatePSF(int roisize, const CSpline_Calibration* calib, cons
 /* Estimators/CSpline/CubicSplinePSF.cu:151-204 — shape {Nz,Ny,Nx};
  * splineCoefs [(Nz-3+2b),(Ny-3+2b),(Nx-3+2b),64] */
 void CSpline_Compute(const int* shape, const float* data, float* splineCoefs, int border);
+/* Estimators/CSpline/CubicSplinePSF.cu:135-146 — a .zstack file (24-byte header {version 1,
+ * shape[3], zmin, zmax} + float32 samples) to a spline PSF (border 0).  The calibration takes
+ * the coefficient table's shape (z-stack shape - 3); the reference passes the z-stack shape. */
+Estimator* CSpline_CreatePSF_FromFile(int roisize, const char* filename, int mode, bool cuda, Context* ctx);
+/* Estimators/CSpline/CubicSplinePSF.cu:208-271 — shape {Nz,Ny,Nx} of splineCoefs[Nz,Ny,Nx,64],
+ * shift {z,y,x}; values[(Nz-2b)(Ny-2b)(Nx-2b), 4] = value, d/dx, d/dy, d/dz. */
+void CSpline_Evaluate(const int* shape, const float* splineCoefs, const float* shift, int border, float* values);
 
 /* Estimators/Estimator.cpp:45-100 */
 void Estimator_Delete(Estimator* e);

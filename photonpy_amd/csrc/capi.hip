@@ -6,6 +6,36 @@
 
 namespace ppa {
 hipError_t launch_spline_coefs(const int* shape, const float* d_data, float* d_out, int border, hipStream_t st);
+hipError_t launch_spline_evaluate(const int* shape, const float* d_coefs, const float* shift, int border,
+                                  float* d_out, hipStream_t st);
+}
+
+// .zstack file (CubicSplinePSF.cu:109-133, smlm/psf.py:256-276): a packed 24-byte header
+// {int version = 1; int shape[3] (z, y, x); float zmin, zmax} and the float32 samples.
+struct ZStackHeader {
+    int version;
+    int shape[3];
+    float zmin, zmax;
+};
+static_assert(sizeof(ZStackHeader) == 24, "zstack header is 24 bytes");
+
+static std::vector<float> load_zstack(const char* filename, ZStackHeader& h) {
+    if (!filename) throw std::runtime_error("photonpy_amd: NULL file name");
+    FILE* f = std::fopen(filename, "rb");
+    if (!f) throw std::runtime_error(std::string("Can't open file ") + filename);
+    std::vector<float> data;
+    const bool ok = std::fread(&h, sizeof(h), 1, f) == 1;
+    if (ok && h.version == 1 && h.shape[0] > 0 && h.shape[1] > 0 && h.shape[2] > 0) {
+        data.resize((size_t)h.shape[0] * h.shape[1] * h.shape[2]);
+        const size_t got = std::fread(data.data(), sizeof(float), data.size(), f);
+        std::fclose(f);
+        if (got != data.size()) throw std::runtime_error(std::string("truncated zstack file ") + filename);
+        return data;
+    }
+    std::fclose(f);
+    if (!ok) throw std::runtime_error(std::string("truncated zstack file ") + filename);
+    if (h.version != 1) throw std::runtime_error("Wrong ZStack file version (should be 1)");
+    throw std::runtime_error("photonpy_amd: invalid zstack shape");
 }
 
 using namespace ppa;
@@ -323,6 +353,57 @@ void CSpline_Compute(const int* shape, const float* data, float* splineCoefs, in
             (void)hipFree(din);
             (void)hipFree(dout);
             hip_check(err, "CSpline_Compute");
+            return 0;
+        },
+        0);
+}
+
+Estimator* CSpline_CreatePSF_FromFile(int roisize, const char* filename, int mode, bool cuda, Context* ctx) {
+    return guarded(
+        [&]() -> Estimator* {
+            ZStackHeader h;
+            std::vector<float> data = load_zstack(filename, h);
+            const int cells[3] = {h.shape[0] - 3, h.shape[1] - 3, h.shape[2] - 3};
+            if (cells[0] < 1 || cells[1] < 1 || cells[2] < 1)
+                throw std::runtime_error("photonpy_amd: z-stack too small for a spline");
+            std::vector<float> coefs((size_t)cells[0] * cells[1] * cells[2] * 64);
+            SMLM_ClearError();
+            CSpline_Compute(h.shape, data.data(), coefs.data(), 0);
+            if (!g_lastError.empty()) throw std::runtime_error(g_lastError);
+            // the coefficient table's shape (the reference passes the z-stack's own
+            // shape here, CubicSplinePSF.cu:141, and so reads past its table)
+            CSpline_Calibration calib;
+            calib.shape[0] = cells[0];
+            calib.shape[1] = cells[1];
+            calib.shape[2] = cells[2];
+            calib.z_min = h.zmin;
+            calib.z_max = h.zmax;
+            calib.coefs = coefs.data();
+            Estimator* e = CSpline_CreatePSF(roisize, &calib, nullptr, mode, cuda, ctx);
+            if (!e) throw std::runtime_error(g_lastError);
+            return e;
+        },
+        nullptr);
+}
+
+void CSpline_Evaluate(const int* shape, const float* splineCoefs, const float* shift, int border, float* values) {
+    guarded(
+        [&]() -> int {
+            if (!shape || !splineCoefs || !shift || !values) throw std::runtime_error("photonpy_amd: NULL argument");
+            const int o0 = shape[0] - 2 * border, o1 = shape[1] - 2 * border, o2 = shape[2] - 2 * border;
+            if (shape[0] < 1 || shape[1] < 1 || shape[2] < 1 || o0 <= 0 || o1 <= 0 || o2 <= 0)
+                throw std::runtime_error("photonpy_amd: invalid spline shape or border");
+            const size_t ncoef = (size_t)shape[0] * shape[1] * shape[2] * 64;
+            const size_t nout = (size_t)o0 * o1 * o2 * 4;
+            float *dc = nullptr, *dout = nullptr;
+            hip_check(hipMalloc(&dc, ncoef * sizeof(float)), "hipMalloc");
+            hipError_t err = hipMalloc(&dout, nout * sizeof(float));
+            if (err == hipSuccess) err = hipMemcpy(dc, splineCoefs, ncoef * sizeof(float), hipMemcpyHostToDevice);
+            if (err == hipSuccess) err = launch_spline_evaluate(shape, dc, shift, border, dout, 0);
+            if (err == hipSuccess) err = hipMemcpy(values, dout, nout * sizeof(float), hipMemcpyDeviceToHost);
+            (void)hipFree(dc);
+            if (dout) (void)hipFree(dout);
+            hip_check(err, "CSpline_Evaluate");
             return 0;
         },
         0);

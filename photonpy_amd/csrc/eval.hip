@@ -114,6 +114,58 @@ __global__ void __launch_bounds__(256) spline_coef_kernel(int s0, int s1, int s2
             }
 }
 
+// CSpline_Evaluate (CubicSplinePSF.cu:208-271): value and x/y/z derivatives of the
+// spline at every grid voxel (z, y, x) of [border, shape - border), shifted by
+// `shift` (z, y, x).  The sub-voxel offset d = -shift - trunc(-shift) is the same
+// for every voxel; voxel (z, y, x) uses cell (clamp(trunc(z - shift_z)), ...).
+__global__ void __launch_bounds__(256) spline_evaluate_kernel(int s0, int s1, int s2, const float* __restrict__ coefs,
+                                                              float h0, float h1, float h2, int border,
+                                                              float* __restrict__ out) {
+    const int o0 = s0 - 2 * border, o1 = s1 - 2 * border, o2 = s2 - 2 * border;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)o0 * o1 * o2) return;
+    const int x = border + (int)(idx % o2);
+    const int y = border + (int)((idx / o2) % o1);
+    const int z = border + (int)(idx / ((size_t)o1 * o2));
+    float dz = -h0; dz -= (float)(int)dz;
+    float dy = -h1; dy -= (float)(int)dy;
+    float dx = -h2; dx -= (float)(int)dx;
+    const int k = clampi((int)((float)z - h0), 0, s0 - 1);
+    const int j = clampi((int)((float)y - h1), 0, s1 - 1);
+    const int i = clampi((int)((float)x - h2), 0, s2 - 1);
+    const float* c = coefs + 64 * (((size_t)k * s1 + j) * s2 + i);
+    const float zp[4] = {1.0f, dz, dz * dz, dz * dz * dz}, yp[4] = {1.0f, dy, dy * dy, dy * dy * dy};
+    const float xp[4] = {1.0f, dx, dx * dx, dx * dx * dx};
+    float v = 0.0f, vx = 0.0f, vy = 0.0f, vz = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+#pragma unroll
+            for (int o = 0; o < 4; o++) {
+                const float q = c[m * 16 + n * 4 + o];
+                v += zp[m] * yp[n] * xp[o] * q;
+                if (o > 0) vx += (float)o * zp[m] * yp[n] * xp[o - 1] * q;
+                if (n > 0) vy += (float)n * zp[m] * yp[n - 1] * xp[o] * q;
+                if (m > 0) vz += (float)m * zp[m - 1] * yp[n] * xp[o] * q;
+            }
+    float* dst = out + 4 * idx;
+    dst[0] = v;
+    dst[1] = vx;
+    dst[2] = vy;
+    dst[3] = vz;
+}
+
+hipError_t launch_spline_evaluate(const int* shape, const float* d_coefs, const float* shift, int border,
+                                  float* d_out, hipStream_t st) {
+    const int o0 = shape[0] - 2 * border, o1 = shape[1] - 2 * border, o2 = shape[2] - 2 * border;
+    if (o0 <= 0 || o1 <= 0 || o2 <= 0) return hipErrorInvalidValue;
+    const size_t n = (size_t)o0 * o1 * o2;
+    hipLaunchKernelGGL(spline_evaluate_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, shape[0],
+                       shape[1], shape[2], d_coefs, shift[0], shift[1], shift[2], border, d_out);
+    return hipGetLastError();
+}
+
 __global__ void set_counter_kernel(int* counter, int value) { *counter = value; }
 
 hipError_t launch_set_counter(int* counter, int value, hipStream_t st) {

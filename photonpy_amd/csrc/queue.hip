@@ -4,9 +4,9 @@
 // + chi^2, copy out -- and moves completed batches to a result list that
 // GetResults drains.  Same entry points, argument meaning and queue semantics as
 // the reference:
-//   - Schedule blocks (5 ms polls) while GetQueueLength() >= maxQueueLen, where the
-//     queue length is the spots in scheduled-but-unlaunched batches plus the number
-//     of batches in flight (EstimationQueue.cpp:219-226, 160-171);
+//   - Schedule blocks while GetQueueLength() >= maxQueueLen, where the queue length
+//     is the spots in scheduled-but-unlaunched batches plus the number of batches in
+//     flight (EstimationQueue.cpp:219-226, 160-171);
 //   - a call with initial values switches the queue to initial values for every
 //     later launch (:223);
 //   - GetResults copies whole finished batches, newest first, while they fit in the
@@ -15,8 +15,13 @@
 // Differences: batches are fitted by this library's fused kernels (CRLB from the
 // fit's own second kernel), diagnostics are empty (numDiag = 0, as the reference's
 // GPU path), and maxQueueLen < 1 is treated as 1 (the reference never returns).
+// Waits use condition variables instead of the reference's 1 ms / 5 ms sleeps:
+// Schedule wakes when the worker starts or finishes a batch, and the worker
+// sleeps on new work when nothing is in flight and polls its streams every 50 us
+// when something is.
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <deque>
 #include <memory>
 #include <thread>
@@ -86,6 +91,7 @@ struct EstimationQueue {
     std::deque<std::unique_ptr<Batch>> todo, finished, recycle;
     int numActive = 0;
     std::mutex todoMtx, finishedMtx, recycleMtx, scheduleMtx;
+    std::condition_variable todoCv;  // todo or numActive changed, or stop (todoMtx)
     std::unique_ptr<Batch> next;
 
     std::thread worker;
@@ -118,7 +124,11 @@ struct EstimationQueue {
     }
 
     ~EstimationQueue() {
-        stop = true;
+        {
+            std::lock_guard<std::mutex> l(todoMtx);
+            stop = true;
+        }
+        todoCv.notify_all();
         if (worker.joinable()) worker.join();
         for (Lane& ln : lanes) {
             if (ln.stream) (void)hipStreamSynchronize(ln.stream);
@@ -155,11 +165,11 @@ struct EstimationQueue {
     }
 
     // Worker thread (EstimationQueue.cpp:54-99): start queued batches on idle
-    // streams, collect completed ones; sleep 1 ms when nothing changed.
+    // streams, collect completed ones; wait for work when nothing is in flight.
     void run() {
         (void)hipSetDevice(device);
         while (!stop) {
-            bool idle = true;
+            bool changed = false, inFlight = false;
             for (Lane& ln : lanes) {
                 if (!ln.current) {
                     std::unique_ptr<Batch> b;
@@ -179,7 +189,7 @@ struct EstimationQueue {
                             if (workerError.empty()) workerError = ex.what();
                             numActive--;
                         }
-                        idle = false;
+                        changed = true;
                     }
                 }
                 if (ln.current && hipEventQuery(ln.done) == hipSuccess) {
@@ -191,10 +201,20 @@ struct EstimationQueue {
                         std::lock_guard<std::mutex> l(todoMtx);
                         numActive--;
                     }
-                    idle = false;
+                    changed = true;
                 }
+                inFlight |= (bool)ln.current;
             }
-            if (idle) std::this_thread::sleep_for(std::chrono::milliseconds(1));
+            if (changed) {
+                todoCv.notify_all();
+                continue;
+            }
+            if (inFlight) {
+                std::this_thread::sleep_for(std::chrono::microseconds(50));
+            } else {
+                std::unique_lock<std::mutex> l(todoMtx);
+                todoCv.wait_for(l, std::chrono::milliseconds(20), [&] { return stop || !todo.empty(); });
+            }
         }
     }
 
@@ -222,6 +242,7 @@ struct EstimationQueue {
             std::lock_guard<std::mutex> l(todoMtx);
             todo.push_back(std::move(next));
         }
+        todoCv.notify_all();
         {
             std::lock_guard<std::mutex> l(recycleMtx);
             if (!recycle.empty()) {
@@ -234,7 +255,14 @@ struct EstimationQueue {
 
     void schedule(int count, const int* ids, const float* samples, const float* initial, const int* roipos) {
         check_worker();
-        while (queue_length() >= maxQueueLen) std::this_thread::sleep_for(std::chrono::milliseconds(5));
+        {
+            std::unique_lock<std::mutex> l(todoMtx);
+            todoCv.wait(l, [&] {
+                int len = numActive;
+                for (auto& b : todo) len += b->numspots;
+                return len < maxQueueLen || !workerError.empty();
+            });
+        }
         useInitial = initial != nullptr;
         std::lock_guard<std::mutex> l(scheduleMtx);
         for (int i = 0; i < count; i++) {

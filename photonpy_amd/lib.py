@@ -96,6 +96,8 @@ def _bind(lib):
         "Gauss2D_CreatePSF_XYZIBgQ": ([I, V, I, V], V),
         "CSpline_CreatePSF": ([I, V, NullableFloatArrayType, I, B, V], V),
         "CSpline_Compute": ([IntArray, FloatArray, FloatArray, I], None),
+        "CSpline_CreatePSF_FromFile": ([I, ct.c_char_p, I, B, V], V),
+        "CSpline_Evaluate": ([IntArray, FloatArray, FloatArray, I, FloatArray], None),
         "Estimator_Delete": ([V], None),
         "Estimator_ParamFormat": ([V], ct.c_char_p),
         "Estimator_GetProperties": ([V, ct.POINTER(EstimatorProperties)], None),
