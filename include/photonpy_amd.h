@@ -87,6 +87,21 @@ void Context_Destroy(Context* ctx);
 Estimator* Gauss2D_CreatePSF_XYIBg(int roisize, float sigmaX, float sigmaY, bool cuda, Context* ctx);
 /* Estimators/Gaussian/GaussianPSF.cu:136-158 — params x,y,I,bg,sigma */
 Estimator* Gauss2D_CreatePSF_XYIBgSigma(int roisize, float initialSigma, bool cuda, Context* ctx);
+/* Estimators/Gaussian/GaussianPSF.cu:108-129 — params x,y,I,bg; constants sigma x,y per spot */
+Estimator* Gauss2D_CreatePSF_XYIBgConstSigma(int roisize, bool cuda, Context* ctx);
+/* Estimators/Gaussian/GaussianPSF.cu:61-85 — params x,y,I,bg,bgx,bgy; constants sigma x,y */
+Estimator* Gauss2D_CreatePSF_XYITiltedBg(int roisize, bool cuda, Context* ctx);
+/* Estimators/Gaussian/GaussianPSF.cu:163-185 — params x,y,I,bg,sx,sy */
+Estimator* Gauss2D_CreatePSF_XYIBgSigmaXY(int roisize, float initialSigmaX, float initialSigmaY, bool cuda,
+                                          Context* ctx);
+/* Estimators/Gaussian/GaussianPSF.cu:187-212 — params x,y,I,bg,bgx,bgy,sx,sy */
+Estimator* Gauss2D_CreatePSF_XYITiltedBgSigmaXY(int roisize, float initialSigmaX, float initialSigmaY, bool cuda,
+                                                Context* ctx);
+/* Estimators/Gaussian/GaussianPSF.cu:216-240 — per spot: I and bg of images[n,imgw,imgw]
+ * at xy[n,2] with width sigma[i] (one float per spot i), IBg[n,2], IBg_crlb[n,2];
+ * maxIterations LM steps from {1, 0}.  roipos unused; computed on the GPU whatever cuda says. */
+void Gauss2D_EstimateIntensityBg(const float* imageData, float* IBg, float* IBg_crlb, int numspots, const float* xy,
+                                 const int* roipos, const float* sigma, int imgw, int maxIterations, bool cuda);
 /* Estimators/Gaussian/GaussianPSF.cu:36-59 — params x,y,z,I,bg */
 Estimator* Gauss2D_CreatePSF_XYZIBg(int roisize, const Gauss3D_Calibration* calib, bool cuda, Context* ctx);
 /* Estimators/CSpline/CubicSplinePSF.cu:55-107 — params x,y,z,I,bg; mode 0 (flat background) only */
@@ -134,8 +149,9 @@ void Estimator_ChiSquareAndCRLB(Estimator* e, const float* params, const float* 
  * numStreams streams (< 0: 3); GetResults copies whole finished batches, newest first,
  * while they fit in maxresults and returns the number of spots copied (estim [n,K],
  * crlb [n,K], roipos [n,2], samples [n,H,W] when keepSamples, result [n]; each may be
- * NULL; diag is unused, numDiag = 0).  constants must be NULL or empty (no model has
- * any).  A queue created in a Context is deleted by Context_Destroy. */
+ * NULL; diag is unused, numDiag = 0).  h_constants [n, NumConstants] is required by the
+ * models that have per-spot constants.  A queue created in a Context is deleted by
+ * Context_Destroy. */
 EstimationQueue* EstimQueue_Create(Estimator* psf, int batchSize, int maxQueueLen, bool keepSamples, int numStreams,
                                    Context* ctx);
 void EstimQueue_Delete(EstimationQueue* q);
@@ -188,13 +204,17 @@ int Estimator_GetPhaseTimes(Estimator* e, float* fitMs, float* crlbMs);
 /* Fused fit on device memory: Levenberg-Marquardt from the centre-of-mass start
  * (or d_initial), then CRLB and chi^2 when d_crlb / d_chisq are non-NULL.
  * d_sample [n,H,W], d_params [n,K], d_iterations [n], d_trace [n,traceLen,K]. */
-int Estimator_EstimateDevice(Estimator* e, int n, const float* d_sample, const float* d_initial, float* d_params,
-                             int* d_iterations, float* d_trace, int traceLen, float* d_crlb, float* d_chisq,
-                             void* stream);
-int Estimator_ExpectedValueDevice(Estimator* e, int n, const float* d_params, float* d_ev, void* stream);
-int Estimator_DerivativesDevice(Estimator* e, int n, const float* d_params, float* d_deriv, float* d_ev, void* stream);
-int Estimator_ChiSquareAndCRLBDevice(Estimator* e, int n, const float* d_params, const float* d_sample, float* d_crlb,
-                                     float* d_chisq, void* stream);
+int Estimator_EstimateDevice(Estimator* e, int n, const float* d_sample, const float* d_constants,
+                             const float* d_initial, float* d_params, int* d_iterations, float* d_trace, int traceLen,
+                             float* d_crlb, float* d_chisq, void* stream);
+/* d_constants [n, NumConstants]: the per-spot constants of the models that have them
+ * (XYIBgConstSigma, XYITiltedBg: sigma x, y), else NULL. */
+int Estimator_ExpectedValueDevice(Estimator* e, int n, const float* d_params, const float* d_constants, float* d_ev,
+                                  void* stream);
+int Estimator_DerivativesDevice(Estimator* e, int n, const float* d_params, const float* d_constants, float* d_deriv,
+                                float* d_ev, void* stream);
+int Estimator_ChiSquareAndCRLBDevice(Estimator* e, int n, const float* d_params, const float* d_constants,
+                                     const float* d_sample, float* d_crlb, float* d_chisq, void* stream);
 
 /* Poisson negative log-likelihood per spot, sum_px mu' - x' log mu' with mu' = max(mu,1e-8),
  * x' = max(x,1e-8) (the negated ComputeLogLikelihood of Estimation.h:138-156), and its adjoint.

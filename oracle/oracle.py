@@ -14,6 +14,14 @@ mathematics in FP64.  Model descriptors:
 * ``astig(R, x4, y4, zrange, B)``     Gauss2D_Model_XYZIBg     (GaussianPSFModels.h:636-751),
   with an optional quartic term ``B`` (per axis) inside the square root
 * ``spline(R, coefs, zmin, zmax)``    CSpline_Model_XYZIBg     (CubicSplinePSFModels.h:14-154)
+* ``gauss2d_const_sigma(R)``          Gauss2D_Model_XYIBgConstSigma (GaussianPSFModels.h:155-229),
+  constants [sigma_x, sigma_y] per spot
+* ``gauss2d_tilted_bg(R)``            Gauss2D_Model_XYITiltedBg (GaussianPSFModels.h:232-304),
+  constants [sigma_x, sigma_y] per spot
+* ``gauss2d_sigma_xy(R, isx, isy)``   Gauss2D_Model_XYIBgSigmaXY (GaussianPSFModels.h:388-467)
+* ``gauss2d_tilted_bg_sigma_xy(R, isx, isy)``  Gauss2D_Model_XYITiltedBgSigmaXY (:471-553)
+* ``intensity_bg(R)``                 IntensityBgModel (GaussianPSFModels.h:557-623), constants
+  [x, y, sigma] per spot; CRLB from |diag| (Estimation.h:318-333)
 """
 from __future__ import annotations
 
@@ -53,12 +61,15 @@ def lib():
         F, D, I, P = ct.POINTER(ct.c_float), ct.POINTER(ct.c_double), ct.POINTER(ct.c_int32), ct.c_void_p
         M = ct.POINTER(OrModel)
         for sfx, R in (("f32", F), ("f64", D)):
-            getattr(_lib, f"or_expected_value_{sfx}").argtypes = [M, ct.c_int, R, R, ct.c_int]
-            getattr(_lib, f"or_derivatives_{sfx}").argtypes = [M, ct.c_int, R, R, R, ct.c_int]
+            getattr(_lib, f"or_expected_value_{sfx}").argtypes = [M, ct.c_int, R, R, R, ct.c_int]
+            getattr(_lib, f"or_derivatives_{sfx}").argtypes = [M, ct.c_int, R, R, R, R, ct.c_int]
             scal = ct.c_float if sfx == "f32" else ct.c_double
-            getattr(_lib, f"or_estimate_{sfx}").argtypes = [M, ct.c_int, R, R, R, I, R, ct.c_int, ct.c_int, scal, ct.c_int]
-            getattr(_lib, f"or_crlb_{sfx}").argtypes = [M, ct.c_int, R, R, R, R, ct.c_int]
-            getattr(_lib, f"or_loglik_{sfx}").argtypes = [M, ct.c_int, R, R, R, ct.c_int]
+            getattr(_lib, f"or_estimate_{sfx}").argtypes = [M, ct.c_int, R, R, R, R, I, R, ct.c_int, ct.c_int, scal,
+                                                            ct.c_int]
+            getattr(_lib, f"or_crlb_{sfx}").argtypes = [M, ct.c_int, R, R, R, R, R, ct.c_int]
+            getattr(_lib, f"or_loglik_{sfx}").argtypes = [M, ct.c_int, R, R, R, R, ct.c_int]
+        _lib.or_num_params.argtypes = [ct.c_int]
+        _lib.or_num_consts.argtypes = [ct.c_int]
         _lib.or_spline_compute.argtypes = [I, F, F, ct.c_int]
     return _lib
 
@@ -86,7 +97,8 @@ class Model:
             self.desc.coefs = self._coefs.ctypes.data
         self.kind = kind
         self.roisize = roisize
-        self.K = 4 if kind == 0 else 5
+        self.K = lib().or_num_params(kind)
+        self.NC = lib().or_num_consts(kind)
 
     # -- helpers -----------------------------------------------------------
     def _arr(self, a, dt, shape=None):
@@ -100,58 +112,73 @@ class Model:
             return None
         return a.ctypes.data_as(ct.POINTER(ct.c_float if dt == np.float32 else ct.c_double))
 
+    def _cst(self, constants, n, dt):
+        if self.NC == 0:
+            return None
+        if constants is None:
+            raise ValueError(f"model kind {self.kind} needs constants of shape ({n}, {self.NC})")
+        return self._arr(constants, dt, (n, self.NC))
+
     # -- API ---------------------------------------------------------------
-    def expected_value(self, params, dtype=np.float32, nthreads=None):
+    def expected_value(self, params, dtype=np.float32, nthreads=None, constants=None):
         p = self._arr(params, dtype, (-1, self.K))
         n, R = len(p), self.roisize
+        c = self._cst(constants, n, dtype)
         ev = np.zeros((n, R, R), dtype)
         f = lib().or_expected_value_f32 if dtype == np.float32 else lib().or_expected_value_f64
-        f(ct.byref(self.desc), n, self._ptr(p, dtype), self._ptr(ev, dtype), _nthreads(nthreads))
+        f(ct.byref(self.desc), n, self._ptr(p, dtype), self._ptr(c, dtype), self._ptr(ev, dtype), _nthreads(nthreads))
         return ev
 
-    def derivatives(self, params, dtype=np.float32, nthreads=None):
+    def derivatives(self, params, dtype=np.float32, nthreads=None, constants=None):
         p = self._arr(params, dtype, (-1, self.K))
         n, R = len(p), self.roisize
+        c = self._cst(constants, n, dtype)
         ev = np.zeros((n, R, R), dtype)
         dv = np.zeros((n, self.K, R, R), dtype)
         f = lib().or_derivatives_f32 if dtype == np.float32 else lib().or_derivatives_f64
-        f(ct.byref(self.desc), n, self._ptr(p, dtype), self._ptr(dv, dtype), self._ptr(ev, dtype), _nthreads(nthreads))
+        f(ct.byref(self.desc), n, self._ptr(p, dtype), self._ptr(c, dtype), self._ptr(dv, dtype), self._ptr(ev, dtype),
+          _nthreads(nthreads))
         return dv, ev
 
-    def estimate(self, samples, initial=None, iterations=50, lambda_=1e-18, trace_len=0, dtype=np.float32, nthreads=None):
+    def estimate(self, samples, initial=None, iterations=50, lambda_=1e-18, trace_len=0, dtype=np.float32, nthreads=None,
+                 constants=None):
         R = self.roisize
         s = self._arr(samples, dtype, (-1, R, R))
         n = len(s)
+        c = self._cst(constants, n, dtype)
         ini = None if initial is None else self._arr(initial, dtype, (n, self.K))
         params = np.zeros((n, self.K), dtype)
         iters = np.zeros(n, np.int32)
         trace = np.zeros((n, max(trace_len, 1), self.K), dtype) if trace_len > 0 else None
         f = lib().or_estimate_f32 if dtype == np.float32 else lib().or_estimate_f64
-        f(ct.byref(self.desc), n, self._ptr(s, dtype), self._ptr(ini, dtype), self._ptr(params, dtype),
+        f(ct.byref(self.desc), n, self._ptr(s, dtype), self._ptr(c, dtype), self._ptr(ini, dtype), self._ptr(params, dtype),
           iters.ctypes.data_as(ct.POINTER(ct.c_int32)), self._ptr(trace, dtype), trace_len, iterations,
           lambda_, _nthreads(nthreads))
         if trace_len > 0:
             return params, iters, trace
         return params, iters
 
-    def crlb(self, params, samples=None, dtype=np.float32, nthreads=None):
+    def crlb(self, params, samples=None, dtype=np.float32, nthreads=None, constants=None):
         p = self._arr(params, dtype, (-1, self.K))
         n, R = len(p), self.roisize
+        c = self._cst(constants, n, dtype)
         s = None if samples is None else self._arr(samples, dtype, (n, R, R))
         out = np.zeros((n, self.K), dtype)
         chi = np.zeros(n, dtype)
         f = lib().or_crlb_f32 if dtype == np.float32 else lib().or_crlb_f64
-        f(ct.byref(self.desc), n, self._ptr(p, dtype), self._ptr(s, dtype), self._ptr(out, dtype),
+        f(ct.byref(self.desc), n, self._ptr(p, dtype), self._ptr(c, dtype), self._ptr(s, dtype), self._ptr(out, dtype),
           self._ptr(chi, dtype), _nthreads(nthreads))
         return out, chi
 
-    def loglik(self, params, samples, dtype=np.float32, nthreads=None):
+    def loglik(self, params, samples, dtype=np.float32, nthreads=None, constants=None):
         p = self._arr(params, dtype, (-1, self.K))
         n, R = len(p), self.roisize
+        c = self._cst(constants, n, dtype)
         s = self._arr(samples, dtype, (n, R, R))
         ll = np.zeros(n, dtype)
         f = lib().or_loglik_f32 if dtype == np.float32 else lib().or_loglik_f64
-        f(ct.byref(self.desc), n, self._ptr(p, dtype), self._ptr(s, dtype), self._ptr(ll, dtype), _nthreads(nthreads))
+        f(ct.byref(self.desc), n, self._ptr(p, dtype), self._ptr(c, dtype), self._ptr(s, dtype), self._ptr(ll, dtype),
+          _nthreads(nthreads))
         return ll
 
 
@@ -172,6 +199,26 @@ def spline(roisize, coefs, zmin, zmax):
     coefs = np.ascontiguousarray(coefs, dtype=np.float32)
     assert coefs.ndim == 4 and coefs.shape[3] == 64
     return Model(3, roisize, (zmin, zmax), shape=coefs.shape[:3], coefs=coefs)
+
+
+def gauss2d_const_sigma(roisize):
+    return Model(4, roisize)
+
+
+def gauss2d_tilted_bg(roisize):
+    return Model(5, roisize)
+
+
+def gauss2d_sigma_xy(roisize, initial_sigma_x, initial_sigma_y):
+    return Model(6, roisize, (initial_sigma_x, initial_sigma_y))
+
+
+def gauss2d_tilted_bg_sigma_xy(roisize, initial_sigma_x, initial_sigma_y):
+    return Model(7, roisize, (initial_sigma_x, initial_sigma_y))
+
+
+def intensity_bg(roisize):
+    return Model(8, roisize)
 
 
 def spline_compute(data, border=0):

@@ -25,6 +25,12 @@
 //   Gauss2D_Model_XYIBgSigma   cpp/SMLMLib/Estimators/Gaussian/GaussianPSFModels.h:308-387
 //   Gauss2D_Model_XYZIBg       cpp/SMLMLib/Estimators/Gaussian/GaussianPSFModels.h:636-751
 //     (+ quartic term B*t^4/d^2 inside the square root; B=0 is the reference)
+//   Gauss2D_Model_XYIBgConstSigma   GaussianPSFModels.h:155-229
+//   Gauss2D_Model_XYITiltedBg       GaussianPSFModels.h:232-304
+//   Gauss2D_Model_XYIBgSigmaXY      GaussianPSFModels.h:388-467
+//   Gauss2D_Model_XYITiltedBgSigmaXY GaussianPSFModels.h:471-553 (mu without the tilt, as there)
+//   IntensityBgModel + Gauss2D_EstimateIntensityBg  GaussianPSFModels.h:557-623, GaussianPSF.cu:216-240
+//   ComputeFisherMatrix / ComputeCRLB (|diag|)      Estimation.h:22-60, 318-333
 //   CSpline_Model_XYZIBg       cpp/SMLMLib/Estimators/CSpline/CubicSplinePSFModels.h:14-154
 //   CSpline_Compute            cpp/SMLMLib/Estimators/CSpline/CubicSplinePSF.cu:151-204
 //
@@ -76,17 +82,23 @@ template <> constexpr float kPiT<float> = 3.141592653589793f;
 //       1 = 2D Gaussian free sigma (x,y,I,bg,sigma)
 //       2 = astigmatic 3D Gaussian (x,y,z,I,bg)
 //       3 = cubic spline 3D (x,y,z,I,bg)
+//       4 = 2D Gaussian, sigma x,y per-spot constants (x,y,I,bg)
+//       5 = tilted background, sigma x,y per-spot constants (x,y,I,bg,bgx,bgy)
+//       6 = free sigma x and y (x,y,I,bg,sx,sy)
+//       7 = tilted background + free sigma x and y (x,y,I,bg,bgx,bgy,sx,sy)
+//       8 = intensity / background at per-spot constants x, y, sigma (I,bg)
 struct OrModel {
     int kind;
     int roisize;
-    double p[16]; // 0: sx,sy | 1: initialSigma | 2: x[4],y[4],minz,maxz,Bx,By | 3: zmin,zmax
+    double p[16]; // 0: sx,sy | 1: initialSigma | 2: x[4],y[4],minz,maxz,Bx,By | 3: zmin,zmax | 6,7: isx,isy
     int shape[3]; // spline (Nz,Ny,Nx)
     const float* coefs;
 };
 
 template <typename T>
 struct Gauss2DFixed {
-    enum { K = 4 };
+    enum { K = 4, NC = 0, CrlbAbs = 0 };
+    const T* cst = nullptr;
     int R;
     T sx, sy;
     static T StopLimit(int k) { const T s[4] = {T(1e-5f), T(1e-5f), T(1e-1f), T(1e-5f)}; return s[k]; }
@@ -123,7 +135,8 @@ struct Gauss2DFixed {
 
 template <typename T>
 struct Gauss2DSigma {
-    enum { K = 5 };
+    enum { K = 5, NC = 0, CrlbAbs = 0 };
+    const T* cst = nullptr;
     int R;
     T initialSigma;
     static T StopLimit(int k) { const T s[5] = {T(1e-5f), T(1e-5f), T(1e-1f), T(1e-4f), T(1e-5f)}; return s[k]; }
@@ -165,7 +178,8 @@ struct Gauss2DSigma {
 
 template <typename T>
 struct Astig3D {
-    enum { K = 5 };
+    enum { K = 5, NC = 0, CrlbAbs = 0 };
+    const T* cst = nullptr;
     int R;
     T cx[4], cy[4], minz, maxz, Bx, By;
     static T StopLimit(int k) { const T s[5] = {T(1e-4f), T(1e-4f), T(1e-1f), T(1e-4f), T(1e-4f)}; return s[k]; }
@@ -225,7 +239,8 @@ struct Astig3D {
 
 template <typename T>
 struct Spline3D {
-    enum { K = 5 };
+    enum { K = 5, NC = 0, CrlbAbs = 0 };
+    const T* cst = nullptr;
     int R;
     int Nz, Ny, Nx;
     T zmin, zmax;
@@ -295,6 +310,164 @@ struct Spline3D {
 };
 
 // ---------------------------------------------------- small dense solvers
+// One axis of the pixel-integrated Gaussian (GaussianPSFModels.h:76-94 pattern).
+template <typename T>
+struct GAxis {
+    T D, dE, dS;  // DeltaX, dEx (without I), (e1 a1 - e0 a0)/sqrt(pi) (the sigma term without I)
+};
+
+template <typename T>
+GAxis<T> gaxis(int px, T t, T sigma) {
+    const T cp = T(1.0f) / (std::sqrt(T(2) * kPiT<T>) * sigma);
+    const T c2 = T(1.0f) / (std::sqrt(T(2)) * sigma);
+    const T isp = T(1.0f) / std::sqrt(kPiT<T>);
+    T a0 = (T(px) - t + T(.5f)) * c2;
+    T a1 = (T(px) - t - T(.5f)) * c2;
+    GAxis<T> g;
+    g.D = T(0.5f) * std::erf(a0) - T(0.5f) * std::erf(a1);
+    T e1 = std::exp(-a1 * a1), e0 = std::exp(-a0 * a0);
+    g.dE = cp * (e1 - e0);
+    g.dS = (e1 * a1 - e0 * a0) * isp;
+    return g;
+}
+
+template <typename T>
+void xy_clamp(T* t, int R) {
+    t[0] = clampf_<T>(t[0], T(1.5f), T(R) - T(1.5f) - T(1));
+    t[1] = clampf_<T>(t[1], T(1.5f), T(R) - T(1.5f) - T(1));
+}
+
+// XYIBgConstSigma (GaussianPSFModels.h:155-229): sigma x, y per spot in cst.
+template <typename T>
+struct Gauss2DConstSigma {
+    enum { K = 4, NC = 2, CrlbAbs = 0 };
+    const T* cst = nullptr;
+    int R;
+    static T StopLimit(int k) { const T s[4] = {T(1e-5f), T(1e-5f), T(1e-1f), T(1e-5f)}; return s[k]; }
+    void CheckLimits(T* t) const {
+        xy_clamp(t, R);
+        if (t[2] < T(25.0f)) t[2] = T(25.0f);
+        if (t[3] < T(0.0f)) t[3] = T(0.0f);
+    }
+    void Initial(const T com[3], T* t) const { t[0] = com[0]; t[1] = com[1]; t[2] = com[2] * T(0.7f); t[3] = 0; }
+    template <typename CB> void Eval(const T* th, CB cb) const {
+        const T tx = th[0], ty = th[1], tI = th[2], tbg = th[3];
+        for (int y = 0; y < R; y++) {
+            GAxis<T> gy = gaxis<T>(y, ty, cst[1]);
+            for (int x = 0; x < R; x++) {
+                GAxis<T> gx = gaxis<T>(x, tx, cst[0]);
+                T mu = tbg + tI * gx.D * gy.D;
+                const T J[4] = {tI * gx.dE * gy.D, tI * gy.dE * gx.D, gx.D * gy.D, T(1)};
+                cb(y * R + x, mu, J);
+            }
+        }
+    }
+};
+
+// XYITiltedBg (GaussianPSFModels.h:232-304): mu = bg + bgx (x - R/2) + bgy (y - R/2) + I G.
+template <typename T>
+struct Gauss2DTiltedBg {
+    enum { K = 6, NC = 2, CrlbAbs = 0 };
+    const T* cst = nullptr;
+    int R;
+    static T StopLimit(int k) { return k == 2 ? T(1e-1f) : T(1e-5f); }
+    void CheckLimits(T* t) const {
+        xy_clamp(t, R);
+        if (t[2] < T(25.0f)) t[2] = T(25.0f);
+        if (t[3] < T(0.0f)) t[3] = T(0.0f);
+    }
+    void Initial(const T com[3], T* t) const {
+        t[0] = com[0]; t[1] = com[1]; t[2] = com[2] * T(0.7f); t[3] = 0; t[4] = 0; t[5] = 0;
+    }
+    template <typename CB> void Eval(const T* th, CB cb) const {
+        const T tx = th[0], ty = th[1], tI = th[2], tbg = th[3], bgx = th[4], bgy = th[5];
+        const float half = R * 0.5f;
+        for (int y = 0; y < R; y++) {
+            GAxis<T> gy = gaxis<T>(y, ty, cst[1]);
+            for (int x = 0; x < R; x++) {
+                GAxis<T> gx = gaxis<T>(x, tx, cst[0]);
+                T mu = tbg + bgx * (T(x) - T(half)) + bgy * (T(y) - T(half)) + tI * gx.D * gy.D;
+                const T J[6] = {tI * gx.dE * gy.D, tI * gy.dE * gx.D, gx.D * gy.D, T(1), T(x) - T(half),
+                                T(y) - T(half)};
+                cb(y * R + x, mu, J);
+            }
+        }
+    }
+};
+
+// XYIBgSigmaXY (GaussianPSFModels.h:388-467) and, with TILT, XYITiltedBgSigmaXY
+// (:471-553), whose mu leaves the tilt out while its Jacobian has the tilt columns.
+template <typename T, bool TILT>
+struct Gauss2DSigmaXY {
+    enum { K = TILT ? 8 : 6, NC = 0, CrlbAbs = 0 };
+    const T* cst = nullptr;
+    int R;
+    T isx, isy;
+    static constexpr int iS = TILT ? 6 : 4;
+    static T StopLimit(int k) {
+        if (k == 2) return T(1e-1f);
+        if (k == 3 || (TILT && (k == 4 || k == 5))) return T(1e-4f);
+        return T(1e-5f);
+    }
+    void CheckLimits(T* t) const {
+        xy_clamp(t, R);
+        if (t[2] < T(10.0f)) t[2] = T(10.0f);
+        if (t[3] < T(0.0f)) t[3] = T(0.0f);
+        t[iS] = clampf_<T>(t[iS], T(1.0f), T(R) * T(0.5f));
+        t[iS + 1] = clampf_<T>(t[iS + 1], T(1.0f), T(R) * T(0.5f));
+    }
+    void Initial(const T com[3], T* t) const {
+        t[0] = com[0]; t[1] = com[1]; t[2] = com[2]; t[3] = 0;
+        if (TILT) { t[4] = 0; t[5] = 0; }
+        t[iS] = isx; t[iS + 1] = isy;
+    }
+    template <typename CB> void Eval(const T* th, CB cb) const {
+        const T tx = th[0], ty = th[1], tI = th[2], tbg = th[3];
+        const float half = R * 0.5f;
+        for (int y = 0; y < R; y++) {
+            GAxis<T> gy = gaxis<T>(y, ty, th[iS + 1]);
+            for (int x = 0; x < R; x++) {
+                GAxis<T> gx = gaxis<T>(x, tx, th[iS]);
+                T mu = tbg + tI * gx.D * gy.D;
+                T J[8];
+                J[0] = tI * gx.dE * gy.D;
+                J[1] = tI * gy.dE * gx.D;
+                J[2] = gx.D * gy.D;
+                J[3] = T(1);
+                if (TILT) { J[4] = T(x) - T(half); J[5] = T(y) - T(half); }
+                J[iS] = tI * gx.dS * gy.D;
+                J[iS + 1] = tI * gy.dS * gx.D;
+                cb(y * R + x, mu, J);
+            }
+        }
+    }
+};
+
+// IntensityBgModel (GaussianPSFModels.h:557-623): per-spot constants x, y, sigma.
+template <typename T>
+struct IntensityBg {
+    enum { K = 2, NC = 3, CrlbAbs = 1 };
+    const T* cst = nullptr;
+    int R;
+    static T StopLimit(int k) { return k == 0 ? T(1e-2f) : T(1e-4f); }
+    void CheckLimits(T* t) const {
+        if (t[0] < T(1e-5f)) t[0] = T(1e-5f);
+        if (t[1] < T(1e-8f)) t[1] = T(1e-8f);
+    }
+    void Initial(const T*, T* t) const { t[0] = 1; t[1] = 0; }
+    template <typename CB> void Eval(const T* th, CB cb) const {
+        for (int y = 0; y < R; y++) {
+            GAxis<T> gy = gaxis<T>(y, cst[1], cst[2]);
+            for (int x = 0; x < R; x++) {
+                GAxis<T> gx = gaxis<T>(x, cst[0], cst[2]);
+                const T psf = gx.D * gy.D;
+                const T J[2] = {psf, T(1)};
+                cb(y * R + x, th[0] * psf + th[1], J);
+            }
+        }
+    }
+};
+
 template <typename T>
 bool cholesky(int d, const T* S, T* D) {
     for (int k = 0; k < d; ++k) {
@@ -465,7 +638,7 @@ void crlb_chisq(const M& m, const T* theta, const T* smp, T* crlb, T* chisq) {
     std::memcpy(A, fi, sizeof(A));
     if (lup_decompose<T>(K, A, T(1e-9f), P)) {
         lup_invert<T>(K, A, P, inv);
-        for (int k = 0; k < K; k++) crlb[k] = std::sqrt(inv[k * K + k]);
+        for (int k = 0; k < K; k++) crlb[k] = std::sqrt(M::CrlbAbs ? std::fabs(inv[k * K + k]) : inv[k * K + k]);
     } else {
         for (int k = 0; k < K; k++) crlb[k] = T(INFINITY);
     }
@@ -504,28 +677,51 @@ void dispatch(const OrModel* d, F fn) {
         fn(m);
         break;
     }
+    case 4: { Gauss2DConstSigma<T> m{}; m.R = d->roisize; fn(m); break; }
+    case 5: { Gauss2DTiltedBg<T> m{}; m.R = d->roisize; fn(m); break; }
+    case 6: { Gauss2DSigmaXY<T, false> m{}; m.R = d->roisize; m.isx = d->p[0]; m.isy = d->p[1]; fn(m); break; }
+    case 7: { Gauss2DSigmaXY<T, true> m{}; m.R = d->roisize; m.isx = d->p[0]; m.isy = d->p[1]; fn(m); break; }
+    case 8: { IntensityBg<T> m{}; m.R = d->roisize; fn(m); break; }
     default: break;
     }
 }
 
-int num_params(int kind) { return kind == 0 ? 4 : 5; }
+int num_params(int kind) {
+    switch (kind) {
+    case 0: case 4: return 4;
+    case 5: case 6: return 6;
+    case 7: return 8;
+    case 8: return 2;
+    default: return 5;
+    }
+}
+
+int num_consts(int kind) { return kind == 4 || kind == 5 ? 2 : (kind == 8 ? 3 : 0); }
+
+// The model for spot s: a copy that points at the spot's constants.
+template <typename M, typename T>
+M for_spot(const M& m, const T* cst, int s) {
+    M c = m;
+    if (M::NC > 0) c.cst = cst + (size_t)s * M::NC;
+    return c;
+}
 
 template <typename T>
-void expected_value(const OrModel* d, int n, const T* params, T* ev, int nth) {
+void expected_value(const OrModel* d, int n, const T* params, const T* cst, T* ev, int nth) {
     const int S = d->roisize * d->roisize, K = num_params(d->kind);
     dispatch<T>(d, [&](auto& m) {
         pool_for(n, nth, [&](int s) {
-            m.Eval(params + (size_t)s * K, [&](int idx, T mu, const T*) { ev[(size_t)s * S + idx] = mu; });
+            for_spot(m, cst, s).Eval(params + (size_t)s * K, [&](int idx, T mu, const T*) { ev[(size_t)s * S + idx] = mu; });
         });
     });
 }
 
 template <typename T>
-void derivatives(const OrModel* d, int n, const T* params, T* deriv, T* ev, int nth) {
+void derivatives(const OrModel* d, int n, const T* params, const T* cst, T* deriv, T* ev, int nth) {
     const int S = d->roisize * d->roisize, K = num_params(d->kind);
     dispatch<T>(d, [&](auto& m) {
         pool_for(n, nth, [&](int s) {
-            m.Eval(params + (size_t)s * K, [&](int idx, T mu, const T* J) {
+            for_spot(m, cst, s).Eval(params + (size_t)s * K, [&](int idx, T mu, const T* J) {
                 for (int k = 0; k < K; k++) deriv[((size_t)s * K + k) * S + idx] = J[k];
                 ev[(size_t)s * S + idx] = mu;
             });
@@ -534,11 +730,12 @@ void derivatives(const OrModel* d, int n, const T* params, T* deriv, T* ev, int 
 }
 
 template <typename T>
-void estimate(const OrModel* d, int n, const T* smp, const T* initial, T* params, int* iters, T* trace,
+void estimate(const OrModel* d, int n, const T* smp, const T* cst, const T* initial, T* params, int* iters, T* trace,
               int traceLen, int maxIter, T lambda, int nth) {
     const int R = d->roisize, S = R * R, K = num_params(d->kind);
-    dispatch<T>(d, [&](auto& m) {
+    dispatch<T>(d, [&](auto& m0) {
         pool_for(n, nth, [&](int s) {
+            const auto m = for_spot(m0, cst, s);
             const T* sp = smp + (size_t)s * S;
             T* th = params + (size_t)s * K;
             if (initial) {
@@ -556,22 +753,25 @@ void estimate(const OrModel* d, int n, const T* smp, const T* initial, T* params
 }
 
 template <typename T>
-void crlb(const OrModel* d, int n, const T* params, const T* smp, T* out, T* chisq, int nth) {
+void crlb(const OrModel* d, int n, const T* params, const T* cst, const T* smp, T* out, T* chisq, int nth) {
     const int S = d->roisize * d->roisize, K = num_params(d->kind);
     dispatch<T>(d, [&](auto& m) {
         pool_for(n, nth, [&](int s) {
             T c;
-            crlb_chisq<T>(m, params + (size_t)s * K, smp ? smp + (size_t)s * S : nullptr, out + (size_t)s * K, &c);
+            crlb_chisq<T>(for_spot(m, cst, s), params + (size_t)s * K, smp ? smp + (size_t)s * S : nullptr,
+                          out + (size_t)s * K, &c);
             if (chisq) chisq[s] = c;
         });
     });
 }
 
 template <typename T>
-void loglik_all(const OrModel* d, int n, const T* params, const T* smp, T* ll, int nth) {
+void loglik_all(const OrModel* d, int n, const T* params, const T* cst, const T* smp, T* ll, int nth) {
     const int S = d->roisize * d->roisize, K = num_params(d->kind);
     dispatch<T>(d, [&](auto& m) {
-        pool_for(n, nth, [&](int s) { ll[s] = loglik<T>(m, params + (size_t)s * K, smp + (size_t)s * S); });
+        pool_for(n, nth, [&](int s) {
+            ll[s] = loglik<T>(for_spot(m, cst, s), params + (size_t)s * K, smp + (size_t)s * S);
+        });
     });
 }
 
@@ -580,23 +780,25 @@ void loglik_all(const OrModel* d, int n, const T* params, const T* smp, T* ll, i
 extern "C" {
 
 int or_num_params(int kind) { return num_params(kind); }
+int or_num_consts(int kind) { return num_consts(kind); }
 
-void or_expected_value_f32(const OrModel* d, int n, const float* p, float* ev, int nth) { expected_value<float>(d, n, p, ev, nth); }
-void or_expected_value_f64(const OrModel* d, int n, const double* p, double* ev, int nth) { expected_value<double>(d, n, p, ev, nth); }
-void or_derivatives_f32(const OrModel* d, int n, const float* p, float* dv, float* ev, int nth) { derivatives<float>(d, n, p, dv, ev, nth); }
-void or_derivatives_f64(const OrModel* d, int n, const double* p, double* dv, double* ev, int nth) { derivatives<double>(d, n, p, dv, ev, nth); }
-void or_estimate_f32(const OrModel* d, int n, const float* smp, const float* initial, float* params, int* iters,
-                     float* trace, int traceLen, int maxIter, float lambda, int nth) {
-    estimate<float>(d, n, smp, initial, params, iters, trace, traceLen, maxIter, lambda, nth);
+// cst: [n, NC] per-spot constants of the models that have them, else NULL.
+void or_expected_value_f32(const OrModel* d, int n, const float* p, const float* cst, float* ev, int nth) { expected_value<float>(d, n, p, cst, ev, nth); }
+void or_expected_value_f64(const OrModel* d, int n, const double* p, const double* cst, double* ev, int nth) { expected_value<double>(d, n, p, cst, ev, nth); }
+void or_derivatives_f32(const OrModel* d, int n, const float* p, const float* cst, float* dv, float* ev, int nth) { derivatives<float>(d, n, p, cst, dv, ev, nth); }
+void or_derivatives_f64(const OrModel* d, int n, const double* p, const double* cst, double* dv, double* ev, int nth) { derivatives<double>(d, n, p, cst, dv, ev, nth); }
+void or_estimate_f32(const OrModel* d, int n, const float* smp, const float* cst, const float* initial, float* params,
+                     int* iters, float* trace, int traceLen, int maxIter, float lambda, int nth) {
+    estimate<float>(d, n, smp, cst, initial, params, iters, trace, traceLen, maxIter, lambda, nth);
 }
-void or_estimate_f64(const OrModel* d, int n, const double* smp, const double* initial, double* params, int* iters,
-                     double* trace, int traceLen, int maxIter, double lambda, int nth) {
-    estimate<double>(d, n, smp, initial, params, iters, trace, traceLen, maxIter, lambda, nth);
+void or_estimate_f64(const OrModel* d, int n, const double* smp, const double* cst, const double* initial,
+                     double* params, int* iters, double* trace, int traceLen, int maxIter, double lambda, int nth) {
+    estimate<double>(d, n, smp, cst, initial, params, iters, trace, traceLen, maxIter, lambda, nth);
 }
-void or_crlb_f32(const OrModel* d, int n, const float* p, const float* smp, float* out, float* chisq, int nth) { crlb<float>(d, n, p, smp, out, chisq, nth); }
-void or_crlb_f64(const OrModel* d, int n, const double* p, const double* smp, double* out, double* chisq, int nth) { crlb<double>(d, n, p, smp, out, chisq, nth); }
-void or_loglik_f32(const OrModel* d, int n, const float* p, const float* smp, float* ll, int nth) { loglik_all<float>(d, n, p, smp, ll, nth); }
-void or_loglik_f64(const OrModel* d, int n, const double* p, const double* smp, double* ll, int nth) { loglik_all<double>(d, n, p, smp, ll, nth); }
+void or_crlb_f32(const OrModel* d, int n, const float* p, const float* cst, const float* smp, float* out, float* chisq, int nth) { crlb<float>(d, n, p, cst, smp, out, chisq, nth); }
+void or_crlb_f64(const OrModel* d, int n, const double* p, const double* cst, const double* smp, double* out, double* chisq, int nth) { crlb<double>(d, n, p, cst, smp, out, chisq, nth); }
+void or_loglik_f32(const OrModel* d, int n, const float* p, const float* cst, const float* smp, float* ll, int nth) { loglik_all<float>(d, n, p, cst, smp, ll, nth); }
+void or_loglik_f64(const OrModel* d, int n, const double* p, const double* cst, const double* smp, double* ll, int nth) { loglik_all<double>(d, n, p, cst, smp, ll, nth); }
 
 // Spline coefficients for a z-stack [Nz,Ny,Nx]; output [(Nz-3+2b),(Ny-3+2b),(Nx-3+2b),64].
 // Follows CSpline_Compute (CubicSplinePSF.cu:151-204): one 64x64 system whose row

@@ -36,6 +36,8 @@ GRID_GROUPS = {3: [4], 4: [4], 5: [4], 6: [4], 7: [4, 8], 8: [8], 9: [8, 16], 10
 
 def key(est):
     kind = _KIND.get(est.fmt)
+    if est.numconst > 0 or est.fmt not in _KIND and est.numparams != 5:
+        return f"{est.fmt}/{est.numconst}:{int(est.sampleshape[0])}"
     if kind is None:
         kind = "astig"
         if est.num_shared() > 10:
@@ -45,7 +47,10 @@ def key(est):
 
 def compiled_groups(est):
     R = int(est.sampleshape[0])
-    return (GRID_GROUPS if key(est).startswith("splinegrid") else COMPILED_GROUPS).get(R, [])
+    k = key(est)
+    if "/" in k:  # the other Gaussian variants: default group only
+        return []
+    return (GRID_GROUPS if k.startswith("splinegrid") else COMPILED_GROUPS).get(R, [])
 
 
 def load():

@@ -62,7 +62,7 @@ static void launch_bins(MixedBatch* mb, const float* const* d_samples, float* co
         for (int bin : mb->order[b]) {
             Estimator* e = mb->est[bin];
             std::lock_guard<std::mutex> l(e->mtx);
-            run_fit(e, mb->counts[bin], d_samples[bin], nullptr, d_params[bin], d_iters ? d_iters[bin] : nullptr,
+            run_fit(e, mb->counts[bin], d_samples[bin], nullptr, nullptr, d_params[bin], d_iters ? d_iters[bin] : nullptr,
                     nullptr, 0, d_crlb ? d_crlb[bin] : nullptr, d_chisq ? d_chisq[bin] : nullptr, s, false);
         }
         hip_check(hipEventRecord(mb->joins[b], s), "hipEventRecord(join)");
@@ -84,6 +84,8 @@ MixedBatch* MixedBatch_Create(int nbins, Estimator* const* ests, const int* coun
                 for (int b = 0; b < nbins; b++) {
                     Estimator* e = ests[b];
                     if (!e) throw std::runtime_error("photonpy_amd: NULL estimator in mixed batch");
+                    if (e->NC() > 0)
+                        throw std::runtime_error("photonpy_amd: models with per-spot constants are not batched");
                     if (counts[b] < 0) throw std::runtime_error("photonpy_amd: negative spot count");
                     const int R = e->roisize;
                     if (R < kMinFitR || R > kMaxFitR)
@@ -189,7 +191,7 @@ int MixedBatch_RunPlain(MixedBatch* mb, const float* const* d_samples, float* co
             for (size_t b = 0; b < mb->est.size(); b++) {
                 Estimator* e = mb->est[b];
                 std::lock_guard<std::mutex> l(e->mtx);
-                run_fit(e, mb->counts[b], d_samples[b], nullptr, d_params[b], d_iters ? d_iters[b] : nullptr, nullptr,
+                run_fit(e, mb->counts[b], d_samples[b], nullptr, nullptr, d_params[b], d_iters ? d_iters[b] : nullptr, nullptr,
                         0, d_crlb ? d_crlb[b] : nullptr, d_chisq ? d_chisq[b] : nullptr, as_stream(stream), false);
             }
             return 0;

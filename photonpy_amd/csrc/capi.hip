@@ -132,6 +132,20 @@ static Estimator* make_gauss(int kind, int roisize, const GaussCalib& gc) {
     } else if (kind == kKindSigma) {
         e->fmt = "x,y,\tI,bg,sigma";  // literal tab as in GaussianPSFModels.h:316
         e->limits = {{b, hi}, {b, hi}, {10.0f, 1e9f}, {1e-6f, 1e9f}, {1.0f, roisize * 2.0f}};  // GaussianPSF.cu:141-147
+    } else if (kind == kKindConstSigma) {
+        e->fmt = "x,y,I,bg";
+        e->limits = {{b, hi}, {b, hi}, {10.0f, 1e9f}, {1e-6f, 1e9f}};  // GaussianPSF.cu:113-118
+    } else if (kind == kKindTiltedBg) {
+        e->fmt = "x,y,I,bg,bgx,bgy";
+        e->limits = {{b, hi}, {b, hi}, {10.0f, 1e9f}, {1e-6f, 1e9f}, {-1e9f, 1e9f}, {-1e9f, 1e9f}};  // :66-73
+    } else if (kind == kKindSigmaXY) {
+        e->fmt = "x,y,I,bg,sx,sy";
+        e->limits = {{b, hi}, {b, hi}, {10.0f, 1e9f}, {1e-6f, 1e9f}, {1.0f, roisize * 2.0f},
+                     {1.0f, roisize * 2.0f}};  // :169-176
+    } else if (kind == kKindTiltedBgSigmaXY) {
+        e->fmt = "x,y,I,bg,bgx,bgy,sx,sy";
+        e->limits = {{b, hi}, {b, hi}, {10.0f, 1e9f}, {1e-6f, 1e9f}, {-1e9f, 1e9f}, {-1e9f, 1e9f},
+                     {1.0f, roisize * 2.0f}, {1.0f, roisize * 2.0f}};  // :193-202
     } else {
         e->fmt = "x,y,z,I,bg";
         e->limits = {{b, hi}, {b, hi}, {gc.minz, gc.maxz}, {10.0f, 1e9f}, {1e-6f, 1e9f}};  // GaussianPSF.cu:42-48
@@ -140,8 +154,9 @@ static Estimator* make_gauss(int kind, int roisize, const GaussCalib& gc) {
 }
 
 // ---------------------------------------------------------------- launch helpers
-void run_fit(Estimator* e, int n, const float* d_sample, const float* d_initial, float* d_params, int* d_iter,
-             float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st, bool timing) {
+void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constants, const float* d_initial,
+             float* d_params, int* d_iter, float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st,
+             bool timing) {
     e->ensure_device();
     const int R = e->roisize;
     if (R < kMinFitR || R > kMaxFitR)
@@ -161,6 +176,10 @@ void run_fit(Estimator* e, int n, const float* d_sample, const float* d_initial,
     a.lambda = e->lm.lambda;
     a.refill = e->refill;
     a.counter = e->next_counter();
+    a.constants = d_constants;
+    if (e->NC() > 0 && !d_constants)
+        throw std::runtime_error("photonpy_amd: this estimator needs " + std::to_string(e->NC()) +
+                                 " constants per spot");
     hipEvent_t* events = nullptr;
     if (e->phaseTiming && timing) {
         for (int i = 0; i < 3; i++) {
@@ -252,6 +271,97 @@ Estimator* Gauss2D_CreatePSF_XYIBgSigma(int roisize, float initialSigma, bool cu
             return e;
         },
         nullptr);
+}
+
+// GaussianPSF.cu:108-129 -- x,y,I,bg with sigma x, y as per-spot constants
+Estimator* Gauss2D_CreatePSF_XYIBgConstSigma(int roisize, bool cuda, Context* ctx) {
+    return guarded(
+        [&]() -> Estimator* {
+            require_gpu(cuda);
+            Estimator* e = make_gauss(kKindConstSigma, roisize, GaussCalib{});
+            attach(e, ctx);
+            return e;
+        },
+        nullptr);
+}
+
+// GaussianPSF.cu:61-85 -- x,y,I,bg,bgx,bgy with sigma x, y as per-spot constants
+Estimator* Gauss2D_CreatePSF_XYITiltedBg(int roisize, bool cuda, Context* ctx) {
+    return guarded(
+        [&]() -> Estimator* {
+            require_gpu(cuda);
+            Estimator* e = make_gauss(kKindTiltedBg, roisize, GaussCalib{});
+            attach(e, ctx);
+            return e;
+        },
+        nullptr);
+}
+
+static Estimator* create_sigmaxy(int kind, int roisize, float isx, float isy, bool cuda, Context* ctx) {
+    require_gpu(cuda);
+    GaussCalib gc{};
+    gc.initialSigma = isx;
+    gc.initialSigmaY = isy;
+    Estimator* e = make_gauss(kind, roisize, gc);
+    attach(e, ctx);
+    return e;
+}
+
+// GaussianPSF.cu:163-185 -- x,y,I,bg,sx,sy
+Estimator* Gauss2D_CreatePSF_XYIBgSigmaXY(int roisize, float initialSigmaX, float initialSigmaY, bool cuda,
+                                          Context* ctx) {
+    return guarded([&]() -> Estimator* {
+        return create_sigmaxy(kKindSigmaXY, roisize, initialSigmaX, initialSigmaY, cuda, ctx);
+    }, nullptr);
+}
+
+// GaussianPSF.cu:187-212 -- x,y,I,bg,bgx,bgy,sx,sy
+Estimator* Gauss2D_CreatePSF_XYITiltedBgSigmaXY(int roisize, float initialSigmaX, float initialSigmaY, bool cuda,
+                                                Context* ctx) {
+    return guarded([&]() -> Estimator* {
+        return create_sigmaxy(kKindTiltedBgSigmaXY, roisize, initialSigmaX, initialSigmaY, cuda, ctx);
+    }, nullptr);
+}
+
+// GaussianPSF.cu:216-240.  Per spot i: images [n, imgw, imgw], xy [n, 2] and sigma[i] (one
+// float per spot: the i-th element of the array, as the reference reads it); a
+// Levenberg-Marquardt fit of {I, bg} from {1, 0} with maxIterations steps and lambda 1e-15,
+// then the CRLB of {I, bg}.  roipos is unused, as in the reference; the fit always runs on
+// the GPU (the cuda flag chose the reference's CPU or CUDA loop).
+void Gauss2D_EstimateIntensityBg(const float* imageData, float* IBg, float* IBg_crlb, int numspots, const float* xy,
+                                 const int* roipos, const float* sigma, int imgw, int maxIterations, bool cuda) {
+    (void)roipos;
+    (void)cuda;
+    if (numspots <= 0) return;
+    guarded(
+        [&]() -> int {
+            if (!imageData || !xy || !sigma || !IBg) throw std::runtime_error("photonpy_amd: NULL argument");
+            Estimator e;
+            e.roisize = imgw;
+            e.desc.kind = kKindIBg;
+            e.lm.iterations = maxIterations;
+            e.lm.lambda = 1e-15f;  // LevMarOptimize's default step (Estimation.h:194-196)
+            const size_t n = numspots, S = (size_t)imgw * imgw;
+            std::vector<float> cst(n * 3);
+            for (size_t i = 0; i < n; i++) {
+                cst[3 * i] = xy[2 * i];
+                cst[3 * i + 1] = xy[2 * i + 1];
+                cst[3 * i + 2] = sigma[i];
+            }
+            Slicer sl{(char*)e.workspace(Slicer::bytes({n * S * 4, n * 3 * 4, n * 2 * 4, n * 2 * 4, n * 4}))};
+            float* ds = sl.take<float>(n * S);
+            float* dc = sl.take<float>(n * 3);
+            float* dp = sl.take<float>(n * 2);
+            float* dcr = sl.take<float>(n * 2);
+            int* dit = sl.take<int>(n);
+            hip_check(hipMemcpy(ds, imageData, n * S * 4, hipMemcpyHostToDevice), "H2D images");
+            hip_check(hipMemcpy(dc, cst.data(), n * 3 * 4, hipMemcpyHostToDevice), "H2D constants");
+            run_fit(&e, numspots, ds, dc, nullptr, dp, dit, nullptr, 0, dcr, nullptr, 0);
+            hip_check(hipMemcpy(IBg, dp, n * 2 * 4, hipMemcpyDeviceToHost), "D2H IBg");
+            if (IBg_crlb) hip_check(hipMemcpy(IBg_crlb, dcr, n * 2 * 4, hipMemcpyDeviceToHost), "D2H IBg CRLB");
+            return 0;
+        },
+        0);
 }
 
 static Estimator* create_astig(int roisize, const float* x, const float* y, float minz, float maxz, float Bx, float By,
@@ -422,7 +532,7 @@ void Estimator_GetProperties(Estimator* e, EstimatorProperties* p) {
     p->numParams = e->K();
     p->sampleCount = e->S();
     p->numDiag = 0;  // the reference's GPU estimators report no diagnostics (EstimatorImpl.h:34)
-    p->numConst = 0;
+    p->numConst = e->NC();
     p->sampleIndexDims = 2;
 }
 
@@ -505,21 +615,32 @@ int Estimator_GetPhaseTimes(Estimator* e, float* fitMs, float* crlbMs) {
         -1);
 }
 
+// Uploads the per-spot constants of a host call (NULL when the model has none).
+static const float* upload_constants(Estimator* e, Slicer& sl, int n, const float* constants) {
+    const size_t C = e->NC();
+    if (C == 0) return nullptr;
+    if (!constants)
+        throw std::runtime_error("photonpy_amd: this estimator needs " + std::to_string(C) + " constants per spot");
+    float* dc = sl.take<float>(n * C);
+    hip_check(hipMemcpy(dc, constants, n * C * 4, hipMemcpyHostToDevice), "H2D constants");
+    return dc;
+}
+
 void Estimator_ComputeExpectedValue(Estimator* e, int n, const float* params, const float* constants,
                                     const int* spotpos, float* ev) {
-    (void)constants;
     (void)spotpos;
     if (!e || n <= 0) return;
     guarded(
         [&]() -> int {
             std::lock_guard<std::mutex> l(e->mtx);
             e->ensure_device();
-            const size_t K = e->K(), S = e->S();
-            Slicer sl{(char*)e->workspace(Slicer::bytes({n * K * 4, n * S * 4}))};
+            const size_t K = e->K(), S = e->S(), C = e->NC();
+            Slicer sl{(char*)e->workspace(Slicer::bytes({n * K * 4, n * S * 4, n * C * 4}))};
             float* dp = sl.take<float>(n * K);
             float* dev = sl.take<float>(n * S);
+            const float* dcn = upload_constants(e, sl, n, constants);
             hip_check(hipMemcpy(dp, params, n * K * 4, hipMemcpyHostToDevice), "H2D params");
-            hip_check(launch_ev_any(e->desc, e->roisize, n, dp, dev, nullptr, 0), "expected value kernel");
+            hip_check(launch_ev_any(e->desc, e->roisize, n, dp, dcn, dev, nullptr, 0), "expected value kernel");
             hip_check(hipMemcpy(ev, dev, n * S * 4, hipMemcpyDeviceToHost), "D2H ev");
             return 0;
         },
@@ -528,20 +649,20 @@ void Estimator_ComputeExpectedValue(Estimator* e, int n, const float* params, co
 
 void Estimator_ComputeDerivatives(Estimator* e, int n, const float* params, const float* constants,
                                   const int* spotpos, float* deriv, float* ev) {
-    (void)constants;
     (void)spotpos;
     if (!e || n <= 0) return;
     guarded(
         [&]() -> int {
             std::lock_guard<std::mutex> l(e->mtx);
             e->ensure_device();
-            const size_t K = e->K(), S = e->S();
-            Slicer sl{(char*)e->workspace(Slicer::bytes({n * K * 4, n * S * 4, n * K * S * 4}))};
+            const size_t K = e->K(), S = e->S(), C = e->NC();
+            Slicer sl{(char*)e->workspace(Slicer::bytes({n * K * 4, n * S * 4, n * K * S * 4, n * C * 4}))};
             float* dp = sl.take<float>(n * K);
             float* dev = sl.take<float>(n * S);
             float* dd = sl.take<float>(n * K * S);
+            const float* dcn = upload_constants(e, sl, n, constants);
             hip_check(hipMemcpy(dp, params, n * K * 4, hipMemcpyHostToDevice), "H2D params");
-            hip_check(launch_ev_any(e->desc, e->roisize, n, dp, dev, dd, 0), "derivatives kernel");
+            hip_check(launch_ev_any(e->desc, e->roisize, n, dp, dcn, dev, dd, 0), "derivatives kernel");
             if (ev) hip_check(hipMemcpy(ev, dev, n * S * 4, hipMemcpyDeviceToHost), "D2H ev");
             if (deriv) hip_check(hipMemcpy(deriv, dd, n * K * S * 4, hipMemcpyDeviceToHost), "D2H deriv");
             return 0;
@@ -552,26 +673,26 @@ void Estimator_ComputeDerivatives(Estimator* e, int n, const float* params, cons
 void Estimator_Estimate(Estimator* e, int n, const float* sample, const float* constants, const int* spotpos,
                         const float* initial, float* params, float* diagnostics, int* iterations, float* trace,
                         int traceBufLen) {
-    (void)constants;
     (void)spotpos;
     (void)diagnostics;
     if (!e || n <= 0) return;
     guarded(
         [&]() -> int {
             std::lock_guard<std::mutex> l(e->mtx);
-            const size_t K = e->K(), S = e->S();
+            const size_t K = e->K(), S = e->S(), C = e->NC();
             const size_t tl = (trace && traceBufLen > 0) ? (size_t)traceBufLen : 0;
-            Slicer sl{(char*)e->workspace(
-                Slicer::bytes({n * S * 4, initial ? n * K * 4 : 0, n * K * 4, (size_t)n * 4, n * tl * K * 4}))};
+            Slicer sl{(char*)e->workspace(Slicer::bytes(
+                {n * S * 4, initial ? n * K * 4 : 0, n * K * 4, (size_t)n * 4, n * tl * K * 4, n * C * 4}))};
             float* ds = sl.take<float>(n * S);
             float* di = initial ? sl.take<float>(n * K) : nullptr;
             float* dp = sl.take<float>(n * K);
             int* dit = sl.take<int>(n);
             float* dt = tl ? sl.take<float>(n * tl * K) : nullptr;
+            const float* dcn = upload_constants(e, sl, n, constants);
             hip_check(hipMemcpy(ds, sample, n * S * 4, hipMemcpyHostToDevice), "H2D samples");
             if (di) hip_check(hipMemcpy(di, initial, n * K * 4, hipMemcpyHostToDevice), "H2D initial");
             if (dt) hip_check(hipMemset(dt, 0, n * tl * K * 4), "clear trace");
-            run_fit(e, n, ds, di, dp, dit, dt, (int)tl, nullptr, nullptr, 0);
+            run_fit(e, n, ds, dcn, di, dp, dit, dt, (int)tl, nullptr, nullptr, 0);
             hip_check(hipMemcpy(params, dp, n * K * 4, hipMemcpyDeviceToHost), "D2H params");
             if (iterations) hip_check(hipMemcpy(iterations, dit, (size_t)n * 4, hipMemcpyDeviceToHost), "D2H iterations");
             if (dt) hip_check(hipMemcpy(trace, dt, n * tl * K * 4, hipMemcpyDeviceToHost), "D2H trace");
@@ -582,22 +703,23 @@ void Estimator_Estimate(Estimator* e, int n, const float* sample, const float* c
 
 void Estimator_ChiSquareAndCRLB(Estimator* e, const float* params, const float* sample, const float* constants,
                                 const int* spotpos, int n, float* crlb, float* chisq) {
-    (void)constants;
     (void)spotpos;
     if (!e || n <= 0) return;
     guarded(
         [&]() -> int {
             std::lock_guard<std::mutex> l(e->mtx);
             e->ensure_device();
-            const size_t K = e->K(), S = e->S();
-            Slicer sl{(char*)e->workspace(Slicer::bytes({n * K * 4, sample ? n * S * 4 : 0, n * K * 4, (size_t)n * 4}))};
+            const size_t K = e->K(), S = e->S(), C = e->NC();
+            Slicer sl{(char*)e->workspace(
+                Slicer::bytes({n * K * 4, sample ? n * S * 4 : 0, n * K * 4, (size_t)n * 4, n * C * 4}))};
             float* dp = sl.take<float>(n * K);
             float* ds = sample ? sl.take<float>(n * S) : nullptr;
             float* dc = sl.take<float>(n * K);
             float* dx = sl.take<float>(n);
+            const float* dcn = upload_constants(e, sl, n, constants);
             hip_check(hipMemcpy(dp, params, n * K * 4, hipMemcpyHostToDevice), "H2D params");
             if (ds) hip_check(hipMemcpy(ds, sample, n * S * 4, hipMemcpyHostToDevice), "H2D samples");
-            hip_check(launch_crlb_any(e->desc, e->roisize, n, dp, ds, dc, dx, 0), "crlb kernel");
+            hip_check(launch_crlb_any(e->desc, e->roisize, n, dp, dcn, ds, dc, dx, 0), "crlb kernel");
             if (crlb) hip_check(hipMemcpy(crlb, dc, n * K * 4, hipMemcpyDeviceToHost), "D2H crlb");
             if (chisq) hip_check(hipMemcpy(chisq, dx, (size_t)n * 4, hipMemcpyDeviceToHost), "D2H chisq");
             return 0;
@@ -606,57 +728,70 @@ void Estimator_ChiSquareAndCRLB(Estimator* e, const float* params, const float* 
 }
 
 // ---------------------------------------------------------------- device-pointer API
-int Estimator_EstimateDevice(Estimator* e, int n, const float* d_sample, const float* d_initial, float* d_params,
-                             int* d_iterations, float* d_trace, int traceLen, float* d_crlb, float* d_chisq,
-                             void* stream) {
+static void require_constants(Estimator* e, const float* d_constants) {
+    if (e->NC() > 0 && !d_constants)
+        throw std::runtime_error("photonpy_amd: this estimator needs " + std::to_string(e->NC()) +
+                                 " constants per spot");
+}
+
+int Estimator_EstimateDevice(Estimator* e, int n, const float* d_sample, const float* d_constants,
+                             const float* d_initial, float* d_params, int* d_iterations, float* d_trace, int traceLen,
+                             float* d_crlb, float* d_chisq, void* stream) {
     if (!e) return 1;
     if (n <= 0) return 0;
     return guarded(
         [&]() -> int {
             std::lock_guard<std::mutex> l(e->mtx);
-            run_fit(e, n, d_sample, d_initial, d_params, d_iterations, d_trace, traceLen, d_crlb, d_chisq,
+            run_fit(e, n, d_sample, d_constants, d_initial, d_params, d_iterations, d_trace, traceLen, d_crlb, d_chisq,
                     as_stream(stream));
             return 0;
         },
         2);
 }
 
-int Estimator_ExpectedValueDevice(Estimator* e, int n, const float* d_params, float* d_ev, void* stream) {
+int Estimator_ExpectedValueDevice(Estimator* e, int n, const float* d_params, const float* d_constants, float* d_ev,
+                                  void* stream) {
     if (!e) return 1;
     if (n <= 0) return 0;
     return guarded(
         [&]() -> int {
             std::lock_guard<std::mutex> l(e->mtx);
             e->ensure_device();
-            hip_check(launch_ev_any(e->desc, e->roisize, n, d_params, d_ev, nullptr, as_stream(stream)), "ev kernel");
+            require_constants(e, d_constants);
+            hip_check(launch_ev_any(e->desc, e->roisize, n, d_params, d_constants, d_ev, nullptr, as_stream(stream)),
+                      "ev kernel");
             return 0;
         },
         2);
 }
 
-int Estimator_DerivativesDevice(Estimator* e, int n, const float* d_params, float* d_deriv, float* d_ev, void* stream) {
+int Estimator_DerivativesDevice(Estimator* e, int n, const float* d_params, const float* d_constants, float* d_deriv,
+                                float* d_ev, void* stream) {
     if (!e) return 1;
     if (n <= 0) return 0;
     return guarded(
         [&]() -> int {
             std::lock_guard<std::mutex> l(e->mtx);
             e->ensure_device();
-            hip_check(launch_ev_any(e->desc, e->roisize, n, d_params, d_ev, d_deriv, as_stream(stream)),
+            require_constants(e, d_constants);
+            hip_check(launch_ev_any(e->desc, e->roisize, n, d_params, d_constants, d_ev, d_deriv, as_stream(stream)),
                       "derivatives kernel");
             return 0;
         },
         2);
 }
 
-int Estimator_ChiSquareAndCRLBDevice(Estimator* e, int n, const float* d_params, const float* d_sample, float* d_crlb,
-                                     float* d_chisq, void* stream) {
+int Estimator_ChiSquareAndCRLBDevice(Estimator* e, int n, const float* d_params, const float* d_constants,
+                                     const float* d_sample, float* d_crlb, float* d_chisq, void* stream) {
     if (!e) return 1;
     if (n <= 0) return 0;
     return guarded(
         [&]() -> int {
             std::lock_guard<std::mutex> l(e->mtx);
             e->ensure_device();
-            hip_check(launch_crlb_any(e->desc, e->roisize, n, d_params, d_sample, d_crlb, d_chisq, as_stream(stream)),
+            require_constants(e, d_constants);
+            hip_check(launch_crlb_any(e->desc, e->roisize, n, d_params, d_constants, d_sample, d_crlb, d_chisq,
+                                      as_stream(stream)),
                       "crlb kernel");
             return 0;
         },
@@ -681,6 +816,9 @@ int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const fl
         [&]() -> int {
             std::lock_guard<std::mutex> l(e->mtx);
             if (e->roisize > 32) throw std::runtime_error("photonpy_amd: the adjoint supports roisize <= 32");
+            if (e->desc.kind > kKindSpline)
+                throw std::runtime_error("photonpy_amd: the adjoint covers the fixed-sigma, free-sigma, astigmatic and "
+                                         "spline models");
             AdjArgs a{};
             a.n = n;
             a.R = e->roisize;

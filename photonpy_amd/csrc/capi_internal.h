@@ -96,6 +96,7 @@ struct Estimator {
     }
 
     int K() const { return num_params(desc.kind); }
+    int NC() const { return num_consts(desc.kind); }
 
     // Shared PSF parameters in the adjoint's layout
     std::vector<float> shared_values() const {
@@ -197,5 +198,6 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // Fused fit (+ CRLB / chi^2 when d_crlb / d_chisq are given) of n spots on st with the
 // estimator's model, LM settings, lane-group size, refill batch and next work counter;
 // timing = false keeps phase-timing events out (graph capture).
-void run_fit(Estimator* e, int n, const float* d_sample, const float* d_initial, float* d_params, int* d_iter,
-             float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st, bool timing = true);
+void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constants, const float* d_initial,
+             float* d_params, int* d_iter, float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st,
+             bool timing = true);

@@ -157,9 +157,9 @@ PP_DEV bool lm_step_f64(const float* alphaTri, const float* beta, float lambda, 
 
 // CRLB from the packed Fisher matrix: FP64 LU with partial pivoting and the
 // reference's failure rule (largest pivot < 1e-9 -> Inf), sqrt of the
-// diagonal of the inverse without abs (SolveMatrix.h:219-263, 344-365,
-// 441-452; Estimation.h:101).
-template <int K>
+// diagonal of the inverse, without abs (SolveMatrix.h:219-263, 344-365,
+// 441-452; Estimation.h:101) or with it (ABS: ComputeCRLB, Estimation.h:318-333).
+template <int K, bool ABS = false>
 PP_DEV void crlb_from_fisher(const float* fiTri, float* crlb) {
     double A[K][K];
 #pragma unroll
@@ -225,7 +225,7 @@ PP_DEV void crlb_from_fisher(const float* fiTri, float* crlb) {
             for (int k = i + 1; k < K; k++) v -= A[i][k] * x[k];
             x[i] = v / A[i][i];
         }
-        crlb[j] = (float)sqrt(x[j]);
+        crlb[j] = (float)sqrt(ABS ? fabs(x[j]) : x[j]);
     }
 }
 

@@ -8,7 +8,17 @@
 
 namespace ppa {
 
-enum ModelKind { kKindFixed = 0, kKindSigma = 1, kKindAstig = 2, kKindSpline = 3 };
+enum ModelKind {
+    kKindFixed = 0,
+    kKindSigma = 1,
+    kKindAstig = 2,
+    kKindSpline = 3,
+    kKindConstSigma = 4,
+    kKindTiltedBg = 5,
+    kKindSigmaXY = 6,
+    kKindTiltedBgSigmaXY = 7,
+    kKindIBg = 8,
+};
 
 struct ModelDesc {
     int kind;
@@ -18,7 +28,26 @@ struct ModelDesc {
     bool useGrid;        // spline: fit / evaluate from sg instead of sm
 };
 
-inline int num_params(int kind) { return kind == kKindFixed ? 4 : 5; }
+inline int num_params(int kind) {
+    switch (kind) {
+    case kKindFixed: case kKindConstSigma: return GaussModel<kFixed>::K;
+    case kKindTiltedBg: return GaussModel<kTiltedBg>::K;
+    case kKindSigmaXY: return GaussModel<kSigmaXY>::K;
+    case kKindTiltedBgSigmaXY: return GaussModel<kTiltedBgSigmaXY>::K;
+    case kKindIBg: return IBgModel::K;
+    default: return 5;
+    }
+}
+
+// Per-spot constants of a model (the reference's NumConstants).
+inline int num_consts(int kind) {
+    switch (kind) {
+    case kKindConstSigma: return GaussModel<kConstSigma>::NC;
+    case kKindTiltedBg: return GaussModel<kTiltedBg>::NC;
+    case kKindIBg: return IBgModel::NC;
+    default: return 0;
+    }
+}
 
 // Lane-group size used for an R x R ROI unless the tuning table says otherwise:
 // the smallest power of two that keeps <= ~64 samples per lane in registers.
@@ -53,10 +82,18 @@ hipError_t launch_fit_astig(const ModelDesc& d, int R, int G, const FitArgs& a, 
 hipError_t launch_fit_spline(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st);
 hipError_t launch_fit_splinegrid(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events,
                                  hipStream_t st);
+hipError_t launch_fit_constsigma(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events,
+                                 hipStream_t st);
+hipError_t launch_fit_tiltedbg(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st);
+hipError_t launch_fit_sigmaxy(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st);
+hipError_t launch_fit_tiltedbgsigmaxy(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events,
+                                      hipStream_t st);
+hipError_t launch_fit_ibg(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st);
 
-hipError_t launch_ev_any(const ModelDesc& d, int R, int n, const float* params, float* ev, float* deriv, hipStream_t st);
-hipError_t launch_crlb_any(const ModelDesc& d, int R, int n, const float* params, const float* sample, float* crlb,
-                           float* chisq, hipStream_t st);
+hipError_t launch_ev_any(const ModelDesc& d, int R, int n, const float* params, const float* constants, float* ev,
+                         float* deriv, hipStream_t st);
+hipError_t launch_crlb_any(const ModelDesc& d, int R, int n, const float* params, const float* constants,
+                           const float* sample, float* crlb, float* chisq, hipStream_t st);
 
 // Poisson NLL and its adjoint (adjoint.hip)
 struct AdjArgs {
@@ -81,6 +118,11 @@ inline hipError_t launch_fit_any(const ModelDesc& d, int R, int G, const FitArgs
     case kKindAstig: return launch_fit_astig(d, R, G, a, events, st);
     case kKindSpline:
         return d.useGrid ? launch_fit_splinegrid(d, R, G, a, events, st) : launch_fit_spline(d, R, G, a, events, st);
+    case kKindConstSigma: return launch_fit_constsigma(d, R, G, a, events, st);
+    case kKindTiltedBg: return launch_fit_tiltedbg(d, R, G, a, events, st);
+    case kKindSigmaXY: return launch_fit_sigmaxy(d, R, G, a, events, st);
+    case kKindTiltedBgSigmaXY: return launch_fit_tiltedbgsigmaxy(d, R, G, a, events, st);
+    case kKindIBg: return launch_fit_ibg(d, R, G, a, events, st);
     default: return hipErrorInvalidValue;
     }
 }

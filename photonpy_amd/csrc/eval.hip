@@ -5,42 +5,48 @@
 namespace ppa {
 
 template <class M>
-static hipError_t ev_t(const M& m, int R, int n, const float* params, float* ev, float* deriv, hipStream_t st) {
+static hipError_t ev_t(const M& m, int R, int n, const float* params, const float* constants, float* ev, float* deriv,
+                       hipStream_t st) {
     const size_t total = (size_t)n * R * R;
     const unsigned blocks = (unsigned)((total + 255) / 256);
-    hipLaunchKernelGGL(ev_kernel<M>, dim3(blocks), dim3(256), 0, st, m, n, R, params, ev, deriv);
+    hipLaunchKernelGGL(ev_kernel<M>, dim3(blocks), dim3(256), 0, st, m, n, R, params, constants, ev, deriv);
     return hipGetLastError();
 }
 
 template <class M>
-static hipError_t crlb_t(const M& m, int R, int n, const float* params, const float* sample, float* crlb,
-                         float* chisq, hipStream_t st) {
+static hipError_t crlb_t(const M& m, int R, int n, const float* params, const float* constants, const float* sample,
+                         float* crlb, float* chisq, hipStream_t st) {
     const unsigned blocks = (unsigned)(((size_t)n * kWave + 255) / 256);
-    hipLaunchKernelGGL(crlb_kernel<M>, dim3(blocks), dim3(256), 0, st, m, n, R, params, sample, crlb, chisq);
+    hipLaunchKernelGGL(crlb_kernel<M>, dim3(blocks), dim3(256), 0, st, m, n, R, params, constants, sample, crlb,
+                       chisq);
     return hipGetLastError();
 }
 
-hipError_t launch_ev_any(const ModelDesc& d, int R, int n, const float* params, float* ev, float* deriv, hipStream_t st) {
+// Calls f(model) with the model object of a descriptor.
+template <class F>
+static hipError_t with_model(const ModelDesc& d, F&& f) {
     switch (d.kind) {
-    case kKindFixed: return ev_t(GaussModel<kFixed>{d.gc}, R, n, params, ev, deriv, st);
-    case kKindSigma: return ev_t(GaussModel<kSigma>{d.gc}, R, n, params, ev, deriv, st);
-    case kKindAstig: return ev_t(GaussModel<kAstig>{d.gc}, R, n, params, ev, deriv, st);
-    case kKindSpline: return d.useGrid ? ev_t(d.sg, R, n, params, ev, deriv, st) : ev_t(d.sm, R, n, params, ev, deriv, st);
+    case kKindFixed: return f(GaussModel<kFixed>{d.gc});
+    case kKindSigma: return f(GaussModel<kSigma>{d.gc});
+    case kKindAstig: return f(GaussModel<kAstig>{d.gc});
+    case kKindSpline: return d.useGrid ? f(d.sg) : f(d.sm);
+    case kKindConstSigma: return f(GaussModel<kConstSigma>{d.gc});
+    case kKindTiltedBg: return f(GaussModel<kTiltedBg>{d.gc});
+    case kKindSigmaXY: return f(GaussModel<kSigmaXY>{d.gc});
+    case kKindTiltedBgSigmaXY: return f(GaussModel<kTiltedBgSigmaXY>{d.gc});
+    case kKindIBg: return f(IBgModel{});
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_crlb_any(const ModelDesc& d, int R, int n, const float* params, const float* sample, float* crlb,
-                           float* chisq, hipStream_t st) {
-    switch (d.kind) {
-    case kKindFixed: return crlb_t(GaussModel<kFixed>{d.gc}, R, n, params, sample, crlb, chisq, st);
-    case kKindSigma: return crlb_t(GaussModel<kSigma>{d.gc}, R, n, params, sample, crlb, chisq, st);
-    case kKindAstig: return crlb_t(GaussModel<kAstig>{d.gc}, R, n, params, sample, crlb, chisq, st);
-    case kKindSpline:
-        return d.useGrid ? crlb_t(d.sg, R, n, params, sample, crlb, chisq, st)
-                         : crlb_t(d.sm, R, n, params, sample, crlb, chisq, st);
-    default: return hipErrorInvalidValue;
-    }
+hipError_t launch_ev_any(const ModelDesc& d, int R, int n, const float* params, const float* constants, float* ev,
+                         float* deriv, hipStream_t st) {
+    return with_model(d, [&](const auto& m) { return ev_t(m, R, n, params, constants, ev, deriv, st); });
+}
+
+hipError_t launch_crlb_any(const ModelDesc& d, int R, int n, const float* params, const float* constants,
+                           const float* sample, float* crlb, float* chisq, hipStream_t st) {
+    return with_model(d, [&](const auto& m) { return crlb_t(m, R, n, params, constants, sample, crlb, chisq, st); });
 }
 
 // ---------------------------------------------------------------------------

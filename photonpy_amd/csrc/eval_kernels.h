@@ -12,19 +12,22 @@ namespace ppa {
 
 template <class M>
 __global__ void __launch_bounds__(256) ev_kernel(M model, int n, int R, const float* __restrict__ params,
-                                                 float* __restrict__ ev, float* __restrict__ deriv) {
-    constexpr int K = M::K;
+                                                 const float* __restrict__ constants, float* __restrict__ ev,
+                                                 float* __restrict__ deriv) {
+    constexpr int K = M::K, KT = M::KT;
     const int S = R * R;
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (size_t)n * S) return;
     const int spot = (int)(idx / S);
     const int p = (int)(idx - (size_t)spot * S);
     const int y = p / R, x = p - (p / R) * R;
-    float th[K];
+    float th[KT];
 #pragma unroll
     for (int k = 0; k < K; k++) th[k] = params[(size_t)spot * K + k];
+#pragma unroll
+    for (int c = 0; c < M::NC; c++) th[K + c] = constants[(size_t)spot * M::NC + c];
     float mu, J[K];
-    model.eval_pixel(th, x, y, mu, J);
+    model.eval_pixel(th, x, y, R, mu, J);
     if (ev) ev[idx] = mu;
     if (deriv) {
 #pragma unroll
@@ -34,23 +37,25 @@ __global__ void __launch_bounds__(256) ev_kernel(M model, int n, int R, const fl
 
 template <class M>
 __global__ void __launch_bounds__(256) crlb_kernel(M model, int n, int R, const float* __restrict__ params,
-                                                   const float* __restrict__ sample, float* __restrict__ crlb,
-                                                   float* __restrict__ chisq) {
-    constexpr int K = M::K, NA = K * (K + 1) / 2;
+                                                   const float* __restrict__ constants, const float* __restrict__ sample,
+                                                   float* __restrict__ crlb, float* __restrict__ chisq) {
+    constexpr int K = M::K, KT = M::KT, NA = K * (K + 1) / 2;
     const int lane = threadIdx.x & (kWave - 1);
     const int spot = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
     if (spot >= n) return;
     const int S = R * R;
-    float th[K];
+    float th[KT];
 #pragma unroll
     for (int k = 0; k < K; k++) th[k] = params[(size_t)spot * K + k];
+#pragma unroll
+    for (int c = 0; c < M::NC; c++) th[K + c] = constants[(size_t)spot * M::NC + c];
     float fi[NA + 1];
 #pragma unroll
     for (int i = 0; i < NA + 1; i++) fi[i] = 0.0f;
     for (int p = lane; p < S; p += kWave) {
         const int y = p / R, x = p - (p / R) * R;
         float mu, J[K];
-        model.eval_pixel(th, x, y, mu, J);
+        model.eval_pixel(th, x, y, R, mu, J);
         const float m = fmaxf(1e-8f, mu);
         const float inv = rcpf(m);
         if (sample) {
@@ -68,7 +73,7 @@ __global__ void __launch_bounds__(256) crlb_kernel(M model, int n, int R, const 
     group_sum_n<kWave>(fi);
     if (lane == 0) {
         float c[K];
-        crlb_from_fisher<K>(fi, c);
+        crlb_from_fisher<K, M::kCrlbAbs>(fi, c);
         if (crlb) {
 #pragma unroll
             for (int k = 0; k < K; k++) crlb[(size_t)spot * K + k] = c[k];

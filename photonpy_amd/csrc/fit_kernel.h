@@ -45,12 +45,14 @@ struct FitArgs {
     float lambda;
     int refill;             // stopped groups per batched refill (<= 0: one eighth of the wave's groups)
     int* counter;           // next unclaimed spot (starts at the grid's group count), or null: static stride
+    const float* constants; // [*, NC] per-spot model constants (models with NC > 0)
 };
 
 template <class M, int R, int G>
 struct FitTraits {
     using L = typename M::template Layout<R, G>;
     static constexpr int K = M::K;
+    static constexpr int KT = M::KT;             // parameters + per-spot constants
     static constexpr int S = R * R;
     static constexpr int NS = L::NS;             // samples per lane
     static constexpr int NA = K * (K + 1) / 2;   // packed alpha
@@ -86,7 +88,7 @@ PP_DEV void load_samples(const float* src, int g, float (&s)[NS], float& sx, flo
 // Fisher matrix (packed) and chi^2 of one spot, written by lane g == 0 of its group.
 template <class M, int R, int G>
 PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTraits<M, R, G>::NS],
-                      const float (&th)[M::K], int g, float* lds, int spot) {
+                      const float (&th)[M::KT], int g, float* lds, int spot) {
     using T = FitTraits<M, R, G>;
     constexpr int K = T::K, NA = T::NA;
     float fi[NA + 1];
@@ -109,7 +111,7 @@ PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTrai
     if (g == 0) {
         if (a.crlb) {
             float c[K];
-            crlb_from_fisher<K>(fi, c);
+            crlb_from_fisher<K, M::kCrlbAbs>(fi, c);
             float* cp = a.crlb + (size_t)spot * K;
 #pragma unroll
             for (int k = 0; k < K; k++) cp[k] = c[k];
@@ -135,14 +137,17 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
     const bool lead = (g == 0);
     const int refill = a.refill > 0 ? a.refill : (P >= 8 ? P / 8 : 1);
 
+    constexpr int KT = T::KT;
     float s[NS];
-    float th[K];
+    float th[KT];
     int iter = 0;
     int spot = g_first;
 
     auto start_spot = [&](int sp) {
         float sx, sy, sum;
         load_samples<R, L, NS, true, true>(a.sample + (size_t)sp * T::S, g, s, sx, sy, sum);
+#pragma unroll
+        for (int c = 0; c < M::NC; c++) th[K + c] = a.constants[(size_t)sp * M::NC + c];
         if (a.initial) {
 #pragma unroll
             for (int k = 0; k < K; k++) th[k] = a.initial[(size_t)sp * K + k];
@@ -256,16 +261,18 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
 template <class M, int R, int G>
 __device__ __forceinline__ void crlb_spots(const M& model, const FitArgs& a, int g_first, int stride, float* lds) {
     using T = FitTraits<M, R, G>;
-    constexpr int K = T::K, NS = T::NS;
+    constexpr int K = T::K, NS = T::NS, KT = T::KT;
     const int g = threadIdx.x & (G - 1);
     float s[NS];
-    float th[K];
+    float th[KT];
     for (int sp = g_first; __any(sp < a.n); sp += stride) {
         if (sp < a.n) {
             float sx, sy, sum;
             load_samples<R, typename T::L, NS, false, false>(a.sample + (size_t)sp * T::S, g, s, sx, sy, sum);
 #pragma unroll
             for (int k = 0; k < K; k++) th[k] = a.params[(size_t)sp * K + k];
+#pragma unroll
+            for (int c = 0; c < M::NC; c++) th[K + c] = a.constants[(size_t)sp * M::NC + c];
             crlb_spot<M, R, G>(model, a, s, th, g, lds, sp);
         }
     }

@@ -15,5 +15,12 @@
 #define PPA_GRID_SIZES_2(X) X(17, 16) X(18, 16) X(19, 16) X(20, 16) X(20, 32) X(21, 16) X(22, 16) X(23, 16)
 #define PPA_GRID_SIZES_3(X) X(24, 16) X(25, 16) X(25, 32) X(26, 16) X(27, 16) X(28, 16) X(29, 16) X(30, 16) X(31, 16) X(32, 16)
 
+// The reference's other Gaussian variants and the intensity/background model:
+// default_group(R) for every R.
+#define PPA_VAR_SIZES_0(X) X(3, 1) X(4, 1) X(5, 2) X(6, 2) X(7, 2) X(8, 2) X(9, 2) X(10, 2)
+#define PPA_VAR_SIZES_1(X) X(11, 2) X(12, 4) X(13, 4) X(14, 4) X(15, 4) X(16, 4)
+#define PPA_VAR_SIZES_2(X) X(17, 8) X(18, 8) X(19, 8) X(20, 8) X(21, 8) X(22, 8) X(23, 8) X(24, 8)
+#define PPA_VAR_SIZES_3(X) X(25, 16) X(26, 16) X(27, 16) X(28, 16) X(29, 16) X(30, 16) X(31, 16) X(32, 16)
+
 #define PPA_FIT_CASE(R_, G_) \
     if (R == R_ && G == G_) return launch_fit_t<Model, R_, G_>(m, a, events, st);

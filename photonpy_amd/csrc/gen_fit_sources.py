@@ -8,7 +8,12 @@ MODELS = [("fixed", "GaussModel<kFixed>", "Model m{d.gc};", "PPA_FIT_SIZES"),
           ("sigma", "GaussModel<kSigma>", "Model m{d.gc};", "PPA_FIT_SIZES"),
           ("astig", "GaussModel<kAstig>", "Model m{d.gc};", "PPA_FIT_SIZES"),
           ("spline", "SplineModel", "Model m = d.sm;", "PPA_FIT_SIZES"),
-          ("splinegrid", "SplineGridModel", "Model m = d.sg;", "PPA_GRID_SIZES")]
+          ("splinegrid", "SplineGridModel", "Model m = d.sg;", "PPA_GRID_SIZES"),
+          ("constsigma", "GaussModel<kConstSigma>", "Model m{d.gc};", "PPA_VAR_SIZES"),
+          ("tiltedbg", "GaussModel<kTiltedBg>", "Model m{d.gc};", "PPA_VAR_SIZES"),
+          ("sigmaxy", "GaussModel<kSigmaXY>", "Model m{d.gc};", "PPA_VAR_SIZES"),
+          ("tiltedbgsigmaxy", "GaussModel<kTiltedBgSigmaXY>", "Model m{d.gc};", "PPA_VAR_SIZES"),
+          ("ibg", "IBgModel", "Model m{};", "PPA_VAR_SIZES")]
 PARTS = 4
 
 for name, ty, ctor, sizes in MODELS:

@@ -7,6 +7,13 @@
 //   GaussModel<kAstig>  x,y,z,I,bg      Gauss2D_Model_XYZIBg     GaussianPSFModels.h:636-751
 //                       (sigma(z) gains a quartic term B*t^4/d^2 under the square root;
 //                        B = 0 is the reference model)
+//   GaussModel<kConstSigma>      x,y,I,bg + sigma x,y per spot    GaussianPSFModels.h:155-229
+//   GaussModel<kTiltedBg>        x,y,I,bg,bgx,bgy + sigma x,y     GaussianPSFModels.h:232-304
+//   GaussModel<kSigmaXY>         x,y,I,bg,sx,sy                   GaussianPSFModels.h:388-467
+//   GaussModel<kTiltedBgSigmaXY> x,y,I,bg,bgx,bgy,sx,sy           GaussianPSFModels.h:471-553
+//   IBgModel            I,bg at a given x,y,sigma   IntensityBgModel GaussianPSFModels.h:557-623
+// Per-spot constants (NC of them) follow the K parameters in the parameter array
+// the kernels pass around; the Levenberg-Marquardt step never touches them.
 //   SplineModel         x,y,z,I,bg      CSpline_Model_XYZIBg     CubicSplinePSFModels.h:14-154
 //
 // The Gaussians are pixel-integrated: for pixel edge e the argument is
@@ -48,7 +55,7 @@ struct FlatLayout {
     PP_DEV static int y(int g, int s) { return index(g, s) / R; }
 };
 
-enum GaussVariant { kFixed = 0, kSigma = 1, kAstig = 2 };
+enum GaussVariant { kFixed = 0, kSigma = 1, kAstig = 2, kConstSigma = 3, kTiltedBg = 4, kSigmaXY = 5, kTiltedBgSigmaXY = 6 };
 
 constexpr float kPiF = 3.141592653589793f;
 
@@ -106,9 +113,10 @@ PP_DEV Edge fetch_edge(const float (&Ev)[PER], const float (&Gv)[PER], const flo
 // Calibration of the Gaussian models.  ax/ay = {s0, gamma, d, A, B} per axis.
 struct GaussCalib {
     float sx, sy;          // kFixed
-    float initialSigma;    // kSigma
+    float initialSigma;    // kSigma; x for kSigmaXY / kTiltedBgSigmaXY
     float ax[5], ay[5];    // kAstig
     float minz, maxz;      // kAstig
+    float initialSigmaY;   // kSigmaXY / kTiltedBgSigmaXY
 };
 
 // Per-spot, per-iteration shape constants of one axis.
@@ -120,26 +128,38 @@ struct AxisShape {
 
 template <int V>
 struct GaussModel {
-    static constexpr int K = (V == kFixed) ? 4 : 5;
+    static constexpr bool kTilt = (V == kTiltedBg || V == kTiltedBgSigmaXY);
+    static constexpr bool kSigXY = (V == kSigmaXY || V == kTiltedBgSigmaXY);
+    static constexpr int K = (V == kFixed || V == kConstSigma) ? 4
+                             : (V == kTiltedBg || V == kSigmaXY) ? 6
+                             : (V == kTiltedBgSigmaXY) ? 8 : 5;
+    static constexpr int NC = (V == kConstSigma || V == kTiltedBg) ? 2 : 0;  // sigma x, y per spot
+    static constexpr int KT = K + NC;
     static constexpr int iX = 0, iY = 1;
     static constexpr int iZ = 2;                               // kAstig only
     static constexpr int iI = (V == kAstig) ? 3 : 2;
     static constexpr int iBg = (V == kAstig) ? 4 : 3;
     static constexpr int iS = (V == kSigma) ? 4 : (V == kAstig ? 2 : -1);  // column fed by S terms
+    static constexpr int iBgX = 4, iBgY = 5;                   // tilted background
+    static constexpr int iSx = (V == kSigmaXY) ? 4 : 6;        // sigma x, y columns
+    static constexpr int iSy = iSx + 1;
+    static constexpr bool kCrlbAbs = false;
 
     GaussCalib c;
 
     PP_DEV static float stop_limit(int k) {
-        if (V == kFixed) return (k == 2) ? 1e-1f : 1e-5f;
-        if (V == kSigma) return (k == 2) ? 1e-1f : (k == 3 ? 1e-4f : 1e-5f);
+        if (V == kFixed || V == kConstSigma) return (k == 2) ? 1e-1f : 1e-5f;
+        if (V == kTiltedBg) return (k == 2) ? 1e-1f : 1e-5f;
+        if (V == kSigma || V == kSigmaXY) return (k == 2) ? 1e-1f : (k == 3 ? 1e-4f : 1e-5f);
+        if (V == kTiltedBgSigmaXY) return (k == 2) ? 1e-1f : ((k >= 3 && k <= 5) ? 1e-4f : 1e-5f);
         return (k == 2) ? 1e-1f : 1e-4f;
     }
 
-    PP_DEV void check_limits(float (&t)[K], int R) const {
+    PP_DEV void check_limits(float (&t)[KT], int R) const {
         const float hi = (float)R - 1.5f - 1.0f;
         t[0] = clampf(t[0], 1.5f, hi);
         t[1] = clampf(t[1], 1.5f, hi);
-        if (V == kFixed) {
+        if (V == kFixed || V == kConstSigma || V == kTiltedBg) {
             if (t[2] < 25.0f) t[2] = 25.0f;
             if (t[3] < 0.0f) t[3] = 0.0f;
         } else if (V == kSigma) {
@@ -147,6 +167,11 @@ struct GaussModel {
             if (t[3] < 0.0f) t[3] = 0.0f;
             if (t[4] < 1.0f) t[4] = 1.0f;
             if (t[4] > (float)R * 0.5f) t[4] = (float)R * 0.5f;
+        } else if (kSigXY) {
+            if (t[2] < 10.0f) t[2] = 10.0f;
+            if (t[3] < 0.0f) t[3] = 0.0f;
+            t[iSx] = clampf(t[iSx], 1.0f, (float)R * 0.5f);
+            t[iSy] = clampf(t[iSy], 1.0f, (float)R * 0.5f);
         } else {
             t[2] = clampf(t[2], c.minz, c.maxz);
             t[3] = fmaxf(t[3], 10.f);
@@ -154,11 +179,19 @@ struct GaussModel {
         }
     }
 
-    PP_DEV void initial(float cx, float cy, float sum, float (&t)[K]) const {
+    PP_DEV void initial(float cx, float cy, float sum, float (&t)[KT]) const {
         t[0] = cx;
         t[1] = cy;
-        if (V == kFixed) { t[2] = sum * 0.7f; t[3] = 0.0f; }
+        if (V == kFixed || V == kConstSigma) { t[2] = sum * 0.7f; t[3] = 0.0f; }
+        else if (V == kTiltedBg) { t[2] = sum * 0.7f; t[3] = 0.0f; t[4] = 0.0f; t[5] = 0.0f; }
         else if (V == kSigma) { t[2] = sum; t[3] = 0.0f; t[4] = c.initialSigma; }
+        else if (kSigXY) {
+            t[2] = sum;
+            t[3] = 0.0f;
+            if (kTilt) { t[4] = 0.0f; t[5] = 0.0f; }
+            t[iSx] = c.initialSigma;
+            t[iSy] = c.initialSigmaY;
+        }
         else { t[2] = 0.0f; t[3] = sum; t[4] = 0.01f; }
     }
 
@@ -173,11 +206,20 @@ struct GaussModel {
         dsdz = s0 * (2 * t / d2 + A * 3 * t2 / d2 + B * 4 * t3 / d2) / (2 * sq);
     }
 
-    PP_DEV void shapes(const float (&t)[K], AxisShape& X, AxisShape& Y) const {
+    PP_DEV static void fixed_shape(float sigma, float r2pi, float r2, AxisShape& A) {
+        A.cp = 1.0f / (r2pi * sigma);
+        A.c2 = 1.0f / (r2 * sigma);
+        A.cs = 0.0f;
+    }
+
+    PP_DEV void shapes(const float (&t)[KT], AxisShape& X, AxisShape& Y) const {
         const float r2pi = sqrtf(2 * kPiF), r2 = sqrtf(2.0f), ispi = 1.0f / sqrtf(kPiF);
         if (V == kFixed) {
-            X.cp = 1.0f / (r2pi * c.sx); X.c2 = 1.0f / (r2 * c.sx); X.cs = 0.0f;
-            Y.cp = 1.0f / (r2pi * c.sy); Y.c2 = 1.0f / (r2 * c.sy); Y.cs = 0.0f;
+            fixed_shape(c.sx, r2pi, r2, X);
+            fixed_shape(c.sy, r2pi, r2, Y);
+        } else if (NC == 2) {
+            fixed_shape(t[K], r2pi, r2, X);
+            fixed_shape(t[K + 1], r2pi, r2, Y);
         } else if (V == kSigma) {
             const float s = t[4];
             X.cp = Y.cp = 1.0f / (r2pi * s);
@@ -185,6 +227,11 @@ struct GaussModel {
             // The reference's d(DeltaX)/d(sigma) omits the 1/sigma factor
             // (GaussianPSFModels.h:359,368); kept for parity of Estimate,
             // Derivatives and CRLB.
+            X.cs = Y.cs = ispi;
+        } else if (kSigXY) {
+            // same 1/sigma omission per axis (GaussianPSFModels.h:434-447)
+            fixed_shape(t[iSx], r2pi, r2, X);
+            fixed_shape(t[iSy], r2pi, r2, Y);
             X.cs = Y.cs = ispi;
         } else {
             float sgx, dsx, sgy, dsy;
@@ -195,17 +242,29 @@ struct GaussModel {
         }
     }
 
-    // mu and Jacobian of one pixel from its column (X) and row (Y) terms, whose
-    // dE and S are already multiplied by I.
-    PP_DEV void pixel(const AxisTerms& X, const AxisTerms& Y, const float (&t)[K], float& mu, float (&J)[K]) const {
+    // mu and Jacobian of pixel (x, y) from its column (X) and row (Y) terms, whose dE
+    // and S are already multiplied by I; half = R / 2 (the tilt's origin).  The
+    // kTiltedBgSigmaXY reference leaves the tilt out of mu while its Jacobian has
+    // the tilt columns (GaussianPSFModels.h:537-540); kept for parity.
+    PP_DEV void pixel(const AxisTerms& X, const AxisTerms& Y, float x, float y, float half, const float (&t)[KT],
+                      float& mu, float (&J)[K]) const {
         const float I = t[iI], bg = t[iBg];
         const float p = X.D * Y.D;
         mu = bg + I * p;
+        if (V == kTiltedBg) mu = bg + t[iBgX] * (x - half) + t[iBgY] * (y - half) + I * p;
         J[0] = X.dE * Y.D;
         J[1] = Y.dE * X.D;
         J[iI] = p;
         J[iBg] = 1.0f;
-        if (V != kFixed) J[iS] = X.S * Y.D + Y.S * X.D;
+        if (V == kSigma || V == kAstig) J[iS] = X.S * Y.D + Y.S * X.D;
+        if (kSigXY) {
+            J[iSx] = X.S * Y.D;
+            J[iSy] = Y.S * X.D;
+        }
+        if (kTilt) {
+            J[iBgX] = x - half;
+            J[iBgY] = y - half;
+        }
     }
 
     PP_DEV static AxisTerms scaled(AxisTerms a, float I) {
@@ -214,13 +273,13 @@ struct GaussModel {
     }
 
     // Single-pixel evaluation (expected value / derivatives kernels).
-    PP_DEV void eval_pixel(const float (&t)[K], int x, int y, float& mu, float (&J)[K]) const {
+    PP_DEV void eval_pixel(const float (&t)[KT], int x, int y, int R, float& mu, float (&J)[K]) const {
         AxisShape SX, SY;
         shapes(t, SX, SY);
         const Edge x0 = gauss_edge((float)x, t[0], SX.c2), x1 = gauss_edge((float)(x + 1), t[0], SX.c2);
         const Edge y0 = gauss_edge((float)y, t[1], SY.c2), y1 = gauss_edge((float)(y + 1), t[1], SY.c2);
-        pixel(scaled(axis_terms(x0, x1, SX.cp, SX.cs), t[iI]), scaled(axis_terms(y0, y1, SY.cp, SY.cs), t[iI]), t,
-              mu, J);
+        pixel(scaled(axis_terms(x0, x1, SX.cp, SX.cs), t[iI]), scaled(axis_terms(y0, y1, SY.cp, SY.cs), t[iI]),
+              (float)x, (float)y, (float)R * 0.5f, t, mu, J);
     }
 
     template <int R, int G>
@@ -230,7 +289,7 @@ struct GaussModel {
     // Fit-kernel evaluation of lane g's pixels: fn(slot, mu, J) for every slot of
     // RowLayout<R, G> that lies inside the ROI.
     template <int R, int G, class Fn>
-    PP_DEV void for_pixels(const float (&t)[K], int g, float*, Fn&& fn) const {
+    PP_DEV void for_pixels(const float (&t)[KT], int g, float*, Fn&& fn) const {
         constexpr int RL = RowLayout<R, G>::RL;
         for_rows<R, RL, G>(t, g * RL, [&](int r, int x, float mu, const float (&J)[K]) { fn(r * R + x, mu, J); });
     }
@@ -241,7 +300,7 @@ struct GaussModel {
     // them by lane broadcast.  f(r, x, mu, J) is called for every pixel of rows
     // y0+r < R.
     template <int R, int RL, int G, class F>
-    PP_DEV void for_rows(const float (&t)[K], int y0, F&& f) const {
+    PP_DEV void for_rows(const float (&t)[KT], int y0, F&& f) const {
         AxisShape SX, SY;
         shapes(t, SX, SY);
         AxisTerms col[R];
@@ -283,7 +342,7 @@ struct GaussModel {
 #pragma unroll
                 for (int x = 0; x < R; x++) {
                     float mu, J[K];
-                    pixel(col[x], row, t, mu, J);
+                    pixel(col[x], row, (float)x, (float)y, (float)R * 0.5f, t, mu, J);
                     f(r, x, mu, J);
                 }
             }
@@ -297,6 +356,8 @@ struct GaussModel {
 // Grid, clamps, stop limits and initial values shared by both spline forms.
 struct SplineCommon {
     static constexpr int K = 5;
+    static constexpr int NC = 0, KT = K;  // no per-spot constants
+    static constexpr bool kCrlbAbs = false;
     static constexpr int iI = 3, iBg = 4;
     int Nz, Ny, Nx;  // cells per axis
     float zmin, zmax;
@@ -393,7 +454,7 @@ struct SplineModel : SplineCommon {
         mu = I * psf + t[4];
     }
 
-    PP_DEV void eval_pixel(const float (&t)[K], int x, int y, float& mu, float (&J)[K]) const {
+    PP_DEV void eval_pixel(const float (&t)[K], int x, int y, int, float& mu, float (&J)[K]) const {
         Frame F;
         frame(t, F);
         const float* c = row_ptr(F, y) + 64 * col_index(F, x);
@@ -511,7 +572,7 @@ struct SplineGridModel : SplineCommon {
     }
 
     // Single-pixel evaluation straight from the grid (expected value / derivatives).
-    PP_DEV void eval_pixel(const float (&t)[K], int x, int y, float& mu, float (&J)[K]) const {
+    PP_DEV void eval_pixel(const float (&t)[K], int x, int y, int, float& mu, float (&J)[K]) const {
         Frame F;
         frame(t, F);
         const int i = cell_x(F, x), j = cell_y(F, y);
@@ -615,6 +676,69 @@ struct SplineGridModel : SplineCommon {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Intensity and background of a spot at a known position and width
+// (IntensityBgModel, GaussianPSFModels.h:557-623, fitted by
+// Gauss2D_EstimateIntensityBg, GaussianPSF.cu:216-240): mu = I * DX * DY + bg with
+// the pixel-integrated Gaussian of width sigma at (x, y), the three per-spot
+// constants.  Start {I, bg} = {1, 0}; stop limits {1e-2, 1e-4}; I >= 1e-5,
+// bg >= 1e-8; the CRLB takes |diag(FI^-1)| (ComputeCRLB, Estimation.h:318-333).
+struct IBgModel {
+    static constexpr int K = 2, NC = 3, KT = 5;
+    static constexpr int iI = 0, iBg = 1;
+    static constexpr bool kCrlbAbs = true;
+
+    template <int R, int G>
+    using Layout = RowLayout<R, G>;
+    static constexpr int lds_floats(int) { return 0; }
+
+    PP_DEV static float stop_limit(int k) { return k == 0 ? 1e-2f : 1e-4f; }
+
+    PP_DEV void check_limits(float (&t)[KT], int) const {
+        if (t[0] < 1e-5f) t[0] = 1e-5f;
+        if (t[1] < 1e-8f) t[1] = 1e-8f;
+    }
+
+    PP_DEV void initial(float, float, float, float (&t)[KT]) const {
+        t[0] = 1.0f;
+        t[1] = 0.0f;
+    }
+
+    PP_DEV static float delta(float px, float pos, float c2) {
+        return 0.5f * erff(((px - pos) + 0.5f) * c2) - 0.5f * erff(((px - pos) - 0.5f) * c2);
+    }
+
+    PP_DEV void eval_pixel(const float (&t)[KT], int x, int y, int, float& mu, float (&J)[K]) const {
+        const float c2 = 1.0f / (sqrtf(2.0f) * t[4]);
+        const float p = delta((float)x, t[2], c2) * delta((float)y, t[3], c2);
+        mu = t[0] * p + t[1];
+        J[0] = p;
+        J[1] = 1.0f;
+    }
+
+    template <int R, int G, class Fn>
+    PP_DEV void for_pixels(const float (&t)[KT], int g, float*, Fn&& fn) const {
+        constexpr int RL = RowLayout<R, G>::RL;
+        const float c2 = 1.0f / (sqrtf(2.0f) * t[4]);
+        float dx[R];
+#pragma unroll
+        for (int x = 0; x < R; x++) dx[x] = delta((float)x, t[2], c2);
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            const int y = g * RL + r;
+            if (y < R) {
+                const float dy = delta((float)y, t[3], c2);
+#pragma unroll
+                for (int x = 0; x < R; x++) {
+                    const float p = dx[x] * dy;
+                    const float J[K] = {p, 1.0f};
+                    fn(r * R + x, t[0] * p + t[1], J);
+                }
+            }
+        }
     }
 };
 

@@ -12,6 +12,7 @@
 //   - GetResults copies whole finished batches, newest first, while they fit in the
 //     space left, and returns the number of spots copied (:117-158);
 //   - IsIdle ignores the batch still being filled; Flush queues it (:175-205).
+// Per-spot constants (models that have them) travel with the samples.
 // Differences: batches are fitted by this library's fused kernels (CRLB from the
 // fit's own second kernel), diagnostics are empty (numDiag = 0, as the reference's
 // GPU path), and maxQueueLen < 1 is treated as 1 (the reference never returns).
@@ -56,10 +57,12 @@ struct EstimationQueue {
         float* crlb = nullptr;     // pinned [batch, K]
         float* chisq = nullptr;    // pinned [batch]
         int* iterations = nullptr; // pinned [batch]
+        float* constants = nullptr;  // pinned [batch, NC]
         std::vector<int> roipos;   // [batch, 2], returned with the results
         int numspots = 0;
 
-        Batch(int n, int S, int K) : ids(n), roipos((size_t)n * 2) {
+        Batch(int n, int S, int K, int NC) : ids(n), roipos((size_t)n * 2) {
+            constants = pinned<float>((size_t)n * NC);
             samples = pinned<float>((size_t)n * S);
             params = pinned<float>((size_t)n * K);
             crlb = pinned<float>((size_t)n * K);
@@ -67,7 +70,8 @@ struct EstimationQueue {
             iterations = pinned<int>(n);
         }
         ~Batch() {
-            for (void* p : {(void*)samples, (void*)params, (void*)crlb, (void*)chisq, (void*)iterations})
+            for (void* p : {(void*)samples, (void*)params, (void*)crlb, (void*)chisq, (void*)iterations,
+                            (void*)constants})
                 if (p) (void)hipHostFree(p);
         }
     };
@@ -76,14 +80,14 @@ struct EstimationQueue {
         hipStream_t stream = nullptr;
         hipEvent_t done = nullptr;
         std::unique_ptr<Batch> current;
-        float *samples = nullptr, *params = nullptr, *crlb = nullptr, *chisq = nullptr;
+        float *samples = nullptr, *params = nullptr, *crlb = nullptr, *chisq = nullptr, *constants = nullptr;
         int* iterations = nullptr;
     };
 
     Estimator* est;
     Context* ctx;
     int device = 0;
-    int batchSize, maxQueueLen, K, S;
+    int batchSize, maxQueueLen, K, S, NC;
     bool keepSamples;
     std::atomic<bool> useInitial{false};
     std::vector<Lane> lanes;
@@ -99,7 +103,7 @@ struct EstimationQueue {
     std::string workerError;  // first launch failure, reported by later calls
 
     EstimationQueue(Estimator* e, int batch, int maxLen, bool keep, int streams, Context* c)
-        : est(e), ctx(c), batchSize(batch), maxQueueLen(std::max(1, maxLen)), K(e->K()), S(e->S()),
+        : est(e), ctx(c), batchSize(batch), maxQueueLen(std::max(1, maxLen)), K(e->K()), S(e->S()), NC(e->NC()),
           keepSamples(keep) {
         if (streams < 0) streams = 3;  // EstimationQueue.cpp:16-17
         if (streams == 0) streams = 1;
@@ -118,8 +122,9 @@ struct EstimationQueue {
             ln.crlb = device_alloc<float>((size_t)batchSize * K);
             ln.chisq = device_alloc<float>(batchSize);
             ln.iterations = device_alloc<int>(batchSize);
+            ln.constants = device_alloc<float>((size_t)batchSize * NC);
         }
-        next = std::make_unique<Batch>(batchSize, S, K);
+        next = std::make_unique<Batch>(batchSize, S, K, NC);
         worker = std::thread([this]() { run(); });
     }
 
@@ -132,7 +137,8 @@ struct EstimationQueue {
         if (worker.joinable()) worker.join();
         for (Lane& ln : lanes) {
             if (ln.stream) (void)hipStreamSynchronize(ln.stream);
-            for (void* p : {(void*)ln.samples, (void*)ln.params, (void*)ln.crlb, (void*)ln.chisq, (void*)ln.iterations})
+            for (void* p : {(void*)ln.samples, (void*)ln.params, (void*)ln.crlb, (void*)ln.chisq, (void*)ln.iterations,
+                            (void*)ln.constants})
                 if (p) (void)hipFree(p);
             if (ln.done) (void)hipEventDestroy(ln.done);
             if (ln.stream) (void)hipStreamDestroy(ln.stream);
@@ -143,13 +149,18 @@ struct EstimationQueue {
         const int n = b->numspots;
         hip_check(hipMemcpyAsync(ln.samples, b->samples, sizeof(float) * n * S, hipMemcpyHostToDevice, ln.stream),
                   "queue: samples to device");
+        if (NC > 0)
+            hip_check(hipMemcpyAsync(ln.constants, b->constants, sizeof(float) * n * NC, hipMemcpyHostToDevice,
+                                     ln.stream),
+                      "queue: constants to device");
         const bool init = useInitial;
         if (init)
             hip_check(hipMemcpyAsync(ln.params, b->params, sizeof(float) * n * K, hipMemcpyHostToDevice, ln.stream),
                       "queue: initial values to device");
         {
             std::lock_guard<std::mutex> l(est->mtx);
-            run_fit(est, n, ln.samples, init ? ln.params : nullptr, ln.params, ln.iterations, nullptr, 0, ln.crlb,
+            run_fit(est, n, ln.samples, NC > 0 ? ln.constants : nullptr, init ? ln.params : nullptr, ln.params,
+                    ln.iterations, nullptr, 0, ln.crlb,
                     ln.chisq, ln.stream, false);
         }
         hip_check(hipMemcpyAsync(b->params, ln.params, sizeof(float) * n * K, hipMemcpyDeviceToHost, ln.stream),
@@ -250,11 +261,14 @@ struct EstimationQueue {
                 recycle.pop_front();
             }
         }
-        if (!next) next = std::make_unique<Batch>(batchSize, S, K);
+        if (!next) next = std::make_unique<Batch>(batchSize, S, K, NC);
     }
 
-    void schedule(int count, const int* ids, const float* samples, const float* initial, const int* roipos) {
+    void schedule(int count, const int* ids, const float* samples, const float* initial, const float* constants,
+                  const int* roipos) {
         check_worker();
+        if (NC > 0 && !constants)
+            throw std::runtime_error("photonpy_amd: this estimator needs " + std::to_string(NC) + " constants per spot");
         {
             std::unique_lock<std::mutex> l(todoMtx);
             todoCv.wait(l, [&] {
@@ -273,6 +287,7 @@ struct EstimationQueue {
             b.roipos[2 * j] = roipos ? roipos[2 * i] : 0;
             b.roipos[2 * j + 1] = roipos ? roipos[2 * i + 1] : 0;
             if (initial) std::memcpy(b.params + (size_t)j * K, initial + (size_t)i * K, sizeof(float) * K);
+            if (NC > 0) std::memcpy(b.constants + (size_t)j * NC, constants + (size_t)i * NC, sizeof(float) * NC);
             if (b.numspots == batchSize) flush_locked();
         }
     }
@@ -349,12 +364,11 @@ void EstimQueue_Delete(EstimationQueue* q) {
 
 void EstimQueue_Schedule(EstimationQueue* q, int numspots, const int* ids, const float* h_samples,
                          const float* h_initial, const float* h_constants, const int* h_roipos) {
-    (void)h_constants;  // none of the models has per-spot constants
     if (!q || numspots <= 0) return;
     guarded(
         [&]() -> int {
             if (!h_samples) throw std::runtime_error("photonpy_amd: NULL samples");
-            q->schedule(numspots, ids, h_samples, h_initial, h_roipos);
+            q->schedule(numspots, ids, h_samples, h_initial, h_constants, h_roipos);
             return 0;
         },
         0);

@@ -228,14 +228,25 @@ class Estimator:
             raise ValueError(f"expected shape {tuple(shape)}, got {tuple(t.shape)}")
         return t.data_ptr()
 
-    def estimate_device(self, samples, initial=None, crlb=True, iterations_out=True, trace_len=0, stream=None):
+    def _dev_constants(self, constants, n):
+        import torch
+        if self.numconst == 0:
+            return None
+        if constants is None:
+            raise ValueError(f"this estimator needs constants [{n}, {self.numconst}]")
+        return self._dev(constants, torch.float32, (n, self.numconst))
+
+    def estimate_device(self, samples, initial=None, crlb=True, iterations_out=True, trace_len=0, stream=None,
+                        constants=None):
         """Fit device-resident ROIs [N,H,W] (float32).  Returns a dict of tensors
-        params [N,K], iterations [N], and crlb [N,K] + chisq [N] when crlb=True."""
+        params [N,K], iterations [N], and crlb [N,K] + chisq [N] when crlb=True.
+        constants [N, NumConstants] for the models that have per-spot constants."""
         import torch
         n = samples.shape[0]
         K = self.numparams
         dev = samples.device
         ps = self._dev(samples, torch.float32, (n, *self.sampleshape))
+        pc = self._dev_constants(constants, n)
         pi = None if initial is None else self._dev(initial, torch.float32, (n, K))
         out = {"params": torch.empty((n, K), dtype=torch.float32, device=dev)}
         out["iterations"] = torch.empty(n, dtype=torch.int32, device=dev) if iterations_out else None
@@ -246,7 +257,7 @@ class Estimator:
         if tr is not None:
             out["trace"] = tr
         rc = self._lib.Estimator_EstimateDevice(
-            self.inst, n, ps, pi, out["params"].data_ptr(),
+            self.inst, n, ps, pc, pi, out["params"].data_ptr(),
             out["iterations"].data_ptr() if out["iterations"] is not None else None,
             tr.data_ptr() if tr is not None else None, trace_len,
             out["crlb"].data_ptr() if crlb else None, out["chisq"].data_ptr() if crlb else None,
@@ -256,37 +267,41 @@ class Estimator:
             raise RuntimeError(f"Estimator_EstimateDevice failed ({rc})")
         return out
 
-    def expected_value_device(self, params, stream=None):
+    def expected_value_device(self, params, stream=None, constants=None):
         import torch
         n = params.shape[0]
         pp = self._dev(params, torch.float32, (n, self.numparams))
+        pc = self._dev_constants(constants, n)
         ev = torch.empty((n, *self.sampleshape), dtype=torch.float32, device=params.device)
-        rc = self._lib.Estimator_ExpectedValueDevice(self.inst, n, pp, ev.data_ptr(), self._stream(stream))
+        rc = self._lib.Estimator_ExpectedValueDevice(self.inst, n, pp, pc, ev.data_ptr(), self._stream(stream))
         if rc != 0:
             self._check()
             raise RuntimeError(f"Estimator_ExpectedValueDevice failed ({rc})")
         return ev
 
-    def derivatives_device(self, params, stream=None):
+    def derivatives_device(self, params, stream=None, constants=None):
         import torch
         n = params.shape[0]
         pp = self._dev(params, torch.float32, (n, self.numparams))
+        pc = self._dev_constants(constants, n)
         ev = torch.empty((n, *self.sampleshape), dtype=torch.float32, device=params.device)
         dv = torch.empty((n, self.numparams, *self.sampleshape), dtype=torch.float32, device=params.device)
-        rc = self._lib.Estimator_DerivativesDevice(self.inst, n, pp, dv.data_ptr(), ev.data_ptr(), self._stream(stream))
+        rc = self._lib.Estimator_DerivativesDevice(self.inst, n, pp, pc, dv.data_ptr(), ev.data_ptr(),
+                                                   self._stream(stream))
         if rc != 0:
             self._check()
             raise RuntimeError(f"Estimator_DerivativesDevice failed ({rc})")
         return dv, ev
 
-    def crlb_device(self, params, samples=None, stream=None):
+    def crlb_device(self, params, samples=None, stream=None, constants=None):
         import torch
         n = params.shape[0]
         pp = self._dev(params, torch.float32, (n, self.numparams))
+        pc = self._dev_constants(constants, n)
         ps = None if samples is None else self._dev(samples, torch.float32, (n, *self.sampleshape))
         crlb = torch.empty((n, self.numparams), dtype=torch.float32, device=params.device)
         chisq = torch.zeros(n, dtype=torch.float32, device=params.device)
-        rc = self._lib.Estimator_ChiSquareAndCRLBDevice(self.inst, n, pp, ps, crlb.data_ptr(), chisq.data_ptr(),
+        rc = self._lib.Estimator_ChiSquareAndCRLBDevice(self.inst, n, pp, pc, ps, crlb.data_ptr(), chisq.data_ptr(),
                                                         self._stream(stream))
         if rc != 0:
             self._check()

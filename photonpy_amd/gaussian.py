@@ -88,15 +88,66 @@ class GaussianPSFMethods:
         return Estimator(self.ctx, inst, calib)
 
     def CreatePSF_XYIBg(self, roisize, sigma, cuda=True) -> Estimator:
-        """Parameters [x,y,I,bg].  sigma may be a scalar or (sigma_x, sigma_y)."""
-        if sigma is None:
-            raise NotImplementedError("photonpy_amd: per-spot constant sigma (XYIBgConstSigma) is not implemented")
-        if np.isscalar(sigma):
-            sigma_x, sigma_y = sigma, sigma
+        """Parameters [x,y,I,bg].  sigma may be a scalar or (sigma_x, sigma_y); None gives
+        the model whose sigma x, y are per-spot constants (gaussian.py:175-191)."""
+        if sigma is not None:
+            if np.isscalar(sigma):
+                sigma_x, sigma_y = sigma, sigma
+            else:
+                sigma_x, sigma_y = sigma
+            inst = self._lib.Gauss2D_CreatePSF_XYIBg(roisize, sigma_x, sigma_y, int(cuda), self.ctx.inst)
         else:
-            sigma_x, sigma_y = sigma
-        inst = self._lib.Gauss2D_CreatePSF_XYIBg(roisize, sigma_x, sigma_y, int(cuda), self.ctx.inst)
+            inst = self._lib.Gauss2D_CreatePSF_XYIBgConstSigma(roisize, int(cuda), self.ctx.inst)
         return self._make(inst, sigma)
+
+    def CreatePSF_XYITiltedBg(self, roisize, cuda=True) -> Estimator:
+        """Parameters [x,y,I,bg,bgx,bgy], mu = bg + bgx (x - roisize/2) + bgy (y - roisize/2)
+        + I Gaussian(x, y); sigma x, y are per-spot constants (gaussian.py:193-200)."""
+        inst = self._lib.Gauss2D_CreatePSF_XYITiltedBg(roisize, int(cuda), self.ctx.inst)
+        return self._make(inst, None)
+
+    def CreatePSF_XYIBgSigmaXY(self, roisize, initialSigma, cuda=True) -> Estimator:
+        """Parameters [x,y,I,bg,sx,sy] (gaussian.py:212-221)."""
+        if np.isscalar(initialSigma):
+            initialSigma = [initialSigma, initialSigma]
+        inst = self._lib.Gauss2D_CreatePSF_XYIBgSigmaXY(roisize, initialSigma[0], initialSigma[1], int(cuda),
+                                                        self.ctx.inst)
+        return self._make(inst, initialSigma)
+
+    def CreatePSF_XYITiltedBgSigmaXY(self, roisize, initialSigma, cuda=True) -> Estimator:
+        """Parameters [x,y,I,bg,bgx,bgy,sx,sy] (gaussian.py:223-232)."""
+        if np.isscalar(initialSigma):
+            initialSigma = [initialSigma, initialSigma]
+        inst = self._lib.Gauss2D_CreatePSF_XYITiltedBgSigmaXY(roisize, initialSigma[0], initialSigma[1], int(cuda),
+                                                              self.ctx.inst)
+        return self._make(inst, initialSigma)
+
+    def EstimateIBg(self, images, sigma, xy, roipos=None, useCuda=False):
+        """Intensity and background of spots at known positions (gaussian.py:154-172):
+        returns (IBg [n,2], crlb [n,2]).  sigma is broadcast to [n,2] as the reference
+        does, and the native side reads its i-th element for spot i."""
+        images = np.ascontiguousarray(images, dtype=np.float32)
+        xy = np.ascontiguousarray(xy, dtype=np.float32)
+        if images.ndim != 3:
+            raise ValueError("images must be [n, imgw, imgw]")
+        numspots = len(images)
+        if xy.shape != (numspots, 2):
+            raise ValueError(f"xy must have shape {(numspots, 2)}")
+        imgw = images.shape[1]
+        result = np.zeros((numspots, 2), dtype=np.float32)
+        crlb = np.zeros((numspots, 2), dtype=np.float32)
+        sigma_ = np.zeros((numspots, 2), dtype=np.float32)
+        sigma_[:] = sigma
+        if roipos is None:
+            roipos = np.zeros((numspots, 2))
+        roipos = np.ascontiguousarray(roipos, dtype=np.int32)
+        if roipos.shape != (numspots, 2):
+            raise ValueError(f"roipos must have shape {(numspots, 2)}")
+        if numspots > 0:
+            self._lib.Gauss2D_EstimateIntensityBg(images, result, crlb, numspots, xy, roipos, sigma_, imgw, 100,
+                                                  bool(useCuda))
+            self.ctx.smlm.check()
+        return result, crlb
 
     def CreatePSF_XYIBgSigma(self, roisize, initialSigma, cuda=True) -> Estimator:
         """Parameters [x,y,I,bg,sigma]."""

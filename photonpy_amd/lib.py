@@ -96,6 +96,12 @@ def _bind(lib):
         "Gauss2D_CreatePSF_XYZIBgQ": ([I, V, I, V], V),
         "CSpline_CreatePSF": ([I, V, NullableFloatArrayType, I, B, V], V),
         "CSpline_Compute": ([IntArray, FloatArray, FloatArray, I], None),
+        "Gauss2D_CreatePSF_XYIBgConstSigma": ([I, B, V], V),
+        "Gauss2D_CreatePSF_XYITiltedBg": ([I, B, V], V),
+        "Gauss2D_CreatePSF_XYIBgSigmaXY": ([I, ct.c_float, ct.c_float, B, V], V),
+        "Gauss2D_CreatePSF_XYITiltedBgSigmaXY": ([I, ct.c_float, ct.c_float, B, V], V),
+        "Gauss2D_EstimateIntensityBg": ([FloatArray, FloatArray, FloatArray, I, FloatArray, IntArray, FloatArray, I, I,
+                                         B], None),
         "CSpline_CreatePSF_FromFile": ([I, ct.c_char_p, I, B, V], V),
         "CSpline_Evaluate": ([IntArray, FloatArray, FloatArray, I, FloatArray], None),
         "Estimator_Delete": ([V], None),
@@ -118,10 +124,10 @@ def _bind(lib):
         "Estimator_ComputeDerivatives": ([V, I, FloatArray, FloatArray, IntArray, FloatArray, FloatArray], None),
         "Estimator_ChiSquareAndCRLB": ([V, FloatArray, NullableFloatArrayType, FloatArray, IntArray, I,
                                         FloatArray, FloatArray], None),
-        "Estimator_EstimateDevice": ([V, I, V, V, V, V, V, I, V, V, V], ct.c_int),
-        "Estimator_ExpectedValueDevice": ([V, I, V, V, V], ct.c_int),
-        "Estimator_DerivativesDevice": ([V, I, V, V, V, V], ct.c_int),
-        "Estimator_ChiSquareAndCRLBDevice": ([V, I, V, V, V, V, V], ct.c_int),
+        "Estimator_EstimateDevice": ([V, I, V, V, V, V, V, V, I, V, V, V], ct.c_int),
+        "Estimator_ExpectedValueDevice": ([V, I, V, V, V, V], ct.c_int),
+        "Estimator_DerivativesDevice": ([V, I, V, V, V, V, V], ct.c_int),
+        "Estimator_ChiSquareAndCRLBDevice": ([V, I, V, V, V, V, V, V], ct.c_int),
         "Estimator_NLLGradDevice": ([V, I, V, V, V, V, V, V, V, V], ct.c_int),
         "Estimator_NumShared": ([V], ct.c_int),
         "EstimQueue_Create": ([V, I, I, ct.c_bool, I, V], V),

@@ -99,3 +99,54 @@ def spline_zstack(nz=64, ny=29, nx=29, dz=13.3, z0=-418.95, emitter=14.0):
     th[:, 4] = 0.0
     assert ny == nx
     return mu_astig(nx, th, ASTIG_X, ASTIG_Y, ASTIG_B, ASTIG_B), z
+
+
+def mu_tilted(R, theta, sx, sy, tilt=True):
+    """mu = bg + bgx (x - R/2) + bgy (y - R/2) + I DeltaX DeltaY for theta [x, y, I, bg, bgx, bgy, ...];
+    with tilt=False the tilt terms are left out."""
+    th = np.asarray(theta, np.float64)
+    mu = gauss_mu(R, th[:, 0], th[:, 1], th[:, 2], th[:, 3], sx, sy)
+    if tilt:
+        c = np.arange(R, dtype=np.float64) - R / 2
+        mu = mu + th[:, 4, None, None] * c[None, None, :] + th[:, 5, None, None] * c[None, :, None]
+    return mu
+
+
+def synth_variant(kind, R, n, rng):
+    """Seeded true parameters and per-spot constants of the model variants:
+    const_sigma, tilted_bg ([sx, sy] constants), sigma_xy, tilted_bg_sigma_xy, intensity_bg
+    ([x, y, sigma] constants)."""
+    c = (R - 1) / 2
+    x = c + rng.uniform(-1.5, 1.5, n)
+    y = c + rng.uniform(-1.5, 1.5, n)
+    I = rng.uniform(500, 5000, n)
+    bg = rng.uniform(2, 20, n)
+    sx, sy = rng.uniform(1.2, 2.0, n), rng.uniform(1.2, 2.0, n)
+    if kind == "const_sigma":
+        return np.stack([x, y, I, bg], 1), np.stack([sx, sy], 1)
+    if kind == "tilted_bg":
+        # tilts small enough that mu stays positive over the ROI
+        bgx, bgy = rng.uniform(-0.2, 0.2, n), rng.uniform(-0.2, 0.2, n)
+        return np.stack([x, y, I, bg, bgx, bgy], 1), np.stack([sx, sy], 1)
+    if kind == "sigma_xy":
+        return np.stack([x, y, I, bg, sx, sy], 1), None
+    if kind == "tilted_bg_sigma_xy":
+        return np.stack([x, y, I, bg, np.zeros(n), np.zeros(n), sx, sy], 1), None
+    if kind == "intensity_bg":
+        return np.stack([I, bg], 1), np.stack([x, y, sx], 1)
+    raise ValueError(kind)
+
+
+def mu_variant(kind, R, theta, cst=None):
+    th = np.asarray(theta, np.float64)
+    if kind == "const_sigma":
+        return gauss_mu(R, th[:, 0], th[:, 1], th[:, 2], th[:, 3], cst[:, 0], cst[:, 1])
+    if kind == "tilted_bg":
+        return mu_tilted(R, th, cst[:, 0], cst[:, 1])
+    if kind == "sigma_xy":
+        return gauss_mu(R, th[:, 0], th[:, 1], th[:, 2], th[:, 3], th[:, 4], th[:, 5])
+    if kind == "tilted_bg_sigma_xy":
+        return mu_tilted(R, th, th[:, 6], th[:, 7], tilt=False)
+    if kind == "intensity_bg":
+        return gauss_mu(R, cst[:, 0], cst[:, 1], th[:, 0], th[:, 1], cst[:, 2], cst[:, 2])
+    raise ValueError(kind)

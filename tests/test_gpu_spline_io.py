@@ -47,6 +47,9 @@ def test_missing_file_raises(ctx):
     lib = ctx.smlm.lib
     assert not lib.CSpline_CreatePSF_FromFile(20, b"/nonexistent.zstack", 0, True, ctx.inst)
     assert "Can't open" in lib.SMLM_GetLastError().decode()
+    # the error stays set until cleared, as documented for SMLM_GetLastError
+    lib.SMLM_ClearError()
+    assert lib.SMLM_GetLastError() == b""
 
 
 def test_evaluate_reproduces_samples_and_matches_numpy(ctx):
