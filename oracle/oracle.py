@@ -20,6 +20,8 @@ mathematics in FP64.  Model descriptors:
   constants [sigma_x, sigma_y] per spot
 * ``gauss2d_sigma_xy(R, isx, isy)``   Gauss2D_Model_XYIBgSigmaXY (GaussianPSFModels.h:388-467)
 * ``gauss2d_tilted_bg_sigma_xy(R, isx, isy)``  Gauss2D_Model_XYITiltedBgSigmaXY (:471-553)
+* ``spline(R, coefs, zmin, zmax, mode)``  mode 1: CSpline_Model_XYZITiltedBg (CubicSplinePSFModels.h:157-298),
+  mode 2: CSpline_Model_XYZIBgImage (:300-443), constants = an R x R background image per spot
 * ``intensity_bg(R)``                 IntensityBgModel (GaussianPSFModels.h:557-623), constants
   [x, y, sigma] per spot; CRLB from |diag| (Estimation.h:318-333)
 """
@@ -69,7 +71,7 @@ def lib():
             getattr(_lib, f"or_crlb_{sfx}").argtypes = [M, ct.c_int, R, R, R, R, R, ct.c_int]
             getattr(_lib, f"or_loglik_{sfx}").argtypes = [M, ct.c_int, R, R, R, R, ct.c_int]
         _lib.or_num_params.argtypes = [ct.c_int]
-        _lib.or_num_consts.argtypes = [ct.c_int]
+        _lib.or_num_consts.argtypes = [ct.c_int, ct.c_int]
         _lib.or_spline_compute.argtypes = [I, F, F, ct.c_int]
     return _lib
 
@@ -98,7 +100,7 @@ class Model:
         self.kind = kind
         self.roisize = roisize
         self.K = lib().or_num_params(kind)
-        self.NC = lib().or_num_consts(kind)
+        self.NC = lib().or_num_consts(kind, roisize)
 
     # -- helpers -----------------------------------------------------------
     def _arr(self, a, dt, shape=None):
@@ -195,10 +197,11 @@ def astig(roisize, x, y, zrange, B=(0.0, 0.0)):
     return Model(2, roisize, tuple(x[:4]) + tuple(y[:4]) + (zrange[0], zrange[1], B[0], B[1]))
 
 
-def spline(roisize, coefs, zmin, zmax):
+def spline(roisize, coefs, zmin, zmax, mode=0):
+    """mode: 0 flat background, 1 tilted background, 2 background image (CSpline_CreatePSF modes)."""
     coefs = np.ascontiguousarray(coefs, dtype=np.float32)
     assert coefs.ndim == 4 and coefs.shape[3] == 64
-    return Model(3, roisize, (zmin, zmax), shape=coefs.shape[:3], coefs=coefs)
+    return Model({0: 3, 1: 9, 2: 10}[mode], roisize, (zmin, zmax), shape=coefs.shape[:3], coefs=coefs)
 
 
 def gauss2d_const_sigma(roisize):

@@ -32,6 +32,8 @@
 //   IntensityBgModel + Gauss2D_EstimateIntensityBg  GaussianPSFModels.h:557-623, GaussianPSF.cu:216-240
 //   ComputeFisherMatrix / ComputeCRLB (|diag|)      Estimation.h:22-60, 318-333
 //   CSpline_Model_XYZIBg       cpp/SMLMLib/Estimators/CSpline/CubicSplinePSFModels.h:14-154
+//   CSpline_Model_XYZITiltedBg CubicSplinePSFModels.h:157-298 (x,y,z,I,bg,bgx,bgy)
+//   CSpline_Model_XYZIBgImage  CubicSplinePSFModels.h:300-443 (background image per spot, R*R constants)
 //   CSpline_Compute            cpp/SMLMLib/Estimators/CSpline/CubicSplinePSF.cu:151-204
 //
 // The reference's own per-spot std::async scheduler (ThreadUtils.h:22-32) is
@@ -87,10 +89,12 @@ template <> constexpr float kPiT<float> = 3.141592653589793f;
 //       6 = free sigma x and y (x,y,I,bg,sx,sy)
 //       7 = tilted background + free sigma x and y (x,y,I,bg,bgx,bgy,sx,sy)
 //       8 = intensity / background at per-spot constants x, y, sigma (I,bg)
+//       9 = cubic spline 3D, tilted background (x,y,z,I,bg,bgx,bgy)
+//      10 = cubic spline 3D, background image per spot (x,y,z,I,bg; R*R constants)
 struct OrModel {
     int kind;
     int roisize;
-    double p[16]; // 0: sx,sy | 1: initialSigma | 2: x[4],y[4],minz,maxz,Bx,By | 3: zmin,zmax | 6,7: isx,isy
+    double p[16]; // 0: sx,sy | 1: initialSigma | 2: x[4],y[4],minz,maxz,Bx,By | 3,9,10: zmin,zmax | 6,7: isx,isy
     int shape[3]; // spline (Nz,Ny,Nx)
     const float* coefs;
 };
@@ -237,15 +241,16 @@ struct Astig3D {
     }
 };
 
-template <typename T>
+// BG: 0 flat background, 1 tilted, 2 background image (per-spot constants).
+template <typename T, int BG = 0>
 struct Spline3D {
-    enum { K = 5, NC = 0, CrlbAbs = 0 };
+    enum { K = BG == 1 ? 7 : 5, NC = 0, CrlbAbs = 0 };
     const T* cst = nullptr;
     int R;
     int Nz, Ny, Nx;
     T zmin, zmax;
     const float* coefs;
-    static T StopLimit(int k) { const T s[5] = {T(1e-4f), T(1e-4f), T(1e-7f), T(1e-4f), T(1e-4f)}; return s[k]; }
+    static T StopLimit(int k) { return k == 2 ? T(1e-7f) : T(1e-4f); }
     void CheckLimits(T* t) const {
         t[0] = clampf_<T>(t[0], T(1.5f), T(R) - T(1.0f) - T(1.5f));
         t[1] = clampf_<T>(t[1], T(1.5f), T(R) - T(1.0f) - T(1.5f));
@@ -253,12 +258,23 @@ struct Spline3D {
         t[3] = std::fmax(t[3], T(10.f));
         t[4] = std::fmax(t[4], T(0.0f));
     }
-    void Initial(const T com[3], T* t) const { t[0] = com[0]; t[1] = com[1]; t[2] = 0; t[3] = com[2]; t[4] = 0; }
+    void Initial(const T com[3], T* t) const {
+        t[0] = com[0]; t[1] = com[1]; t[2] = 0; t[3] = com[2]; t[4] = 0;
+        for (int k = 5; k < K; k++) t[k] = 0;
+    }
     template <typename CB> void Eval(const T* th, CB cb) const {
         const T du = (zmax - zmin) / T(Nz);
         T tx = th[0], ty = th[1], tz = th[2], tI = th[3], tbg = th[4];
-        tx -= T(Nx) * T(0.5f);
-        ty -= T(Ny) * T(0.5f);
+        if (BG == 2) {  // CubicSplinePSFModels.h:374-377
+            tx = tx - T(R) / T(2.0f) + (T(R) - T(Nx) + T(1)) / T(2.0f);
+            ty = ty - T(R) / T(2.0f) + (T(R) - T(Ny) + T(1)) / T(2.0f);
+            tx += T(0.5f);
+            ty += T(0.5f);
+        } else {
+            tx -= T(Nx) * T(0.5f);
+            ty -= T(Ny) * T(0.5f);
+        }
+        const T half = T(R) * T(0.5f);
         T bf[64] = {0}, bx[64] = {0}, by[64] = {0}, bz[64] = {0};
         T spx = -tx; spx -= T(int(spx));
         T spy = -ty; spy -= T(int(spy));
@@ -301,9 +317,20 @@ struct Spline3D {
                 gy *= -tI;
                 gz *= tI / du;
                 if (psf < T(0.0f)) psf = T(0.0f);
-                T mu = tI * psf + tbg;
-                const T J[5] = {gx, gy, gz, psf, T(1)};
-                cb(y * R + x, mu, J);
+                if (BG == 1) {
+                    T mu = tI * psf + tbg + th[5] * (T(x) - half) + th[6] * (T(y) - half);
+                    const T J[7] = {gx, gy, gz, psf, T(1), T(x) - half, T(y) - half};
+                    cb(y * R + x, mu, J);
+                } else if (BG == 2) {
+                    const T b = cst[y * R + x];
+                    T mu = tI * psf + tbg * b;
+                    const T J[5] = {gx, gy, gz, psf, b};
+                    cb(y * R + x, mu, J);
+                } else {
+                    T mu = tI * psf + tbg;
+                    const T J[5] = {gx, gy, gz, psf, T(1)};
+                    cb(y * R + x, mu, J);
+                }
             }
         }
     }
@@ -657,6 +684,14 @@ T loglik(const M& m, const T* theta, const T* smp) {
     return LL;
 }
 
+template <typename T, int BG, typename F>
+void spline_case(const OrModel* d, F&& fn) {
+    Spline3D<T, BG> m{};
+    m.R = d->roisize; m.Nz = d->shape[0]; m.Ny = d->shape[1]; m.Nx = d->shape[2];
+    m.zmin = d->p[0]; m.zmax = d->p[1]; m.coefs = d->coefs;
+    fn(m);
+}
+
 template <typename T, typename F>
 void dispatch(const OrModel* d, F fn) {
     switch (d->kind) {
@@ -670,13 +705,9 @@ void dispatch(const OrModel* d, F fn) {
         fn(m);
         break;
     }
-    case 3: {
-        Spline3D<T> m{};
-        m.R = d->roisize; m.Nz = d->shape[0]; m.Ny = d->shape[1]; m.Nx = d->shape[2];
-        m.zmin = d->p[0]; m.zmax = d->p[1]; m.coefs = d->coefs;
-        fn(m);
-        break;
-    }
+    case 3: spline_case<T, 0>(d, fn); break;
+    case 9: spline_case<T, 1>(d, fn); break;
+    case 10: spline_case<T, 2>(d, fn); break;
     case 4: { Gauss2DConstSigma<T> m{}; m.R = d->roisize; fn(m); break; }
     case 5: { Gauss2DTiltedBg<T> m{}; m.R = d->roisize; fn(m); break; }
     case 6: { Gauss2DSigmaXY<T, false> m{}; m.R = d->roisize; m.isx = d->p[0]; m.isy = d->p[1]; fn(m); break; }
@@ -692,17 +723,18 @@ int num_params(int kind) {
     case 5: case 6: return 6;
     case 7: return 8;
     case 8: return 2;
+    case 9: return 7;
     default: return 5;
     }
 }
 
-int num_consts(int kind) { return kind == 4 || kind == 5 ? 2 : (kind == 8 ? 3 : 0); }
+int num_consts(int kind, int R) { return kind == 4 || kind == 5 ? 2 : kind == 8 ? 3 : kind == 10 ? R * R : 0; }
 
-// The model for spot s: a copy that points at the spot's constants.
+// The model for spot s: a copy that points at the spot's nc constants.
 template <typename M, typename T>
-M for_spot(const M& m, const T* cst, int s) {
+M for_spot(const M& m, const T* cst, int s, int nc) {
     M c = m;
-    if (M::NC > 0) c.cst = cst + (size_t)s * M::NC;
+    if (nc > 0) c.cst = cst + (size_t)s * nc;
     return c;
 }
 
@@ -711,7 +743,7 @@ void expected_value(const OrModel* d, int n, const T* params, const T* cst, T* e
     const int S = d->roisize * d->roisize, K = num_params(d->kind);
     dispatch<T>(d, [&](auto& m) {
         pool_for(n, nth, [&](int s) {
-            for_spot(m, cst, s).Eval(params + (size_t)s * K, [&](int idx, T mu, const T*) { ev[(size_t)s * S + idx] = mu; });
+            for_spot(m, cst, s, num_consts(d->kind, d->roisize)).Eval(params + (size_t)s * K, [&](int idx, T mu, const T*) { ev[(size_t)s * S + idx] = mu; });
         });
     });
 }
@@ -721,7 +753,7 @@ void derivatives(const OrModel* d, int n, const T* params, const T* cst, T* deri
     const int S = d->roisize * d->roisize, K = num_params(d->kind);
     dispatch<T>(d, [&](auto& m) {
         pool_for(n, nth, [&](int s) {
-            for_spot(m, cst, s).Eval(params + (size_t)s * K, [&](int idx, T mu, const T* J) {
+            for_spot(m, cst, s, num_consts(d->kind, d->roisize)).Eval(params + (size_t)s * K, [&](int idx, T mu, const T* J) {
                 for (int k = 0; k < K; k++) deriv[((size_t)s * K + k) * S + idx] = J[k];
                 ev[(size_t)s * S + idx] = mu;
             });
@@ -735,7 +767,7 @@ void estimate(const OrModel* d, int n, const T* smp, const T* cst, const T* init
     const int R = d->roisize, S = R * R, K = num_params(d->kind);
     dispatch<T>(d, [&](auto& m0) {
         pool_for(n, nth, [&](int s) {
-            const auto m = for_spot(m0, cst, s);
+            const auto m = for_spot(m0, cst, s, num_consts(d->kind, d->roisize));
             const T* sp = smp + (size_t)s * S;
             T* th = params + (size_t)s * K;
             if (initial) {
@@ -758,7 +790,7 @@ void crlb(const OrModel* d, int n, const T* params, const T* cst, const T* smp, 
     dispatch<T>(d, [&](auto& m) {
         pool_for(n, nth, [&](int s) {
             T c;
-            crlb_chisq<T>(for_spot(m, cst, s), params + (size_t)s * K, smp ? smp + (size_t)s * S : nullptr,
+            crlb_chisq<T>(for_spot(m, cst, s, num_consts(d->kind, d->roisize)), params + (size_t)s * K, smp ? smp + (size_t)s * S : nullptr,
                           out + (size_t)s * K, &c);
             if (chisq) chisq[s] = c;
         });
@@ -770,7 +802,7 @@ void loglik_all(const OrModel* d, int n, const T* params, const T* cst, const T*
     const int S = d->roisize * d->roisize, K = num_params(d->kind);
     dispatch<T>(d, [&](auto& m) {
         pool_for(n, nth, [&](int s) {
-            ll[s] = loglik<T>(for_spot(m, cst, s), params + (size_t)s * K, smp + (size_t)s * S);
+            ll[s] = loglik<T>(for_spot(m, cst, s, num_consts(d->kind, d->roisize)), params + (size_t)s * K, smp + (size_t)s * S);
         });
     });
 }
@@ -780,7 +812,7 @@ void loglik_all(const OrModel* d, int n, const T* params, const T* cst, const T*
 extern "C" {
 
 int or_num_params(int kind) { return num_params(kind); }
-int or_num_consts(int kind) { return num_consts(kind); }
+int or_num_consts(int kind, int roisize) { return num_consts(kind, roisize); }
 
 // cst: [n, NC] per-spot constants of the models that have them, else NULL.
 void or_expected_value_f32(const OrModel* d, int n, const float* p, const float* cst, float* ev, int nth) { expected_value<float>(d, n, p, cst, ev, nth); }

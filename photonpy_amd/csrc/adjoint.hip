@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(256) spline_adjoint_kernel(SplineModel M, AdjA
 #pragma unroll
     for (int k = 0; k < K; k++) t[k] = a.params[(size_t)spot * K + k];
     SplineModel::Frame F;
-    M.frame(t, F);
+    M.frame(t, R, F);
     const float go = a.gout ? a.gout[spot] : 1.0f;
     float acc[K + 1];
 #pragma unroll
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(256) spline_adjoint_kernel(SplineModel M, AdjA
             const float* c = rp + 64 * i;
             float f, fx, fy, fz, mu, J[K];
             SplineModel::cell(c, F, f, fx, fy, fz);
-            SplineModel::finish(F, t, f, fx, fy, fz, mu, J);
+            SplineModel::finish(F, t, f, fx, fy, fz, x, y, R, mu, J);
             if (f < 0.0f) J[0] = J[1] = J[2] = 0.0f;  // mu does not depend on position where clamped
             const float w = nll_weight(mu, a.sample[(size_t)spot * S + p], acc[K]);
 #pragma unroll

@@ -46,7 +46,7 @@ static double bin_cost(const Estimator* e, int count) {
     case kKindFixed: c = 1.0; break;
     case kKindSigma: c = 1.3; break;
     case kKindAstig: c = 1.4; break;
-    default: c = e->desc.useGrid ? 4.0 : 40.0; break;
+    default: c = is_spline(e->desc.kind) ? (e->desc.useGrid ? 4.0 : 40.0) : 1.3; break;
     }
     return c * (double)count * e->roisize * e->roisize;
 }

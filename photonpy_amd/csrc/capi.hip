@@ -405,10 +405,8 @@ Estimator* CSpline_CreatePSF(int roisize, const CSpline_Calibration* calib, cons
         [&]() -> Estimator* {
             require_gpu(cuda);
             if (!calib) throw std::runtime_error("photonpy_amd: calibration is NULL");
-            if (mode != 0)
-                throw std::runtime_error(mode == 1 || mode == 2
-                                             ? "photonpy_amd: spline background modes other than flat (0) are not implemented"
-                                             : "Invalid model background mode");
+            // CSPLINE_FLATBG / TILTEDBG / BGIMAGE (CubicSplinePSF.cu:72-97)
+            if (mode < 0 || mode > 2) throw std::runtime_error("Invalid model background mode");
             if (roisize < 1) throw std::runtime_error("photonpy_amd: roisize must be positive");
             const float* src = coefs ? coefs : calib->coefs;
             if (!src) throw std::runtime_error("photonpy_amd: spline coefficients are NULL");
@@ -416,7 +414,7 @@ Estimator* CSpline_CreatePSF(int roisize, const CSpline_Calibration* calib, cons
             if (Nz < 1 || Ny < 1 || Nx < 1) throw std::runtime_error("photonpy_amd: invalid spline shape");
             auto* e = new Estimator();
             e->roisize = roisize;
-            e->desc.kind = kKindSpline;
+            e->desc.kind = mode == 1 ? kKindSplineTilted : mode == 2 ? kKindSplineImage : kKindSpline;
             e->desc.sm.Nz = Nz;
             e->desc.sm.Ny = Ny;
             e->desc.sm.Nx = Nx;
@@ -433,9 +431,13 @@ Estimator* CSpline_CreatePSF(int roisize, const CSpline_Calibration* calib, cons
             e->gridForm = spline_grid_from_coefs(Nz, Ny, Nx, e->hostCoefs.data(), e->hostGrid);
             if (!e->gridForm) e->hostGrid.clear();
             e->desc.useGrid = e->gridForm;
-            e->fmt = "x,y,z,I,bg";
+            e->fmt = mode == 1 ? "x,y,z,I,bg,bgx,bgy" : "x,y,z,I,bg";
             const float b = 1.5f, hi = (float)roisize - 1 - b;
             e->limits = {{b, hi}, {b, hi}, {calib->z_min, calib->z_max}, {10.0f, 1e9f}, {1e-6f, 1e9f}};
+            if (mode == 1) {
+                e->limits.push_back({-1e6f, 1e6f});
+                e->limits.push_back({-1e6f, 1e6f});
+            }
             attach(e, ctx);
             return e;
         },
@@ -571,7 +573,7 @@ int Estimator_SetRefill(Estimator* e, int refill) {
 }
 
 int Estimator_SetSplineMode(Estimator* e, int mode) {
-    if (!e || e->desc.kind != kKindSpline || mode < 0 || mode > 1) return 1;
+    if (!e || !is_spline(e->desc.kind) || mode < 0 || mode > 1) return 1;
     std::lock_guard<std::mutex> lk(e->mtx);
     e->splineMode = mode;
     e->desc.useGrid = e->gridForm && mode == 0;
@@ -579,7 +581,7 @@ int Estimator_SetSplineMode(Estimator* e, int mode) {
 }
 
 int Estimator_SplineForm(Estimator* e) {
-    if (!e || e->desc.kind != kKindSpline) return -1;
+    if (!e || !is_spline(e->desc.kind)) return -1;
     return e->desc.useGrid ? 1 : 0;
 }
 

@@ -96,7 +96,7 @@ struct Estimator {
     }
 
     int K() const { return num_params(desc.kind); }
-    int NC() const { return num_consts(desc.kind); }
+    int NC() const { return num_consts(desc.kind, roisize); }
 
     // Shared PSF parameters in the adjoint's layout
     std::vector<float> shared_values() const {
@@ -137,13 +137,13 @@ struct Estimator {
     }
 
     void ensure_device() {
-        if (desc.kind == kKindSpline && !devCoefs) {
+        if (is_spline(desc.kind) && !devCoefs) {
             const size_t bytes = hostCoefs.size() * sizeof(float);
             hip_check(hipMalloc(&devCoefs, bytes), "hipMalloc(spline coefficients)");
             hip_check(hipMemcpy(devCoefs, hostCoefs.data(), bytes, hipMemcpyHostToDevice), "upload spline coefficients");
             desc.sm.coefs = devCoefs;
         }
-        if (desc.kind == kKindSpline && desc.useGrid && !devGrid) {
+        if (is_spline(desc.kind) && desc.useGrid && !devGrid) {
             const size_t bytes = hostGrid.size() * sizeof(float);
             hip_check(hipMalloc(&devGrid, bytes), "hipMalloc(spline grid)");
             hip_check(hipMemcpy(devGrid, hostGrid.data(), bytes, hipMemcpyHostToDevice), "upload spline grid");

@@ -18,7 +18,11 @@ enum ModelKind {
     kKindSigmaXY = 6,
     kKindTiltedBgSigmaXY = 7,
     kKindIBg = 8,
+    kKindSplineTilted = 9,  // CSpline_Model_XYZITiltedBg (CSpline_CreatePSF mode 1)
+    kKindSplineImage = 10,  // CSpline_Model_XYZIBgImage (mode 2)
 };
+
+inline bool is_spline(int kind) { return kind == kKindSpline || kind == kKindSplineTilted || kind == kKindSplineImage; }
 
 struct ModelDesc {
     int kind;
@@ -35,16 +39,19 @@ inline int num_params(int kind) {
     case kKindSigmaXY: return GaussModel<kSigmaXY>::K;
     case kKindTiltedBgSigmaXY: return GaussModel<kTiltedBgSigmaXY>::K;
     case kKindIBg: return IBgModel::K;
+    case kKindSplineTilted: return SplineModelT<kSplineTilted>::K;
     default: return 5;
     }
 }
 
-// Per-spot constants of a model (the reference's NumConstants).
-inline int num_consts(int kind) {
+// Per-spot constants of a model on R x R ROIs (the reference's NumConstants):
+// the background-image spline takes an R x R image per spot.
+inline int num_consts(int kind, int R) {
     switch (kind) {
     case kKindConstSigma: return GaussModel<kConstSigma>::NC;
     case kKindTiltedBg: return GaussModel<kTiltedBg>::NC;
     case kKindIBg: return IBgModel::NC;
+    case kKindSplineImage: return R * R;
     default: return 0;
     }
 }
@@ -68,7 +75,7 @@ inline int default_group_grid(int R) {
 }
 
 inline int default_group_for(const ModelDesc& d, int R) {
-    return (d.kind == kKindSpline && d.useGrid) ? default_group_grid(R) : default_group(R);
+    return (is_spline(d.kind) && d.useGrid) ? default_group_grid(R) : default_group(R);
 }
 
 constexpr int kMinFitR = 3, kMaxFitR = 32;
@@ -89,6 +96,14 @@ hipError_t launch_fit_sigmaxy(const ModelDesc& d, int R, int G, const FitArgs& a
 hipError_t launch_fit_tiltedbgsigmaxy(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events,
                                       hipStream_t st);
 hipError_t launch_fit_ibg(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st);
+hipError_t launch_fit_splinetilted(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events,
+                                  hipStream_t st);
+hipError_t launch_fit_splinetiltedgrid(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events,
+                                      hipStream_t st);
+hipError_t launch_fit_splineimage(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events,
+                                  hipStream_t st);
+hipError_t launch_fit_splineimagegrid(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events,
+                                      hipStream_t st);
 
 hipError_t launch_ev_any(const ModelDesc& d, int R, int n, const float* params, const float* constants, float* ev,
                          float* deriv, hipStream_t st);
@@ -123,6 +138,12 @@ inline hipError_t launch_fit_any(const ModelDesc& d, int R, int G, const FitArgs
     case kKindSigmaXY: return launch_fit_sigmaxy(d, R, G, a, events, st);
     case kKindTiltedBgSigmaXY: return launch_fit_tiltedbgsigmaxy(d, R, G, a, events, st);
     case kKindIBg: return launch_fit_ibg(d, R, G, a, events, st);
+    case kKindSplineTilted:
+        return d.useGrid ? launch_fit_splinetiltedgrid(d, R, G, a, events, st)
+                         : launch_fit_splinetilted(d, R, G, a, events, st);
+    case kKindSplineImage:
+        return d.useGrid ? launch_fit_splineimagegrid(d, R, G, a, events, st)
+                         : launch_fit_splineimage(d, R, G, a, events, st);
     default: return hipErrorInvalidValue;
     }
 }

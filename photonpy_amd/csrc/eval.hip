@@ -35,6 +35,10 @@ static hipError_t with_model(const ModelDesc& d, F&& f) {
     case kKindSigmaXY: return f(GaussModel<kSigmaXY>{d.gc});
     case kKindTiltedBgSigmaXY: return f(GaussModel<kTiltedBgSigmaXY>{d.gc});
     case kKindIBg: return f(IBgModel{});
+    case kKindSplineTilted:
+        return d.useGrid ? f(spline_as<kSplineTilted>(d.sg)) : f(spline_as<kSplineTilted>(d.sm));
+    case kKindSplineImage:
+        return d.useGrid ? f(spline_as<kSplineImage>(d.sg)) : f(spline_as<kSplineImage>(d.sm));
     default: return hipErrorInvalidValue;
     }
 }

@@ -4,6 +4,7 @@
 //   (layout of ComputeDerivatives, Estimation.h:169-181).
 // crlb_kernel: one wavefront per spot; Fisher matrix FI_ij = sum J_i J_j / max(mu,1e-8)
 //   and chi^2 = sum (x-mu)^2/mu (Estimation.h:63-102), FP64 inversion.
+// Models with a background image (kPixelBg) read it from constants [n, R*R].
 #pragma once
 
 #include "models.h"
@@ -28,6 +29,11 @@ __global__ void __launch_bounds__(256) ev_kernel(M model, int n, int R, const fl
     for (int c = 0; c < M::NC; c++) th[K + c] = constants[(size_t)spot * M::NC + c];
     float mu, J[K];
     model.eval_pixel(th, x, y, R, mu, J);
+    if constexpr (M::kPixelBg) {  // background image: mu + bg * b, d mu / d bg = b
+        const float b = constants[idx];
+        mu += th[M::iBg] * b;
+        J[M::iBg] = b;
+    }
     if (ev) ev[idx] = mu;
     if (deriv) {
 #pragma unroll
@@ -56,6 +62,11 @@ __global__ void __launch_bounds__(256) crlb_kernel(M model, int n, int R, const 
         const int y = p / R, x = p - (p / R) * R;
         float mu, J[K];
         model.eval_pixel(th, x, y, R, mu, J);
+        if constexpr (M::kPixelBg) {
+            const float b = constants[(size_t)spot * S + p];
+            mu += th[M::iBg] * b;
+            J[M::iBg] = b;
+        }
         const float m = fmaxf(1e-8f, mu);
         const float inv = rcpf(m);
         if (sample) {

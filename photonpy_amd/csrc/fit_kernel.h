@@ -57,7 +57,35 @@ struct FitTraits {
     static constexpr int NS = L::NS;             // samples per lane
     static constexpr int NA = K * (K + 1) / 2;   // packed alpha
     static constexpr int LDS = M::lds_floats(R); // LDS floats per group
+    static constexpr int NB = M::kPixelBg ? NS : 1;  // background-image values per lane
 };
+
+// Lane g's share (layout L) of a per-pixel array of the spot, e.g. its background image.
+template <int R, class L, int NS>
+PP_DEV void load_pixels(const float* src, int g, float (&v)[NS]) {
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        const int idx = L::index(g, k);
+        v[k] = idx < R * R ? src[idx] : 0.0f;
+    }
+}
+
+// The model's callback with the background image applied for kPixelBg models:
+// mu + bg * b[k] and d mu / d bg = b[k] (CubicSplinePSFModels.h:433-436).
+template <class M, int NB, class Fn>
+PP_DEV auto pixel_bg(const float (&th)[M::KT], const float (&b)[NB], Fn& fn) {
+    return [&](int k, float mu, const float (&J)[M::K]) {
+        if constexpr (M::kPixelBg) {
+            float Jb[M::K];
+#pragma unroll
+            for (int i = 0; i < M::K; i++) Jb[i] = J[i];
+            Jb[M::iBg] = b[k];
+            fn(k, mu + th[M::iBg] * b[k], Jb);
+        } else {
+            fn(k, mu, J);
+        }
+    };
+}
 
 // Loads lane g's samples of a spot (layout L) and returns the lane's
 // centre-of-mass partial sums.  With CLAMP the stored samples are max(x, 1e-6)
@@ -88,13 +116,14 @@ PP_DEV void load_samples(const float* src, int g, float (&s)[NS], float& sx, flo
 // Fisher matrix (packed) and chi^2 of one spot, written by lane g == 0 of its group.
 template <class M, int R, int G>
 PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTraits<M, R, G>::NS],
-                      const float (&th)[M::KT], int g, float* lds, int spot) {
+                      const float (&pb)[FitTraits<M, R, G>::NB], const float (&th)[M::KT], int g, float* lds,
+                      int spot) {
     using T = FitTraits<M, R, G>;
     constexpr int K = T::K, NA = T::NA;
     float fi[NA + 1];
 #pragma unroll
     for (int i = 0; i < NA + 1; i++) fi[i] = 0.0f;
-    model.template for_pixels<R, G>(th, g, lds, [&](int k, float mu, const float (&J)[K]) {
+    auto body = [&](int k, float mu, const float (&J)[K]) {
         const float m = fmaxf(1e-8f, mu);
         const float inv = rcpf(m);
         const float e = s[k] - m;
@@ -106,7 +135,8 @@ PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTrai
 #pragma unroll
             for (int j = i; j < K; j++) fi[q++] += w * J[j];
         }
-    });
+    };
+    model.template for_pixels<R, G>(th, g, lds, pixel_bg<M>(th, pb, body));
     group_sum_n<G>(fi);
     if (g == 0) {
         if (a.crlb) {
@@ -139,6 +169,7 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
 
     constexpr int KT = T::KT;
     float s[NS];
+    float pb[T::NB];
     float th[KT];
     int iter = 0;
     int spot = g_first;
@@ -146,6 +177,7 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
     auto start_spot = [&](int sp) {
         float sx, sy, sum;
         load_samples<R, L, NS, true, true>(a.sample + (size_t)sp * T::S, g, s, sx, sy, sum);
+        if constexpr (M::kPixelBg) load_pixels<R, L, NS>(a.constants + (size_t)sp * T::S, g, pb);
 #pragma unroll
         for (int c = 0; c < M::NC; c++) th[K + c] = a.constants[(size_t)sp * M::NC + c];
         if (a.initial) {
@@ -215,7 +247,7 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
         float acc[NA + K];
 #pragma unroll
         for (int i = 0; i < NA + K; i++) acc[i] = 0.0f;
-        model.template for_pixels<R, G>(th, g, lds, [&](int k, float mu, const float (&J)[K]) {
+        auto body = [&](int k, float mu, const float (&J)[K]) {
             const float xv = s[k];  // clamped at load
             const float inv = rcpf(fmaxf(mu, 1e-6f));
             const float t = xv * inv;
@@ -230,7 +262,8 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
             }
 #pragma unroll
             for (int i = 0; i < K; i++) acc[NA + i] += bf * J[i];
-        });
+        };
+        model.template for_pixels<R, G>(th, g, lds, pixel_bg<M>(th, pb, body));
         group_sum_n<G>(acc);
 
         double step[K];
@@ -264,16 +297,18 @@ __device__ __forceinline__ void crlb_spots(const M& model, const FitArgs& a, int
     constexpr int K = T::K, NS = T::NS, KT = T::KT;
     const int g = threadIdx.x & (G - 1);
     float s[NS];
+    float pb[T::NB];
     float th[KT];
     for (int sp = g_first; __any(sp < a.n); sp += stride) {
         if (sp < a.n) {
             float sx, sy, sum;
             load_samples<R, typename T::L, NS, false, false>(a.sample + (size_t)sp * T::S, g, s, sx, sy, sum);
+            if constexpr (M::kPixelBg) load_pixels<R, typename T::L, NS>(a.constants + (size_t)sp * T::S, g, pb);
 #pragma unroll
             for (int k = 0; k < K; k++) th[k] = a.params[(size_t)sp * K + k];
 #pragma unroll
             for (int c = 0; c < M::NC; c++) th[K + c] = a.constants[(size_t)sp * M::NC + c];
-            crlb_spot<M, R, G>(model, a, s, th, g, lds, sp);
+            crlb_spot<M, R, G>(model, a, s, pb, th, g, lds, sp);
         }
     }
 }

@@ -1,0 +1,15 @@
+// Fit-kernel instantiations: splinetilted model, size list part 3 (written by gen_fit_sources.py).
+#include "dispatch.h"
+#include "fit_sizes.h"
+
+namespace ppa {
+
+
+hipError_t launch_fit_splinetilted_3(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st) {
+    using Model = SplineModelT<kSplineTilted>;
+    Model m = spline_as<kSplineTilted>(d.sm);
+    PPA_VAR_SIZES_3(PPA_FIT_CASE)
+    return hipErrorInvalidValue;
+}
+
+}  // namespace ppa

@@ -13,7 +13,14 @@ MODELS = [("fixed", "GaussModel<kFixed>", "Model m{d.gc};", "PPA_FIT_SIZES"),
           ("tiltedbg", "GaussModel<kTiltedBg>", "Model m{d.gc};", "PPA_VAR_SIZES"),
           ("sigmaxy", "GaussModel<kSigmaXY>", "Model m{d.gc};", "PPA_VAR_SIZES"),
           ("tiltedbgsigmaxy", "GaussModel<kTiltedBgSigmaXY>", "Model m{d.gc};", "PPA_VAR_SIZES"),
-          ("ibg", "IBgModel", "Model m{};", "PPA_VAR_SIZES")]
+          ("ibg", "IBgModel", "Model m{};", "PPA_VAR_SIZES"),
+          ("splinetilted", "SplineModelT<kSplineTilted>", "Model m = spline_as<kSplineTilted>(d.sm);",
+           "PPA_VAR_SIZES"),
+          ("splinetiltedgrid", "SplineGridModelT<kSplineTilted>", "Model m = spline_as<kSplineTilted>(d.sg);",
+           "PPA_GRID_VAR_SIZES"),
+          ("splineimage", "SplineModelT<kSplineImage>", "Model m = spline_as<kSplineImage>(d.sm);", "PPA_VAR_SIZES"),
+          ("splineimagegrid", "SplineGridModelT<kSplineImage>", "Model m = spline_as<kSplineImage>(d.sg);",
+           "PPA_GRID_VAR_SIZES")]
 PARTS = 4
 
 for name, ty, ctor, sizes in MODELS:

@@ -14,7 +14,14 @@
 //   IBgModel            I,bg at a given x,y,sigma   IntensityBgModel GaussianPSFModels.h:557-623
 // Per-spot constants (NC of them) follow the K parameters in the parameter array
 // the kernels pass around; the Levenberg-Marquardt step never touches them.
-//   SplineModel         x,y,z,I,bg      CSpline_Model_XYZIBg     CubicSplinePSFModels.h:14-154
+//   SplineModelT<kSplineFlat>    x,y,z,I,bg          CSpline_Model_XYZIBg        CubicSplinePSFModels.h:14-154
+//   SplineModelT<kSplineTilted>  x,y,z,I,bg,bgx,bgy  CSpline_Model_XYZITiltedBg  CubicSplinePSFModels.h:157-298
+//   SplineModelT<kSplineImage>   x,y,z,I,bg + a background image per spot (R*R constants)
+//                                                    CSpline_Model_XYZIBgImage   CubicSplinePSFModels.h:300-443
+//   (SplineGridModelT<...>: the same models evaluated from the spline's sample grid)
+// Models with kPixelBg take their background per pixel: mu = (model mu) + bg * b[p]
+// and d mu / d bg = b[p], with b the spot's background image, which the kernels
+// read like the samples and apply to what the model returns.
 //
 // The Gaussians are pixel-integrated: for pixel edge e the argument is
 // a_e = (e - t - 1/2) / (sqrt(2) sigma), and each pixel's terms are differences
@@ -134,6 +141,7 @@ struct GaussModel {
                              : (V == kTiltedBg || V == kSigmaXY) ? 6
                              : (V == kTiltedBgSigmaXY) ? 8 : 5;
     static constexpr int NC = (V == kConstSigma || V == kTiltedBg) ? 2 : 0;  // sigma x, y per spot
+    static constexpr bool kPixelBg = false;
     static constexpr int KT = K + NC;
     static constexpr int iX = 0, iY = 1;
     static constexpr int iZ = 2;                               // kAstig only
@@ -354,14 +362,19 @@ struct GaussModel {
 // Cubic-spline PSF: coefs[Nz][Ny][Nx][64], coefficient q = 16*m + 4*n + o
 // multiplies z^m y^n x^o of the sub-voxel position.
 // Grid, clamps, stop limits and initial values shared by both spline forms.
+enum SplineBg { kSplineFlat = 0, kSplineTilted = 1, kSplineImage = 2 };
+
+template <int BG>
 struct SplineCommon {
-    static constexpr int K = 5;
-    static constexpr int NC = 0, KT = K;  // no per-spot constants
+    static constexpr int K = BG == kSplineTilted ? 7 : 5;
+    static constexpr int NC = 0, KT = K;  // the background image is read per pixel, not as th
     static constexpr bool kCrlbAbs = false;
+    static constexpr bool kPixelBg = BG == kSplineImage;
     static constexpr int iI = 3, iBg = 4;
     int Nz, Ny, Nx;  // cells per axis
     float zmin, zmax;
 
+    // stop limits {1e-4, 1e-4, 1e-7, 1e-4, ...} (CubicSplinePSFModels.h:34-38, 175-179, 321-325)
     PP_DEV static float stop_limit(int k) { return (k == 2) ? 1e-7f : 1e-4f; }
 
     PP_DEV void check_limits(float (&t)[K], int R) const {
@@ -375,10 +388,51 @@ struct SplineCommon {
 
     PP_DEV void initial(float cx, float cy, float sum, float (&t)[K]) const {
         t[0] = cx; t[1] = cy; t[2] = 0.0f; t[3] = sum; t[4] = 0.0f;
+#pragma unroll
+        for (int k = 5; k < K; k++) t[k] = 0.0f;
+    }
+
+    // Spot coordinate relative to the spline grid of n cells: t - n/2, or, for the
+    // background-image model, ((t - R/2) + (R - n + 1)/2) + 1/2 (one pixel further,
+    // CubicSplinePSFModels.h:79-80 vs 374-377), in the reference's operation order.
+    PP_DEV static float rel(float t, int n, int R) {
+        if constexpr (BG == kSplineImage)
+            return ((t - (float)R / 2.0f) + ((float)R - (float)n + 1.0f) / 2.0f) + 0.5f;
+        else
+            return t - (float)n * 0.5f;
+    }
+
+    // mu and the background columns of J at pixel (x, y) from I * psf: flat bg,
+    // bg + bgx (x - R/2) + bgy (y - R/2), or nothing (the image term is added by
+    // the kernel, which owns the image).
+    PP_DEV static void background(const float (&t)[K], float I, float psf, int x, int y, int R, float& mu,
+                                  float (&J)[K]) {
+        if constexpr (BG == kSplineTilted) {
+            const float half = (float)R * 0.5f;
+            const float ox = (float)x - half, oy = (float)y - half;
+            J[4] = 1.0f;
+            J[5] = ox;
+            J[6] = oy;
+            mu = I * psf + t[4] + t[5] * ox + t[6] * oy;
+        } else if constexpr (BG == kSplineImage) {
+            J[4] = 0.0f;
+            mu = I * psf;
+        } else {
+            J[4] = 1.0f;
+            mu = I * psf + t[4];
+        }
     }
 };
 
-struct SplineModel : SplineCommon {
+template <int BG>
+struct SplineModelT : SplineCommon<BG> {
+    using Base = SplineCommon<BG>;
+    using Base::K;
+    using Base::Nz;
+    using Base::Ny;
+    using Base::Nx;
+    using Base::zmin;
+    using Base::zmax;
     const float* coefs;
 
     struct Frame {
@@ -391,10 +445,10 @@ struct SplineModel : SplineCommon {
         const float* slab;         // coefficients of z-slab k
     };
 
-    PP_DEV void frame(const float (&t)[K], Frame& F) const {
+    PP_DEV void frame(const float (&t)[K], int R, Frame& F) const {
         F.du = (zmax - zmin) / (float)Nz;
-        F.tx = t[0] - (float)Nx * 0.5f;
-        F.ty = t[1] - (float)Ny * 0.5f;
+        F.tx = Base::rel(t[0], Nx, R);
+        F.ty = Base::rel(t[1], Ny, R);
         float spx = -F.tx; spx -= (float)(int)spx;   // truncation toward zero (reference quirk)
         float spy = -F.ty; spy -= (float)(int)spy;
         float spz = (t[2] - zmin) / F.du;
@@ -440,27 +494,25 @@ struct SplineModel : SplineCommon {
 
     PP_DEV int col_index(const Frame& F, int x) const { return clampi((int)((float)x - F.tx), 0, Nx - 1); }
 
-    // mu and J from the raw spline value and derivatives.  Also returns the
-    // unclamped value (its sign gates the coefficient gradient).
-    PP_DEV static void finish(const Frame& F, const float (&t)[K], float f, float fx, float fy, float fz, float& mu,
-                              float (&J)[K]) {
+    // mu and J at pixel (x, y) from the raw spline value and derivatives.
+    PP_DEV static void finish(const Frame& F, const float (&t)[K], float f, float fx, float fy, float fz, int x,
+                              int y, int R, float& mu, float (&J)[K]) {
         const float I = t[3];
         J[0] = fx * -I;
         J[1] = fy * -I;
         J[2] = fz * (I / F.du);
         const float psf = f < 0.0f ? 0.0f : f;
         J[3] = psf;
-        J[4] = 1.0f;
-        mu = I * psf + t[4];
+        Base::background(t, I, psf, x, y, R, mu, J);
     }
 
-    PP_DEV void eval_pixel(const float (&t)[K], int x, int y, int, float& mu, float (&J)[K]) const {
+    PP_DEV void eval_pixel(const float (&t)[K], int x, int y, int R, float& mu, float (&J)[K]) const {
         Frame F;
-        frame(t, F);
+        frame(t, R, F);
         const float* c = row_ptr(F, y) + 64 * col_index(F, x);
         float f, fx, fy, fz;
         cell(c, F, f, fx, fy, fz);
-        finish(F, t, f, fx, fy, fz, mu, J);
+        finish(F, t, f, fx, fy, fz, x, y, R, mu, J);
     }
 
     template <int R, int G>
@@ -476,7 +528,7 @@ struct SplineModel : SplineCommon {
     template <int R, int RL, int G, class Fn>
     PP_DEV void for_rows(const float (&t)[K], int y0, Fn&& fn) const {
         Frame F;
-        frame(t, F);
+        frame(t, R, F);
 #pragma unroll
         for (int r = 0; r < RL; r++) {
             const int y = y0 + r;
@@ -487,7 +539,7 @@ struct SplineModel : SplineCommon {
                     const float* c = rp + 64 * col_index(F, x);
                     float f, fx, fy, fz, mu, J[K];
                     cell(c, F, f, fx, fy, fz);
-                    finish(F, t, f, fx, fy, fz, mu, J);
+                    finish(F, t, f, fx, fy, fz, x, y, R, mu, J);
                     fn(r, x, mu, J);
                 }
             }
@@ -508,8 +560,17 @@ struct SplineModel : SplineCommon {
 // lane group contracts the grid's four z-slices over the spot's window into two
 // LDS tables (value and z-derivative weights, I / du folded in); each pixel then
 // reads its 4x4 entries of both.  Cell indices, the truncating sub-pixel offset
-// and the psf >= 0 clamp follow CubicSplinePSFModels.h:64-153 as in SplineModel.
-struct SplineGridModel : SplineCommon {
+// and the psf >= 0 clamp follow CubicSplinePSFModels.h:64-153 as in SplineModelT.
+template <int BG>
+struct SplineGridModelT : SplineCommon<BG> {
+    using Base = SplineCommon<BG>;
+    using Base::K;
+    using Base::Nz;
+    using Base::Ny;
+    using Base::Nx;
+    using Base::zmin;
+    using Base::zmax;
+    using Base::iI;
     const float* grid;
 
     template <int R, int G>
@@ -537,10 +598,10 @@ struct SplineGridModel : SplineCommon {
         float wz[4], wdz[4];   // z weights; derivative weights times I / du
     };
 
-    PP_DEV void frame(const float (&t)[K], Frame& F) const {
+    PP_DEV void frame(const float (&t)[K], int R, Frame& F) const {
         const float du = (zmax - zmin) / (float)Nz;
-        F.tx = t[0] - (float)Nx * 0.5f;
-        F.ty = t[1] - (float)Ny * 0.5f;
+        F.tx = Base::rel(t[0], Nx, R);
+        F.ty = Base::rel(t[1], Ny, R);
         float spx = -F.tx; spx -= (float)(int)spx;   // truncation toward zero (reference quirk)
         float spy = -F.ty; spy -= (float)(int)spy;
         float spz = (t[2] - zmin) / du;
@@ -561,20 +622,20 @@ struct SplineGridModel : SplineCommon {
     PP_DEV int cell_x(const Frame& F, int x) const { return clampi((int)((float)x - F.tx), 0, Nx - 1); }
     PP_DEV int cell_y(const Frame& F, int y) const { return clampi((int)((float)y - F.ty), 0, Ny - 1); }
 
-    PP_DEV void finish(const float (&t)[K], float f, float fx, float fy, float fz, float& mu, float (&J)[K]) const {
+    PP_DEV static void finish(const float (&t)[K], float f, float fx, float fy, float fz, int x, int y, int R,
+                              float& mu, float (&J)[K]) {
         J[0] = fx;  // -I folded into the x/y derivative weights, I / du into the z ones
         J[1] = fy;
         J[2] = fz;
         const float psf = f < 0.0f ? 0.0f : f;
         J[3] = psf;
-        J[4] = 1.0f;
-        mu = t[iI] * psf + t[iBg];
+        Base::background(t, t[iI], psf, x, y, R, mu, J);
     }
 
     // Single-pixel evaluation straight from the grid (expected value / derivatives).
-    PP_DEV void eval_pixel(const float (&t)[K], int x, int y, int, float& mu, float (&J)[K]) const {
+    PP_DEV void eval_pixel(const float (&t)[K], int x, int y, int R, float& mu, float (&J)[K]) const {
         Frame F;
-        frame(t, F);
+        frame(t, R, F);
         const int i = cell_x(F, x), j = cell_y(F, y);
         const int Dy = Ny + 3, Dx = Nx + 3;
         float f = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
@@ -599,7 +660,7 @@ struct SplineGridModel : SplineCommon {
             fy += F.wdy[n] * a;
             fz += F.wy[n] * az;
         }
-        finish(t, f, fx, fy, fz, mu, J);
+        finish(t, f, fx, fy, fz, x, y, R, mu, J);
     }
 
     // Fit-kernel evaluation of lane g's pixels (FlatLayout<R, G>), called by every
@@ -615,7 +676,7 @@ struct SplineGridModel : SplineCommon {
         // in VGPRs for the whole fit.  (An empty asm: it emits no instruction.)
         asm volatile("" : "+v"(g));
         Frame F;
-        frame(t, F);
+        frame(t, R, F);
         const int i0 = cell_x(F, 0), j0 = cell_y(F, 0);
         const int Dy = Ny + 3, Dx = Nx + 3;
         const float* slab = grid + (size_t)F.k * Dy * Dx;
@@ -665,7 +726,7 @@ struct SplineGridModel : SplineCommon {
                     fz += F.wy[n] * az;
                 }
                 float mu, J[K];
-                finish(t, f, fx, fy, fz, mu, J);
+                finish(t, f, fx, fy, fz, x, y, R, mu, J);
                 fn(s, mu, J);
             }
             // one pixel at a time: without this the scheduler hoists the LDS reads of
@@ -679,6 +740,28 @@ struct SplineGridModel : SplineCommon {
     }
 };
 
+using SplineModel = SplineModelT<kSplineFlat>;
+using SplineGridModel = SplineGridModelT<kSplineFlat>;
+
+// The same spline table under another background mode (host side).
+template <int BG>
+inline SplineModelT<BG> spline_as(const SplineModel& s) {
+    SplineModelT<BG> d{};
+    d.Nz = s.Nz; d.Ny = s.Ny; d.Nx = s.Nx;
+    d.zmin = s.zmin; d.zmax = s.zmax;
+    d.coefs = s.coefs;
+    return d;
+}
+
+template <int BG>
+inline SplineGridModelT<BG> spline_as(const SplineGridModel& s) {
+    SplineGridModelT<BG> d{};
+    d.Nz = s.Nz; d.Ny = s.Ny; d.Nx = s.Nx;
+    d.zmin = s.zmin; d.zmax = s.zmax;
+    d.grid = s.grid;
+    return d;
+}
+
 // ---------------------------------------------------------------------------
 // Intensity and background of a spot at a known position and width
 // (IntensityBgModel, GaussianPSFModels.h:557-623, fitted by
@@ -688,6 +771,7 @@ struct SplineGridModel : SplineCommon {
 // bg >= 1e-8; the CRLB takes |diag(FI^-1)| (ComputeCRLB, Estimation.h:318-333).
 struct IBgModel {
     static constexpr int K = 2, NC = 3, KT = 5;
+    static constexpr bool kPixelBg = false;
     static constexpr int iI = 0, iBg = 1;
     static constexpr bool kCrlbAbs = true;
 

@@ -190,3 +190,117 @@ def test_estimate_intensity_bg(ctx, sigma_shape):
     assert np.max(np.abs(crlb - c64) / c64) < 1e-4
     # against the truth: a 2-parameter fit at known position recovers I within a few CRLB
     assert np.mean(np.abs(ibg[:, 0] - th[:, 0]) < 4 * crlb[:, 0]) > 0.99
+
+
+# ------------------------------------------------------- spline background modes
+from tests import _models as MD  # noqa: E402
+
+RS = 20
+
+
+def spline_make(ctx, mode, form):
+    coefs, zmin, zmax = MD.spline_coefs()
+    m = pa.CSplineMethods(ctx)
+    est = m.CreatePSF_XYZIBg(RS, pa.CSplineCalibration(zmin, zmax, coefs), mode, True)
+    est.set_spline_mode(0 if form == "grid" else 1)
+    assert est.spline_form() == form
+    return est, O.spline(RS, coefs, zmin, zmax, mode=mode)
+
+
+def spline_samples(mode, n, seed):
+    rng = np.random.default_rng(seed)
+    K = 7 if mode == 1 else 5
+    th = np.zeros((n, K))
+    th[:, :2] = 9.5 + rng.uniform(-1.5, 1.5, (n, 2))
+    th[:, 2] = rng.uniform(-300, 300, n)
+    th[:, 3] = rng.uniform(1000, 5000, n)
+    th[:, 4] = rng.uniform(2, 10, n)
+    if mode == 1:
+        th[:, 5:7] = rng.uniform(-0.1, 0.1, (n, 2))
+    cst = rng.uniform(0.5, 1.5, (n, RS * RS)).astype(np.float32) if mode == 2 else None
+    coefs, zmin, zmax = MD.spline_coefs()
+    mu = O.spline(RS, coefs, zmin, zmax, mode=mode).expected_value(th, np.float64, constants=cst)
+    smp = rng.poisson(np.maximum(mu, 0)).astype(np.float32)
+    return th.astype(np.float32), cst, smp
+
+
+SPLINE_CASES = [(1, "grid"), (1, "coefficients"), (2, "grid"), (2, "coefficients")]
+
+
+def test_spline_variant_properties(ctx):
+    est, _ = spline_make(ctx, 1, "grid")
+    assert est.ParamFormat() == "x,y,z,I,bg,bgx,bgy" and est.NumConstants() == 0
+    lo, hi = est.ParamLimits()
+    assert lo[5] == -1e6 and hi[6] == 1e6
+    est, _ = spline_make(ctx, 2, "grid")
+    assert est.ParamFormat() == "x,y,z,I,bg" and est.NumConstants() == RS * RS
+
+
+@pytest.mark.parametrize("mode,form", SPLINE_CASES)
+def test_spline_variant_ev_and_derivatives(ctx, mode, form):
+    th, cst, _ = spline_samples(mode, 200, 21)
+    est, om = spline_make(ctx, mode, form)
+    ev = est.ExpectedValue(th, constants=cst)
+    dv, ev2 = est.Derivatives(th, constants=cst)
+    ev64 = om.expected_value(th, np.float64, constants=cst)
+    dv64, _ = om.derivatives(th, np.float64, constants=cst)
+    scale = np.abs(ev64).max(axis=(1, 2))[:, None, None]
+    assert np.max(np.abs(ev - ev64) / scale) < 2e-6
+    assert np.array_equal(ev, ev2)
+    dscale = np.maximum(np.abs(dv64).max(axis=(2, 3), keepdims=True), 1e-6)
+    assert np.max(np.abs(dv - dv64) / dscale) < 5e-5
+
+
+@pytest.mark.parametrize("mode,form", SPLINE_CASES)
+def test_spline_variant_estimate_parity(ctx, mode, form):
+    n, iters = 600, 50
+    th, cst, smp = spline_samples(mode, n, 23)
+    est, om = spline_make(ctx, mode, form)
+    est.SetLevMarParams(1e-18, iters)
+    p_gpu, _, _ = est.Estimate(smp, constants=cst)
+    it_gpu = est.last_iterations
+    p_ref, it_ref, tr = om.estimate(smp, iterations=iters, lambda_=1e-18, trace_len=iters, constants=cst)
+    lo, hi = est.ParamLimits()
+    on_b = lambda p: np.any((p <= lo + 1e-6) | (p >= hi - 1e-6), axis=1)
+    # converged as for the flat spline: the stop fired, or the last step was below
+    # 1e-4 px, 0.1 nm and 1e-5 I (the z and I stop limits are below one FP32 ulp)
+    last = p_ref - tr[:, iters - 1]
+    conv = (it_ref <= iters) | ((np.abs(last[:, 0]) < 1e-4) & (np.abs(last[:, 1]) < 1e-4) &
+                                (np.abs(last[:, 2]) < 0.1) & (np.abs(last[:, 3]) < 1e-5 * np.abs(p_ref[:, 3])))
+    conv &= ~on_b(p_ref) & np.all(np.isfinite(p_ref), axis=1)
+    assert conv.mean() > 0.5, f"only {conv.mean():.3f} converged"
+    a, b = p_gpu[conv], p_ref[conv]
+    ok = np.all(np.abs(a[:, :2] - b[:, :2]) <= 1e-3, axis=1) & (np.abs(a[:, 2] - b[:, 2]) <= 1.0)
+    ok &= np.all(np.abs(a[:, 3:] - b[:, 3:]) <= 1e-4 * np.maximum(np.abs(b[:, 3:]), 1), axis=1)
+    bad = np.where(~ok)[0]
+    other = (on_b(p_gpu) | (it_gpu <= iters))[conv][bad]
+    assert other.all(), f"gpu={a[bad[:3]]} ref={b[bad[:3]]}"
+    assert len(bad) <= max(1, 1e-3 * conv.sum()), f"{len(bad)} of {conv.sum()} differ"
+
+
+@pytest.mark.parametrize("mode,form", [(1, "grid"), (2, "coefficients")])
+def test_spline_variant_crlb(ctx, mode, form):
+    th, cst, smp = spline_samples(mode, 300, 25)
+    est, om = spline_make(ctx, mode, form)
+    crlb, chi = est.CRLBAndChiSquare(th, smp, constants=cst)
+    c64, x64 = om.crlb(th, smp, dtype=np.float64, constants=cst)
+    assert np.max(np.abs(crlb - c64) / c64) < 1e-4
+    assert np.max(np.abs(chi - x64) / np.maximum(x64, 1)) < 1e-4
+
+
+def test_spline_bgimage_queue_matches_direct(ctx):
+    import torch
+    th, cst, smp = spline_samples(2, 1200, 27)
+    est, _ = spline_make(ctx, 2, "grid")
+    est.SetLevMarParams(1e-18, 50)
+    out = est.estimate_device(torch.from_numpy(smp).cuda(), crlb=True, constants=torch.from_numpy(cst).cuda())
+    torch.cuda.synchronize()
+    ids = np.arange(len(smp), dtype=np.int32)
+    with pa.EstimQueue(est, batchSize=256, numStreams=2) as q:
+        q.Schedule(smp[:700], ids=ids[:700], constants=cst[:700])
+        q.Schedule(smp[700:], ids=ids[700:], constants=cst[700:])
+        q.WaitUntilDone()
+        r = q.GetResults()
+    r.SortByID(isUnique=True)
+    assert np.array_equal(r.estim, out["params"].cpu().numpy(), equal_nan=True)
+    assert np.array_equal(r.crlb, out["crlb"].cpu().numpy(), equal_nan=True)

@@ -175,3 +175,95 @@ def test_intensity_bg_crlb_closed_form():
     ref = np.stack([np.sqrt(d / det), np.sqrt(a / det)], 1)
     crlb, _ = model("intensity_bg").crlb(th, dtype=np.float64, constants=cst)
     assert _rel(crlb, ref) < 1e-9
+
+
+# ------------------------------------------------------- spline background modes
+# CSpline_Model_XYZITiltedBg (CubicSplinePSFModels.h:157-298) is the flat-background
+# spline plus bgx (x - R/2) + bgy (y - R/2); CSpline_Model_XYZIBgImage (:300-443)
+# places the spline one pixel further along x and y ((t - R/2) + (R - N + 1)/2 + 1/2
+# = t - N/2 + 1) and scales a per-spot background image by bg.  Both reduce to the
+# flat model, which the spline tests of test_oracle.py pin against the z-stack.
+from tests import _models as MD  # noqa: E402
+
+RS = 20
+
+
+def _spline_models():
+    coefs, zmin, zmax = MD.spline_coefs()
+    return [O.spline(RS, coefs, zmin, zmax, mode=m) for m in (0, 1, 2)]
+
+
+def _spline_theta(n, rng, K=5):
+    th = np.zeros((n, K))
+    th[:, :2] = 9.5 + rng.uniform(-1.5, 1.5, (n, 2))
+    th[:, 2] = rng.uniform(-300, 300, n)
+    th[:, 3] = rng.uniform(1000, 5000, n)
+    th[:, 4] = rng.uniform(2, 10, n)
+    if K == 7:
+        th[:, 5:7] = rng.uniform(-0.1, 0.1, (n, 2))
+    return th
+
+
+def test_spline_tilted_is_flat_plus_tilt():
+    flat, tilted, _ = _spline_models()
+    assert (tilted.K, tilted.NC) == (7, 0)
+    th = _spline_theta(30, RNG, 7)
+    dv, ev = tilted.derivatives(th, np.float64)
+    dvf, evf = flat.derivatives(th[:, :5], np.float64)
+    c = np.arange(RS) - RS / 2
+    tilt = th[:, 5, None, None] * c[None, None, :] + th[:, 6, None, None] * c[None, :, None]
+    assert _rel(ev, evf + tilt) < 1e-12
+    assert np.array_equal(dv[:, :5], dvf)
+    assert np.array_equal(dv[:, 5], np.broadcast_to(c[None, :], (30, RS, RS)))
+    assert np.array_equal(dv[:, 6], np.broadcast_to(c[:, None], (30, RS, RS)))
+
+
+def test_spline_bgimage_is_shifted_flat_times_image():
+    flat, _, image = _spline_models()
+    assert (image.K, image.NC) == (5, RS * RS)
+    th = _spline_theta(30, RNG)
+    b = RNG.uniform(0.5, 1.5, (30, RS * RS))
+    dv, ev = image.derivatives(th, np.float64, constants=b)
+    sh = th.copy()
+    sh[:, :2] += 1.0  # one pixel further
+    sh[:, 4] = 0.0
+    dvf, evf = flat.derivatives(sh, np.float64)
+    bimg = b.reshape(30, RS, RS)
+    assert _rel(ev, evf + th[:, 4, None, None] * bimg) < 1e-10
+    assert _rel(dv[:, :4], dvf[:, :4]) < 1e-10
+    assert np.array_equal(dv[:, 4], bimg)
+    with pytest.raises(ValueError):
+        image.expected_value(th)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_spline_variant_noise_free_fixed_point(mode):
+    m = _spline_models()[mode]
+    rng = np.random.default_rng(17)
+    th = _spline_theta(40, rng, 7 if mode == 1 else 5)
+    cst = rng.uniform(0.5, 1.5, (40, RS * RS)) if mode == 2 else None
+    mu = m.expected_value(th, np.float64, constants=cst)
+    start = th.copy()
+    start[:, :2] += 0.2
+    start[:, 2] += 30
+    start[:, 3] *= 1.05
+    if mode == 1:
+        start[:, 5:7] = 0
+    p, _ = m.estimate(mu, initial=start, iterations=200, lambda_=1e-15, dtype=np.float64, constants=cst)
+    err = np.abs(p - th) / np.maximum(np.abs(th), 1)
+    assert np.max(err) < 1e-4
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_spline_variant_crlb_vs_numpy_fisher(mode):
+    m = _spline_models()[mode]
+    rng = np.random.default_rng(19)
+    th = _spline_theta(10, rng, 7 if mode == 1 else 5)
+    cst = rng.uniform(0.5, 1.5, (10, RS * RS)) if mode == 2 else None
+    dv, mu = m.derivatives(th, np.float64, constants=cst)
+    K = th.shape[1]
+    J = dv.reshape(10, K, -1)
+    FI = np.einsum("nks,nls->nkl", J / np.maximum(mu.reshape(10, 1, -1), 1e-8), J)
+    ref = np.sqrt(np.diagonal(np.linalg.inv(FI), axis1=1, axis2=2))
+    crlb, _ = m.crlb(th, dtype=np.float64, constants=cst)
+    assert _rel(crlb, ref) < 1e-8
