@@ -104,7 +104,9 @@ void Gauss2D_EstimateIntensityBg(const float* imageData, float* IBg, float* IBg_
                                  const int* roipos, const float* sigma, int imgw, int maxIterations, bool cuda);
 /* Estimators/Gaussian/GaussianPSF.cu:36-59 — params x,y,z,I,bg */
 Estimator* Gauss2D_CreatePSF_XYZIBg(int roisize, const Gauss3D_Calibration* calib, bool cuda, Context* ctx);
-/* Estimators/CSpline/CubicSplinePSF.cu:55-107 — params x,y,z,I,bg; mode 0 (flat background) only */
+/* Estimators/CSpline/CubicSplinePSF.cu:55-107 — mode 0 (CSPLINE_FLATBG): params x,y,z,I,bg;
+ * mode 1 (TILTEDBG): x,y,z,I,bg,bgx,bgy; mode 2 (BGIMAGE): x,y,z,I,bg with an R x R background
+ * image per spot as its constants (NumConstants = roisize^2) */
 Estimator* CSpline_CreatePSF(int roisize, const CSpline_Calibration* calib, const float* coefs, int mode, bool cuda,
                              Context* ctx);
 /* Estimators/CSpline/CubicSplinePSF.cu:151-204 — shape {Nz,Ny,Nx};
@@ -131,7 +133,8 @@ void Estimator_GetLMParams(Estimator* e, LMParams* p);
 void Estimator_ComputeExpectedValue(Estimator* e, int numspots, const float* params, const float* constants,
                                     const int* spotpos, float* ev);
 /* Estimators/Estimator.cpp:115-128 — initial may be NULL (centre-of-mass start),
- * trace [numspots, traceBufLen, K] may be NULL; diag has 0 floats per spot */
+ * trace [numspots, traceBufLen, K] may be NULL; diagnostics [numspots, numDiag] (may be NULL):
+ * numDiag is 0 for the models and 3 for a GLRT wrapper */
 void Estimator_Estimate(Estimator* e, int numspots, const float* sample, const float* constants, const int* spotpos,
                         const float* initial, float* params, float* diagnostics, int* iterations, float* trace,
                         int traceBufLen);
@@ -149,7 +152,7 @@ void Estimator_ChiSquareAndCRLB(Estimator* e, const float* params, const float* 
  * numStreams streams (< 0: 3); GetResults copies whole finished batches, newest first,
  * while they fit in maxresults and returns the number of spots copied (estim [n,K],
  * crlb [n,K], roipos [n,2], samples [n,H,W] when keepSamples, result [n]; each may be
- * NULL; diag is unused, numDiag = 0).  h_constants [n, NumConstants] is required by the
+ * NULL; diag [n, numDiag] for a GLRT wrapper).  h_constants [n, NumConstants] is required by the
  * models that have per-spot constants.  A queue created in a Context is deleted by
  * Context_Destroy. */
 EstimationQueue* EstimQueue_Create(Estimator* psf, int batchSize, int maxQueueLen, bool keepSamples, int numStreams,
@@ -164,8 +167,24 @@ int EstimQueue_GetQueueLength(EstimationQueue* q);
 int EstimQueue_GetResults(EstimationQueue* q, int maxresults, float* estim, float* diag, float* crlb, int* roipos,
                           float* samples, EstimatorResult* result);
 
+/* Estimators/GLRT_PSF.cu:58-169 — wraps `model` (not owned: delete the wrapper first);
+ * every call computes with the model and its LM parameters, and Estimate also writes
+ * numDiag = 3 diagnostics per spot {ll_on, ll_off, mean}: the Poisson log-likelihoods
+ * sum d log(mu) - mu of the fit and of a flat background at `mean` (mu, d clamped at 1e-8).
+ * As in the reference, `mean` is the mean sample of the first spot of each call (each
+ * batch for a queue); GLRT_SetMeanOfFirstSpot(e, 0) makes it each spot's own. */
+Estimator* GLRT_CreatePSF(Estimator* model, Context* ctx);
+
 /* ---------------------------------------------------------------------- */
 /* MI355X additions (no reference counterpart)                              */
+
+/* GLRT wrapper: 1 (default) = the reference's mean of the call's first spot, 0 = each
+ * spot's own mean.  Returns 0, or 1 when e is not a GLRT wrapper. */
+int GLRT_SetMeanOfFirstSpot(Estimator* e, int enable);
+/* Diagnostics of fitted spots on the device: d_diag [n, numDiag] from d_sample [n,H,W] and
+ * d_params [n,K] (a GLRT wrapper: {ll_on, ll_off, mean}); a no-op when numDiag = 0. */
+int Estimator_DiagnosticsDevice(Estimator* e, int n, const float* d_sample, const float* d_constants,
+                                const float* d_params, float* d_diag, void* stream);
 
 /* Last error message of the calling thread ("" if none); cleared by SMLM_ClearError. */
 const char* SMLM_GetLastError(void);
@@ -208,7 +227,7 @@ int Estimator_EstimateDevice(Estimator* e, int n, const float* d_sample, const f
                              const float* d_initial, float* d_params, int* d_iterations, float* d_trace, int traceLen,
                              float* d_crlb, float* d_chisq, void* stream);
 /* d_constants [n, NumConstants]: the per-spot constants of the models that have them
- * (XYIBgConstSigma, XYITiltedBg: sigma x, y), else NULL. */
+ * (XYIBgConstSigma, XYITiltedBg: sigma x, y; spline mode 2: the background image), else NULL. */
 int Estimator_ExpectedValueDevice(Estimator* e, int n, const float* d_params, const float* d_constants, float* d_ev,
                                   void* stream);
 int Estimator_DerivativesDevice(Estimator* e, int n, const float* d_params, const float* d_constants, float* d_deriv,

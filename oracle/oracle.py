@@ -70,6 +70,8 @@ def lib():
                                                             ct.c_int]
             getattr(_lib, f"or_crlb_{sfx}").argtypes = [M, ct.c_int, R, R, R, R, R, ct.c_int]
             getattr(_lib, f"or_loglik_{sfx}").argtypes = [M, ct.c_int, R, R, R, R, ct.c_int]
+        for sfx, R in (("f32", F), ("f64", D)):
+            getattr(_lib, f"or_glrt_{sfx}").argtypes = [M, ct.c_int, R, R, R, ct.c_int, R, ct.c_int]
         _lib.or_num_params.argtypes = [ct.c_int]
         _lib.or_num_consts.argtypes = [ct.c_int, ct.c_int]
         _lib.or_spline_compute.argtypes = [I, F, F, ct.c_int]
@@ -171,6 +173,18 @@ class Model:
         f(ct.byref(self.desc), n, self._ptr(p, dtype), self._ptr(c, dtype), self._ptr(s, dtype), self._ptr(out, dtype),
           self._ptr(chi, dtype), _nthreads(nthreads))
         return out, chi
+
+    def glrt(self, params, samples, mean_of_first_spot=True, dtype=np.float32, nthreads=None, constants=None):
+        """GLRT diagnostics [n, 3] = {ll_on, ll_off, mean} of fitted params (GLRT_PSF.cu:104-138)."""
+        p = self._arr(params, dtype, (-1, self.K))
+        n, R = len(p), self.roisize
+        c = self._cst(constants, n, dtype)
+        s = self._arr(samples, dtype, (n, R, R))
+        out = np.zeros((n, 3), dtype)
+        f = lib().or_glrt_f32 if dtype == np.float32 else lib().or_glrt_f64
+        f(ct.byref(self.desc), n, self._ptr(p, dtype), self._ptr(c, dtype), self._ptr(s, dtype),
+          int(bool(mean_of_first_spot)), self._ptr(out, dtype), _nthreads(nthreads))
+        return out
 
     def loglik(self, params, samples, dtype=np.float32, nthreads=None, constants=None):
         p = self._arr(params, dtype, (-1, self.K))

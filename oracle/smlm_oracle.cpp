@@ -30,6 +30,7 @@
 //   Gauss2D_Model_XYIBgSigmaXY      GaussianPSFModels.h:388-467
 //   Gauss2D_Model_XYITiltedBgSigmaXY GaussianPSFModels.h:471-553 (mu without the tilt, as there)
 //   IntensityBgModel + Gauss2D_EstimateIntensityBg  GaussianPSFModels.h:557-623, GaussianPSF.cu:216-240
+//   GLRT_PSF diagnostics {ll_on, ll_off, mean}      GLRT_PSF.cu:104-138
 //   ComputeFisherMatrix / ComputeCRLB (|diag|)      Estimation.h:22-60, 318-333
 //   CSpline_Model_XYZIBg       cpp/SMLMLib/Estimators/CSpline/CubicSplinePSFModels.h:14-154
 //   CSpline_Model_XYZITiltedBg CubicSplinePSFModels.h:157-298 (x,y,z,I,bg,bgx,bgy)
@@ -797,6 +798,36 @@ void crlb(const OrModel* d, int n, const T* params, const T* cst, const T* smp, 
     });
 }
 
+// GLRT_PSF::Estimate diagnostics (GLRT_PSF.cu:104-138) of fitted params: per spot
+// {ll_on, ll_off, mean}, mean over the first spot's samples (meanOfSpot0) or the spot's own.
+template <typename T>
+void glrt_all(const OrModel* d, int n, const T* params, const T* cst, const T* smp, int meanOfSpot0, T* diag,
+              int nth) {
+    const int S = d->roisize * d->roisize, K = num_params(d->kind);
+    dispatch<T>(d, [&](auto& m) {
+        pool_for(n, nth, [&](int s) {
+            const T* own = smp + (size_t)s * S;
+            const T* ms = meanOfSpot0 ? smp : own;
+            T mean = 0;
+            for (int i = 0; i < S; i++) mean += ms[i];
+            mean /= T(S);
+            std::vector<T> ev(S);
+            for_spot(m, cst, s, num_consts(d->kind, d->roisize))
+                .Eval(params + (size_t)s * K, [&](int idx, T mu, const T*) { ev[idx] = mu; });
+            T on = 0, off = 0;
+            for (int i = 0; i < S; i++) {
+                const T mu = std::fmax(ev[i], T(1e-8f));
+                const T x = std::fmax(own[i], T(1e-8f));
+                on += x * std::log(mu) - mu;
+                off += x * std::log(mean) - mean;
+            }
+            diag[3 * (size_t)s] = on;
+            diag[3 * (size_t)s + 1] = off;
+            diag[3 * (size_t)s + 2] = mean;
+        });
+    });
+}
+
 template <typename T>
 void loglik_all(const OrModel* d, int n, const T* params, const T* cst, const T* smp, T* ll, int nth) {
     const int S = d->roisize * d->roisize, K = num_params(d->kind);
@@ -812,6 +843,10 @@ void loglik_all(const OrModel* d, int n, const T* params, const T* cst, const T*
 extern "C" {
 
 int or_num_params(int kind) { return num_params(kind); }
+void or_glrt_f32(const OrModel* d, int n, const float* p, const float* cst, const float* smp, int meanOfSpot0,
+                 float* diag, int nth) { glrt_all<float>(d, n, p, cst, smp, meanOfSpot0, diag, nth); }
+void or_glrt_f64(const OrModel* d, int n, const double* p, const double* cst, const double* smp, int meanOfSpot0,
+                 double* diag, int nth) { glrt_all<double>(d, n, p, cst, smp, meanOfSpot0, diag, nth); }
 int or_num_consts(int kind, int roisize) { return num_consts(kind, roisize); }
 
 // cst: [n, NC] per-spot constants of the models that have them, else NULL.

@@ -84,6 +84,9 @@ MixedBatch* MixedBatch_Create(int nbins, Estimator* const* ests, const int* coun
                 for (int b = 0; b < nbins; b++) {
                     Estimator* e = ests[b];
                     if (!e) throw std::runtime_error("photonpy_amd: NULL estimator in mixed batch");
+                    if (e->numDiag > 0)
+                        throw std::runtime_error("photonpy_amd: a mixed batch returns no diagnostics; pass the "
+                                                 "wrapped estimator instead of its GLRT wrapper");
                     if (e->NC() > 0)
                         throw std::runtime_error("photonpy_amd: models with per-spot constants are not batched");
                     if (counts[b] < 0) throw std::runtime_error("photonpy_amd: negative spot count");

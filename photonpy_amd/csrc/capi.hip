@@ -533,7 +533,7 @@ void Estimator_GetProperties(Estimator* e, EstimatorProperties* p) {
     if (!e || !p) return;
     p->numParams = e->K();
     p->sampleCount = e->S();
-    p->numDiag = 0;  // the reference's GPU estimators report no diagnostics (EstimatorImpl.h:34)
+    p->numDiag = e->numDiag;  // 0 for the models, as the reference's GPU estimators (EstimatorImpl.h:34)
     p->numConst = e->NC();
     p->sampleIndexDims = 2;
 }
@@ -558,21 +558,27 @@ void Estimator_GetLMParams(Estimator* e, LMParams* p) {
 }
 
 int Estimator_SetGroupSize(Estimator* e, int group) {
+    e = model_of(e);
     if (!e) return 1;
     if (group < 0 || group > 64 || (group & (group - 1)) != 0) return 1;
     e->group = group;
     return 0;
 }
 
-int Estimator_GetGroupSize(Estimator* e) { return e ? e->G() : 0; }
+int Estimator_GetGroupSize(Estimator* e) {
+    e = model_of(e);
+    return e ? e->G() : 0;
+}
 
 int Estimator_SetRefill(Estimator* e, int refill) {
+    e = model_of(e);
     if (!e || refill < 0 || refill > 64) return 1;
     e->refill = refill;
     return 0;
 }
 
 int Estimator_SetSplineMode(Estimator* e, int mode) {
+    e = model_of(e);
     if (!e || !is_spline(e->desc.kind) || mode < 0 || mode > 1) return 1;
     std::lock_guard<std::mutex> lk(e->mtx);
     e->splineMode = mode;
@@ -581,11 +587,13 @@ int Estimator_SetSplineMode(Estimator* e, int mode) {
 }
 
 int Estimator_SplineForm(Estimator* e) {
+    e = model_of(e);
     if (!e || !is_spline(e->desc.kind)) return -1;
     return e->desc.useGrid ? 1 : 0;
 }
 
 int Estimator_SetPhaseTiming(Estimator* e, int enable) {
+    e = model_of(e);
     if (!e) return 1;
     std::lock_guard<std::mutex> lk(e->mtx);
     e->phaseTiming = enable != 0;
@@ -594,6 +602,7 @@ int Estimator_SetPhaseTiming(Estimator* e, int enable) {
 }
 
 int Estimator_GetPhaseTimes(Estimator* e, float* fitMs, float* crlbMs) {
+    e = model_of(e);
     if (!e) return -1;
     return guarded(
         [&]() -> int {
@@ -630,6 +639,7 @@ static const float* upload_constants(Estimator* e, Slicer& sl, int n, const floa
 
 void Estimator_ComputeExpectedValue(Estimator* e, int n, const float* params, const float* constants,
                                     const int* spotpos, float* ev) {
+    e = model_of(e);
     (void)spotpos;
     if (!e || n <= 0) return;
     guarded(
@@ -651,6 +661,7 @@ void Estimator_ComputeExpectedValue(Estimator* e, int n, const float* params, co
 
 void Estimator_ComputeDerivatives(Estimator* e, int n, const float* params, const float* constants,
                                   const int* spotpos, float* deriv, float* ev) {
+    e = model_of(e);
     (void)spotpos;
     if (!e || n <= 0) return;
     guarded(
@@ -676,25 +687,33 @@ void Estimator_Estimate(Estimator* e, int n, const float* sample, const float* c
                         const float* initial, float* params, float* diagnostics, int* iterations, float* trace,
                         int traceBufLen) {
     (void)spotpos;
-    (void)diagnostics;
+    Estimator* outer = e;
+    e = model_of(e);
     if (!e || n <= 0) return;
     guarded(
         [&]() -> int {
             std::lock_guard<std::mutex> l(e->mtx);
             const size_t K = e->K(), S = e->S(), C = e->NC();
             const size_t tl = (trace && traceBufLen > 0) ? (size_t)traceBufLen : 0;
+            const size_t D = diagnostics ? (size_t)outer->numDiag : 0;
             Slicer sl{(char*)e->workspace(Slicer::bytes(
-                {n * S * 4, initial ? n * K * 4 : 0, n * K * 4, (size_t)n * 4, n * tl * K * 4, n * C * 4}))};
+                {n * S * 4, initial ? n * K * 4 : 0, n * K * 4, (size_t)n * 4, n * tl * K * 4, n * C * 4, n * D * 4}))};
             float* ds = sl.take<float>(n * S);
             float* di = initial ? sl.take<float>(n * K) : nullptr;
             float* dp = sl.take<float>(n * K);
             int* dit = sl.take<int>(n);
             float* dt = tl ? sl.take<float>(n * tl * K) : nullptr;
             const float* dcn = upload_constants(e, sl, n, constants);
+            float* dd = D ? sl.take<float>(n * D) : nullptr;
             hip_check(hipMemcpy(ds, sample, n * S * 4, hipMemcpyHostToDevice), "H2D samples");
             if (di) hip_check(hipMemcpy(di, initial, n * K * 4, hipMemcpyHostToDevice), "H2D initial");
             if (dt) hip_check(hipMemset(dt, 0, n * tl * K * 4), "clear trace");
             run_fit(e, n, ds, dcn, di, dp, dit, dt, (int)tl, nullptr, nullptr, 0);
+            if (dd) {
+                hip_check(launch_glrt_any(e->desc, e->roisize, n, dp, dcn, ds, outer->glrtMeanOfSpot0, dd, 0),
+                          "glrt kernel");
+                hip_check(hipMemcpy(diagnostics, dd, n * D * 4, hipMemcpyDeviceToHost), "D2H diagnostics");
+            }
             hip_check(hipMemcpy(params, dp, n * K * 4, hipMemcpyDeviceToHost), "D2H params");
             if (iterations) hip_check(hipMemcpy(iterations, dit, (size_t)n * 4, hipMemcpyDeviceToHost), "D2H iterations");
             if (dt) hip_check(hipMemcpy(trace, dt, n * tl * K * 4, hipMemcpyDeviceToHost), "D2H trace");
@@ -705,6 +724,7 @@ void Estimator_Estimate(Estimator* e, int n, const float* sample, const float* c
 
 void Estimator_ChiSquareAndCRLB(Estimator* e, const float* params, const float* sample, const float* constants,
                                 const int* spotpos, int n, float* crlb, float* chisq) {
+    e = model_of(e);
     (void)spotpos;
     if (!e || n <= 0) return;
     guarded(
@@ -739,6 +759,7 @@ static void require_constants(Estimator* e, const float* d_constants) {
 int Estimator_EstimateDevice(Estimator* e, int n, const float* d_sample, const float* d_constants,
                              const float* d_initial, float* d_params, int* d_iterations, float* d_trace, int traceLen,
                              float* d_crlb, float* d_chisq, void* stream) {
+    e = model_of(e);
     if (!e) return 1;
     if (n <= 0) return 0;
     return guarded(
@@ -753,6 +774,7 @@ int Estimator_EstimateDevice(Estimator* e, int n, const float* d_sample, const f
 
 int Estimator_ExpectedValueDevice(Estimator* e, int n, const float* d_params, const float* d_constants, float* d_ev,
                                   void* stream) {
+    e = model_of(e);
     if (!e) return 1;
     if (n <= 0) return 0;
     return guarded(
@@ -769,6 +791,7 @@ int Estimator_ExpectedValueDevice(Estimator* e, int n, const float* d_params, co
 
 int Estimator_DerivativesDevice(Estimator* e, int n, const float* d_params, const float* d_constants, float* d_deriv,
                                 float* d_ev, void* stream) {
+    e = model_of(e);
     if (!e) return 1;
     if (n <= 0) return 0;
     return guarded(
@@ -785,6 +808,7 @@ int Estimator_DerivativesDevice(Estimator* e, int n, const float* d_params, cons
 
 int Estimator_ChiSquareAndCRLBDevice(Estimator* e, int n, const float* d_params, const float* d_constants,
                                      const float* d_sample, float* d_crlb, float* d_chisq, void* stream) {
+    e = model_of(e);
     if (!e) return 1;
     if (n <= 0) return 0;
     return guarded(
@@ -800,9 +824,56 @@ int Estimator_ChiSquareAndCRLBDevice(Estimator* e, int n, const float* d_params,
         2);
 }
 
-int Estimator_NumShared(Estimator* e) { return e ? (int)e->num_shared() : -1; }
+int Estimator_DiagnosticsDevice(Estimator* e, int n, const float* d_sample, const float* d_constants,
+                                const float* d_params, float* d_diag, void* stream) {
+    if (!e) return 1;
+    if (n <= 0 || e->numDiag == 0) return 0;
+    Estimator* outer = e;
+    e = model_of(e);
+    return guarded(
+        [&]() -> int {
+            std::lock_guard<std::mutex> l(e->mtx);
+            e->ensure_device();
+            require_constants(e, d_constants);
+            if (!d_sample || !d_params || !d_diag) throw std::runtime_error("photonpy_amd: NULL argument");
+            hip_check(launch_glrt_any(e->desc, e->roisize, n, d_params, d_constants, d_sample, outer->glrtMeanOfSpot0,
+                                      d_diag, as_stream(stream)),
+                      "glrt kernel");
+            return 0;
+        },
+        2);
+}
+
+Estimator* GLRT_CreatePSF(Estimator* model, Context* ctx) {
+    return guarded(
+        [&]() -> Estimator* {
+            if (!model) throw std::runtime_error("photonpy_amd: GLRT_CreatePSF needs an estimator");
+            auto* g = new Estimator();
+            g->inner = model;
+            g->roisize = model->roisize;
+            g->desc = model->desc;  // for the shapes (K, constants, samples); fits run on the model
+            g->fmt = model->fmt;
+            g->limits = model->limits;
+            g->numDiag = 3;
+            attach(g, ctx);
+            return g;
+        },
+        nullptr);
+}
+
+int GLRT_SetMeanOfFirstSpot(Estimator* e, int enable) {
+    if (!e || e->numDiag != 3) return 1;
+    e->glrtMeanOfSpot0 = enable != 0;
+    return 0;
+}
+
+int Estimator_NumShared(Estimator* e) {
+    e = model_of(e);
+    return e ? (int)e->num_shared() : -1;
+}
 
 int Estimator_GetShared(Estimator* e, float* out) {
+    e = model_of(e);
     if (!e || !out) return 1;
     std::vector<float> v = e->shared_values();
     std::copy(v.begin(), v.end(), out);
@@ -812,6 +883,7 @@ int Estimator_GetShared(Estimator* e, float* out) {
 int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const float* d_sample, const float* d_shared,
                             const float* d_gout, float* d_nll, float* d_grad_params, float* d_grad_shared,
                             void* stream) {
+    e = model_of(e);
     if (!e) return 1;
     if (n <= 0) return 0;
     return guarded(

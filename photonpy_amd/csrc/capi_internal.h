@@ -54,6 +54,12 @@ struct Estimator {
     std::string fmt;
     std::vector<ParamLimit> limits;
     LMParams lm{30, 1e-15f, 0.0f};  // Estimator.cpp:24
+    // A wrapper (GLRT_CreatePSF) computes with `inner`, whose LM parameters it uses
+    // (GLRT_PSF::Estimate calls the wrapped estimator, GLRT_PSF.cu:100-102); it adds
+    // numDiag diagnostics per spot.
+    Estimator* inner = nullptr;
+    int numDiag = 0;
+    int glrtMeanOfSpot0 = 1;  // the reference's GLRT mean (first spot of the call)
     int group = 0;                  // 0: default_group(roisize)
     int refill = 0;                 // stopped groups per batched refill (0: kernel default)
     // spline coefficients: host copy (owned) and lazily uploaded device copy
@@ -162,6 +168,12 @@ struct Estimator {
         return ws;
     }
 };
+
+// The estimator that does the computing: the innermost one a wrapper refers to.
+inline Estimator* model_of(Estimator* e) {
+    while (e && e->inner) e = e->inner;
+    return e;
+}
 
 inline void attach(Estimator* e, Context* ctx) {
     if (!ctx) return;

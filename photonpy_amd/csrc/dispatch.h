@@ -109,6 +109,9 @@ hipError_t launch_ev_any(const ModelDesc& d, int R, int n, const float* params, 
                          float* deriv, hipStream_t st);
 hipError_t launch_crlb_any(const ModelDesc& d, int R, int n, const float* params, const float* constants,
                            const float* sample, float* crlb, float* chisq, hipStream_t st);
+// GLRT diagnostics {ll_on, ll_off, mean} per spot into diag [n, 3].
+hipError_t launch_glrt_any(const ModelDesc& d, int R, int n, const float* params, const float* constants,
+                           const float* sample, int meanOfSpot0, float* diag, hipStream_t st);
 
 // Poisson NLL and its adjoint (adjoint.hip)
 struct AdjArgs {

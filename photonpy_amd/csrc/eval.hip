@@ -22,6 +22,15 @@ static hipError_t crlb_t(const M& m, int R, int n, const float* params, const fl
     return hipGetLastError();
 }
 
+template <class M>
+static hipError_t glrt_t(const M& m, int R, int n, const float* params, const float* constants, const float* sample,
+                         int meanOfSpot0, float* diag, hipStream_t st) {
+    const unsigned blocks = (unsigned)(((size_t)n * kWave + 255) / 256);
+    hipLaunchKernelGGL(glrt_kernel<M>, dim3(blocks), dim3(256), 0, st, m, n, R, params, constants, sample, meanOfSpot0,
+                       diag);
+    return hipGetLastError();
+}
+
 // Calls f(model) with the model object of a descriptor.
 template <class F>
 static hipError_t with_model(const ModelDesc& d, F&& f) {
@@ -51,6 +60,13 @@ hipError_t launch_ev_any(const ModelDesc& d, int R, int n, const float* params, 
 hipError_t launch_crlb_any(const ModelDesc& d, int R, int n, const float* params, const float* constants,
                            const float* sample, float* crlb, float* chisq, hipStream_t st) {
     return with_model(d, [&](const auto& m) { return crlb_t(m, R, n, params, constants, sample, crlb, chisq, st); });
+}
+
+hipError_t launch_glrt_any(const ModelDesc& d, int R, int n, const float* params, const float* constants,
+                           const float* sample, int meanOfSpot0, float* diag, hipStream_t st) {
+    return with_model(d, [&](const auto& m) {
+        return glrt_t(m, R, n, params, constants, sample, meanOfSpot0, diag, st);
+    });
 }
 
 // ---------------------------------------------------------------------------

@@ -93,4 +93,54 @@ __global__ void __launch_bounds__(256) crlb_kernel(M model, int n, int R, const 
     }
 }
 
+// Generalized likelihood-ratio test diagnostics (GLRT_PSF::Estimate,
+// GLRT_PSF.cu:104-138): per spot {ll_on, ll_off, mean} with
+//   ll_on  = sum_p d log(mu) - mu,        mu = max(model mu, 1e-8), d = max(x, 1e-8)
+//   ll_off = sum_p d log(mean) - mean,
+// the log-likelihoods of the fitted model and of a flat background at `mean`.
+// The reference's mean sums the first spot's samples for every spot (its loop
+// indexes the whole batch's sample array with the pixel counter); meanOfSpot0 = 1
+// keeps that, 0 takes each spot's own mean.  One wavefront per spot.
+template <class M>
+__global__ void __launch_bounds__(256) glrt_kernel(M model, int n, int R, const float* __restrict__ params,
+                                                   const float* __restrict__ constants,
+                                                   const float* __restrict__ sample, int meanOfSpot0,
+                                                   float* __restrict__ diag) {
+    constexpr int K = M::K, KT = M::KT;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int spot = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    if (spot >= n) return;
+    const int S = R * R;
+    const float* own = sample + (size_t)spot * S;
+    const float* ms = meanOfSpot0 ? sample : own;
+    float th[KT];
+#pragma unroll
+    for (int k = 0; k < K; k++) th[k] = params[(size_t)spot * K + k];
+#pragma unroll
+    for (int c = 0; c < M::NC; c++) th[K + c] = constants[(size_t)spot * M::NC + c];
+    float sum = 0.0f;
+    for (int p = lane; p < S; p += kWave) sum += ms[p];
+    sum = group_sum<kWave>(sum);
+    const float mean = sum / (float)S;
+    const float logMean = logf(mean);
+    float ll[2] = {0.0f, 0.0f};
+    for (int p = lane; p < S; p += kWave) {
+        const int y = p / R, x = p - (p / R) * R;
+        float mu, J[K];
+        model.eval_pixel(th, x, y, R, mu, J);
+        if constexpr (M::kPixelBg) mu += th[M::iBg] * constants[(size_t)spot * S + p];
+        mu = fmaxf(mu, 1e-8f);
+        const float d = fmaxf(own[p], 1e-8f);
+        ll[0] += d * logf(mu) - mu;
+        ll[1] += d * logMean - mean;
+    }
+    group_sum_n<kWave>(ll);
+    if (lane == 0) {
+        float* o = diag + (size_t)spot * 3;
+        o[0] = ll[0];
+        o[1] = ll[1];
+        o[2] = mean;
+    }
+}
+
 }  // namespace ppa

@@ -14,8 +14,10 @@
 //   - IsIdle ignores the batch still being filled; Flush queues it (:175-205).
 // Per-spot constants (models that have them) travel with the samples.
 // Differences: batches are fitted by this library's fused kernels (CRLB from the
-// fit's own second kernel), diagnostics are empty (numDiag = 0, as the reference's
-// GPU path), and maxQueueLen < 1 is treated as 1 (the reference never returns).
+// fit's own second kernel), and maxQueueLen < 1 is treated as 1 (the reference never
+// returns).  Diagnostics are those of the estimator: none for the models (as the
+// reference's GPU path), {ll_on, ll_off, mean} per spot for a GLRT wrapper, whose
+// mean comes from each batch's first spot as when the reference's queue calls it.
 // Waits use condition variables instead of the reference's 1 ms / 5 ms sleeps:
 // Schedule wakes when the worker starts or finishes a batch, and the worker
 // sleeps on new work when nothing is in flight and polls its streams every 50 us
@@ -58,11 +60,13 @@ struct EstimationQueue {
         float* chisq = nullptr;    // pinned [batch]
         int* iterations = nullptr; // pinned [batch]
         float* constants = nullptr;  // pinned [batch, NC]
+        float* diag = nullptr;       // pinned [batch, D]
         std::vector<int> roipos;   // [batch, 2], returned with the results
         int numspots = 0;
 
-        Batch(int n, int S, int K, int NC) : ids(n), roipos((size_t)n * 2) {
+        Batch(int n, int S, int K, int NC, int D) : ids(n), roipos((size_t)n * 2) {
             constants = pinned<float>((size_t)n * NC);
+            diag = pinned<float>((size_t)n * D);
             samples = pinned<float>((size_t)n * S);
             params = pinned<float>((size_t)n * K);
             crlb = pinned<float>((size_t)n * K);
@@ -71,7 +75,7 @@ struct EstimationQueue {
         }
         ~Batch() {
             for (void* p : {(void*)samples, (void*)params, (void*)crlb, (void*)chisq, (void*)iterations,
-                            (void*)constants})
+                            (void*)constants, (void*)diag})
                 if (p) (void)hipHostFree(p);
         }
     };
@@ -81,13 +85,15 @@ struct EstimationQueue {
         hipEvent_t done = nullptr;
         std::unique_ptr<Batch> current;
         float *samples = nullptr, *params = nullptr, *crlb = nullptr, *chisq = nullptr, *constants = nullptr;
+        float* diag = nullptr;
         int* iterations = nullptr;
     };
 
-    Estimator* est;
+    Estimator* outer;  // the estimator the queue was created with (a GLRT wrapper or a model)
+    Estimator* est;    // the model that fits
     Context* ctx;
     int device = 0;
-    int batchSize, maxQueueLen, K, S, NC;
+    int batchSize, maxQueueLen, K, S, NC, D;
     bool keepSamples;
     std::atomic<bool> useInitial{false};
     std::vector<Lane> lanes;
@@ -103,8 +109,8 @@ struct EstimationQueue {
     std::string workerError;  // first launch failure, reported by later calls
 
     EstimationQueue(Estimator* e, int batch, int maxLen, bool keep, int streams, Context* c)
-        : est(e), ctx(c), batchSize(batch), maxQueueLen(std::max(1, maxLen)), K(e->K()), S(e->S()), NC(e->NC()),
-          keepSamples(keep) {
+        : outer(e), est(model_of(e)), ctx(c), batchSize(batch), maxQueueLen(std::max(1, maxLen)), K(e->K()),
+          S(e->S()), NC(e->NC()), D(e->numDiag), keepSamples(keep) {
         if (streams < 0) streams = 3;  // EstimationQueue.cpp:16-17
         if (streams == 0) streams = 1;
         hip_check(hipGetDevice(&device), "hipGetDevice");
@@ -123,8 +129,9 @@ struct EstimationQueue {
             ln.chisq = device_alloc<float>(batchSize);
             ln.iterations = device_alloc<int>(batchSize);
             ln.constants = device_alloc<float>((size_t)batchSize * NC);
+            ln.diag = device_alloc<float>((size_t)batchSize * D);
         }
-        next = std::make_unique<Batch>(batchSize, S, K, NC);
+        next = std::make_unique<Batch>(batchSize, S, K, NC, D);
         worker = std::thread([this]() { run(); });
     }
 
@@ -138,7 +145,7 @@ struct EstimationQueue {
         for (Lane& ln : lanes) {
             if (ln.stream) (void)hipStreamSynchronize(ln.stream);
             for (void* p : {(void*)ln.samples, (void*)ln.params, (void*)ln.crlb, (void*)ln.chisq, (void*)ln.iterations,
-                            (void*)ln.constants})
+                            (void*)ln.constants, (void*)ln.diag})
                 if (p) (void)hipFree(p);
             if (ln.done) (void)hipEventDestroy(ln.done);
             if (ln.stream) (void)hipStreamDestroy(ln.stream);
@@ -162,7 +169,14 @@ struct EstimationQueue {
             run_fit(est, n, ln.samples, NC > 0 ? ln.constants : nullptr, init ? ln.params : nullptr, ln.params,
                     ln.iterations, nullptr, 0, ln.crlb,
                     ln.chisq, ln.stream, false);
+            if (D > 0)
+                hip_check(launch_glrt_any(est->desc, est->roisize, n, ln.params, NC > 0 ? ln.constants : nullptr,
+                                          ln.samples, outer->glrtMeanOfSpot0, ln.diag, ln.stream),
+                          "queue: glrt kernel");
         }
+        if (D > 0)
+            hip_check(hipMemcpyAsync(b->diag, ln.diag, sizeof(float) * n * D, hipMemcpyDeviceToHost, ln.stream),
+                      "queue: diagnostics to host");
         hip_check(hipMemcpyAsync(b->params, ln.params, sizeof(float) * n * K, hipMemcpyDeviceToHost, ln.stream),
                   "queue: estimates to host");
         hip_check(hipMemcpyAsync(b->crlb, ln.crlb, sizeof(float) * n * K, hipMemcpyDeviceToHost, ln.stream),
@@ -261,7 +275,7 @@ struct EstimationQueue {
                 recycle.pop_front();
             }
         }
-        if (!next) next = std::make_unique<Batch>(batchSize, S, K, NC);
+        if (!next) next = std::make_unique<Batch>(batchSize, S, K, NC, D);
     }
 
     void schedule(int count, const int* ids, const float* samples, const float* initial, const float* constants,
@@ -301,7 +315,6 @@ struct EstimationQueue {
 
     int results(int count, float* estim, float* diag, float* crlb, int* roipos, float* samples,
                 EstimatorResult* res) {
-        (void)diag;  // numDiag = 0
         int copied = 0;
         while (copied < count) {
             const int space = count - copied;
@@ -322,6 +335,7 @@ struct EstimationQueue {
             }
             if (estim) std::memcpy(estim + (size_t)copied * K, b->params, sizeof(float) * n * K);
             if (crlb) std::memcpy(crlb + (size_t)copied * K, b->crlb, sizeof(float) * n * K);
+            if (diag && D > 0) std::memcpy(diag + (size_t)copied * D, b->diag, sizeof(float) * n * D);
             if (roipos) std::memcpy(roipos + (size_t)copied * 2, b->roipos.data(), sizeof(int) * n * 2);
             if (samples && keepSamples) std::memcpy(samples + (size_t)copied * S, b->samples, sizeof(float) * n * S);
             copied += n;

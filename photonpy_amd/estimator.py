@@ -239,7 +239,8 @@ class Estimator:
     def estimate_device(self, samples, initial=None, crlb=True, iterations_out=True, trace_len=0, stream=None,
                         constants=None):
         """Fit device-resident ROIs [N,H,W] (float32).  Returns a dict of tensors
-        params [N,K], iterations [N], and crlb [N,K] + chisq [N] when crlb=True.
+        params [N,K], iterations [N], crlb [N,K] + chisq [N] when crlb=True, and
+        diag [N,numdiag] for estimators with diagnostics (a GLRT wrapper).
         constants [N, NumConstants] for the models that have per-spot constants."""
         import torch
         n = samples.shape[0]
@@ -265,6 +266,13 @@ class Estimator:
         if rc != 0:
             self._check()
             raise RuntimeError(f"Estimator_EstimateDevice failed ({rc})")
+        if self.numdiag > 0:
+            out["diag"] = torch.empty((n, self.numdiag), dtype=torch.float32, device=dev)
+            rc = self._lib.Estimator_DiagnosticsDevice(self.inst, n, ps, pc, out["params"].data_ptr(),
+                                                       out["diag"].data_ptr(), self._stream(stream))
+            if rc != 0:
+                self._check()
+                raise RuntimeError(f"Estimator_DiagnosticsDevice failed ({rc})")
         return out
 
     def expected_value_device(self, params, stream=None, constants=None):

@@ -130,6 +130,9 @@ def _bind(lib):
         "Estimator_ChiSquareAndCRLBDevice": ([V, I, V, V, V, V, V, V], ct.c_int),
         "Estimator_NLLGradDevice": ([V, I, V, V, V, V, V, V, V, V], ct.c_int),
         "Estimator_NumShared": ([V], ct.c_int),
+        "Estimator_DiagnosticsDevice": ([V, I, V, V, V, V, V], ct.c_int),
+        "GLRT_CreatePSF": ([V, V], V),
+        "GLRT_SetMeanOfFirstSpot": ([V, I], ct.c_int),
         "EstimQueue_Create": ([V, I, I, ct.c_bool, I, V], V),
         "EstimQueue_Delete": ([V], None),
         "EstimQueue_Schedule": ([V, I, IntArray, FloatArray, NullableFloatArrayType, NullableFloatArrayType,

@@ -1,0 +1,97 @@
+"""GLRT wrapper (GLRT_PSF.cu, photonpy/cpp/glrt.py) on the GPU.
+
+The wrapper fits with the wrapped estimator and its LM parameters, so its estimates
+equal the wrapped estimator's bit for bit; its diagnostics {ll_on, ll_off, mean}
+match the oracle's restatement evaluated at the GPU's estimates.
+"""
+import numpy as np
+import pytest
+
+import photonpy_amd as pa
+from photonpy_amd import glrt
+from oracle import oracle as O
+from tests import _models as MD
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    with pa.Context() as c:
+        yield c
+
+
+def _close(a, b):
+    # ll sums of ~R^2 terms of size ~|d log mu|: FP32 rounding of the sum
+    scale = np.maximum(np.abs(b), 1.0)
+    return np.all(np.abs(a - b) <= 2e-5 * scale)
+
+
+@pytest.mark.parametrize("kind", ["fixed", "sigma", "astig", "spline"])
+def test_glrt_estimate(ctx, kind):
+    th, smp = MD.samples(kind, 800, 31)
+    est, om, _ = MD.make(kind, ctx)
+    est.SetLevMarParams(1e-15, 30)
+    g = glrt.CreateEstimator(est, ctx)
+    assert g.numdiag == 3 and g.NumParams() == est.NumParams() and g.ParamFormat() == est.ParamFormat()
+    # the wrapper's own LM parameters are ignored (GLRT_PSF::Estimate runs the wrapped estimator)
+    g.SetLevMarParams(1e-3, 3)
+    p_ref, _, _ = est.Estimate(smp)
+    p, diag, _ = g.Estimate(smp)
+    assert np.array_equal(p, p_ref, equal_nan=True)
+    assert np.array_equal(g.last_iterations, est.last_iterations)
+    ref = om.glrt(p, smp, True, dtype=np.float64)
+    fin = np.all(np.isfinite(p), axis=1)
+    assert np.all(diag[:, 2] == diag[0, 2])
+    assert _close(diag[fin], ref[fin])
+    # per-spot mean
+    g2 = glrt.CreateEstimator(est, ctx, meanOfFirstSpot=False)
+    _, diag2, _ = g2.Estimate(smp)
+    assert _close(diag2[fin], om.glrt(p, smp, False, dtype=np.float64)[fin])
+
+
+def test_glrt_device_and_detection(ctx):
+    import torch
+    R = 10
+    est, om, fn = MD.make("fixed", ctx)
+    g = glrt.CreateEstimator(est, ctx, meanOfFirstSpot=False)
+    rng = np.random.default_rng(4)
+    th, smp = MD.samples("fixed", 500, 33)
+    bgonly = rng.poisson(10.0, (500, R, R)).astype(np.float32)
+    allsmp = np.concatenate([smp, bgonly])
+    p, diag, _ = g.Estimate(allsmp)
+    out = g.estimate_device(torch.from_numpy(allsmp).cuda())
+    torch.cuda.synchronize()
+    assert np.array_equal(out["params"].cpu().numpy(), p, equal_nan=True)
+    assert np.array_equal(out["diag"].cpu().numpy(), diag, equal_nan=True)
+    lp = glrt.ComputeLogPFA(diag)
+    # spots (I >= 500 photons) are detected far more strongly than pure background
+    assert np.median(lp[:500]) < np.percentile(lp[500:], 1) - 10
+
+
+def test_glrt_queue_mean_per_batch(ctx):
+    th, smp = MD.samples("sigma", 1000, 35)
+    est, om, _ = MD.make("sigma", ctx)
+    g = glrt.CreateEstimator(est, ctx)
+    ids = np.arange(1000, dtype=np.int32)
+    with pa.EstimQueue(g, batchSize=256, numStreams=2) as q:
+        q.Schedule(smp, ids=ids)
+        q.WaitUntilDone()
+        r = q.GetResults()
+    r.SortByID(isUnique=True)
+    assert r.diagnostics.shape == (1000, 3)
+    # the reference's mean is the first spot of each Estimate call: here each batch of 256
+    first = (np.arange(1000) // 256) * 256
+    assert np.allclose(r.diagnostics[:, 2], smp[first].reshape(1000, -1).mean(1), rtol=1e-6)
+    ref = np.concatenate([om.glrt(r.estim[b:b + 256], smp[b:b + 256], True, dtype=np.float64)
+                          for b in range(0, 1000, 256)])
+    fin = np.all(np.isfinite(r.estim), axis=1)
+    assert _close(r.diagnostics[fin], ref[fin])
+
+
+def test_glrt_not_in_mixed_batch(ctx):
+    est, _, _ = MD.make("fixed", ctx)
+    g = glrt.CreateEstimator(est, ctx)
+    from photonpy_amd.batch import MixedBatch
+    with pytest.raises(Exception, match="GLRT"):
+        MixedBatch([g], [10], tune=False)
