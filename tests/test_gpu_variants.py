@@ -202,7 +202,7 @@ def spline_make(ctx, mode, form):
     coefs, zmin, zmax = MD.spline_coefs()
     m = pa.CSplineMethods(ctx)
     est = m.CreatePSF_XYZIBg(RS, pa.CSplineCalibration(zmin, zmax, coefs), mode, True)
-    est.set_spline_mode(0 if form == "grid" else 1)
+    est.set_spline_mode("auto" if form == "grid" else "coefficients")
     assert est.spline_form() == form
     return est, O.spline(RS, coefs, zmin, zmax, mode=mode)
 
