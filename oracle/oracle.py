@@ -175,7 +175,7 @@ class Model:
         return out, chi
 
     def glrt(self, params, samples, mean_of_first_spot=True, dtype=np.float32, nthreads=None, constants=None):
-        """GLRT diagnostics [n, 3] = {ll_on, ll_off, mean} of fitted params (GLRT_PSF.cu:104-138)."""
+        """GLRT diagnostics [n, 3] = {ll_on, ll_off, mean} of fitted params (GLRT_PSF.cu:103-129)."""
         p = self._arr(params, dtype, (-1, self.K))
         n, R = len(p), self.roisize
         c = self._cst(constants, n, dtype)

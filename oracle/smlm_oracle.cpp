@@ -30,7 +30,7 @@
 //   Gauss2D_Model_XYIBgSigmaXY      GaussianPSFModels.h:388-467
 //   Gauss2D_Model_XYITiltedBgSigmaXY GaussianPSFModels.h:471-553 (mu without the tilt, as there)
 //   IntensityBgModel + Gauss2D_EstimateIntensityBg  GaussianPSFModels.h:557-623, GaussianPSF.cu:216-240
-//   GLRT_PSF diagnostics {ll_on, ll_off, mean}      GLRT_PSF.cu:104-138
+//   GLRT_PSF diagnostics {ll_on, ll_off, mean}      GLRT_PSF.cu:103-129
 //   ComputeFisherMatrix / ComputeCRLB (|diag|)      Estimation.h:22-60, 318-333
 //   CSpline_Model_XYZIBg       cpp/SMLMLib/Estimators/CSpline/CubicSplinePSFModels.h:14-154
 //   CSpline_Model_XYZITiltedBg CubicSplinePSFModels.h:157-298 (x,y,z,I,bg,bgx,bgy)
@@ -798,7 +798,7 @@ void crlb(const OrModel* d, int n, const T* params, const T* cst, const T* smp, 
     });
 }
 
-// GLRT_PSF::Estimate diagnostics (GLRT_PSF.cu:104-138) of fitted params: per spot
+// GLRT_PSF::Estimate diagnostics (GLRT_PSF.cu:103-129) of fitted params: per spot
 // {ll_on, ll_off, mean}, mean over the first spot's samples (meanOfSpot0) or the spot's own.
 template <typename T>
 void glrt_all(const OrModel* d, int n, const T* params, const T* cst, const T* smp, int meanOfSpot0, T* diag,

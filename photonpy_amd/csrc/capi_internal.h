@@ -55,7 +55,7 @@ struct Estimator {
     std::vector<ParamLimit> limits;
     LMParams lm{30, 1e-15f, 0.0f};  // Estimator.cpp:24
     // A wrapper (GLRT_CreatePSF) computes with `inner`, whose LM parameters it uses
-    // (GLRT_PSF::Estimate calls the wrapped estimator, GLRT_PSF.cu:100-102); it adds
+    // (GLRT_PSF::Estimate calls the wrapped estimator, GLRT_PSF.cu:91-94); it adds
     // numDiag diagnostics per spot.
     Estimator* inner = nullptr;
     int numDiag = 0;

@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(256) crlb_kernel(M model, int n, int R, const 
 }
 
 // Generalized likelihood-ratio test diagnostics (GLRT_PSF::Estimate,
-// GLRT_PSF.cu:104-138): per spot {ll_on, ll_off, mean} with
+// GLRT_PSF.cu:103-129): per spot {ll_on, ll_off, mean} with
 //   ll_on  = sum_p d log(mu) - mu,        mu = max(model mu, 1e-8), d = max(x, 1e-8)
 //   ll_off = sum_p d log(mean) - mean,
 // the log-likelihoods of the fitted model and of a flat background at `mean`.

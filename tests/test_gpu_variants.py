@@ -105,7 +105,7 @@ def test_estimate_parity(ctx, kind):
 
 def test_tilted_bg_sigma_xy_never_stops(ctx):
     """The reference's XYITiltedBgSigmaXY leaves the tilt out of mu but keeps the tilt
-    columns in its Jacobian (GaussianPSFModels.h:532-534), so on noisy data the tilt takes
+    columns in its Jacobian (GaussianPSFModels.h:542-548), so on noisy data the tilt takes
     the same nonzero step every iteration and the stop never fires: both sides run to the
     iteration limit.  Its LM path is compared over 10 iterations, where FP32 rounding has
     not yet moved it (FP32 vs FP64 oracle agree there on 99.8% of spots)."""

@@ -1,4 +1,4 @@
-"""Pins the oracle's GLRT diagnostics (GLRT_PSF.cu:104-138) and the Python ComputeLogPFA.
+"""Pins the oracle's GLRT diagnostics (GLRT_PSF.cu:103-129) and the Python ComputeLogPFA.
 
 ll_on = sum max(x,1e-8) log(max(mu,1e-8)) - max(mu,1e-8) with mu from the closed-form
 model; ll_off the same with mu = mean; mean = the first spot's mean sample (the

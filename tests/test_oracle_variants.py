@@ -3,9 +3,9 @@
 * expected values  vs the scipy-erf closed forms of tests/_closed.py, with per-spot
                    constants (sigma x/y; x, y, sigma for the intensity/background model)
 * Jacobians        vs central differences; the sigma columns of the SigmaXY models are
-                   sigma x dmu/dsigma (GaussianPSFModels.h:447-448), and the tilted-background
+                   sigma x dmu/dsigma (GaussianPSFModels.h:459-460), and the tilted-background
                    SigmaXY model's mu leaves the tilt out while its Jacobian carries the tilt
-                   columns (x - R/2, y - R/2) (GaussianPSFModels.h:532-534)
+                   columns (x - R/2, y - R/2) (GaussianPSFModels.h:542-548)
 * LM optimizer     noise-free fixed point; FP64 fit vs scipy minimisation of the Poisson NLL
 * CRLB             vs the numpy inverse of the FD Fisher matrix; the intensity/background
                    CRLB vs the closed-form 2x2 inverse
