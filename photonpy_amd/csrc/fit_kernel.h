@@ -60,6 +60,14 @@ struct FitTraits {
     static constexpr int NB = M::kPixelBg ? NS : 1;  // background-image values per lane
 };
 
+// Models that accumulate the fit's normal equations themselves (M::kRowFactored).
+template <class M, class = void>
+struct RowFactored : std::false_type {};
+template <class M>
+struct RowFactored<M, std::void_t<decltype(M::kRowFactored)>> : std::bool_constant<M::kRowFactored> {};
+template <class M>
+constexpr bool row_factored() { return RowFactored<M>::value; }
+
 // Lane g's share (layout L) of a per-pixel array of the spot, e.g. its background image.
 template <int R, class L, int NS>
 PP_DEV void load_pixels(const float* src, int g, float (&v)[NS]) {
@@ -263,7 +271,11 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
 #pragma unroll
             for (int i = 0; i < K; i++) acc[NA + i] += bf * J[i];
         };
-        model.template for_pixels<R, G>(th, g, lds, pixel_bg<M>(th, pb, body));
+        if constexpr (row_factored<M>()) {
+            model.template accumulate_rows<R, G>(th, g, s, acc);
+        } else {
+            model.template for_pixels<R, G>(th, g, lds, pixel_bg<M>(th, pb, body));
+        }
         group_sum_n<G>(acc);
 
         double step[K];

@@ -76,10 +76,37 @@ struct Edge {
 // below FLT_MIN flush to zero.
 PP_DEV float exp_neg_sq(float a) { return __builtin_amdgcn_exp2f(-(a * a) * 1.44269504088896341f); }
 
+// erf(a) given g = exp(-a^2), without branches:
+//   |a| <  1:  a P(a^2), P of degree 6;
+//   |a| >= 1:  1 - g t Q(t), t = 1 / (1 + 0.71 |a|), Q of degree 3
+// (the second is the form of Abramowitz & Stegun 7.1.26, erfc = e^{-x^2} t Q(t)).  P and Q
+// were fitted for this library to erf(x)/x on [0, 1] and to erfc(x) e^{x^2} / t on [1, 12]
+// by weighted least squares with Lawson's minimax reweighting: approximation error below
+// 6e-9; the FP32 evaluation is within 1.3e-7 (2.3 ulp) of erf.  It reuses the edge's
+// exp(-a^2), where a library erff evaluates its own exponential behind a branch.
+PP_DEV float erf_with_exp(float a, float g) {
+    const float x = fabsf(a), u = a * a;
+    float p = 7.647113349610659e-05f;
+    p = fmaf(p, u, -0.0007942355074335833f);
+    p = fmaf(p, u, 0.005179749416557941f);
+    p = fmaf(p, u, -0.02684859124920865f);
+    p = fmaf(p, u, 0.11283430117086246f);
+    p = fmaf(p, u, -0.3761260598505243f);
+    p = fmaf(p, u, 1.128379158410482f);
+    const float lo = x * p;
+    const float t = rcpf(fmaf(0.71f, x, 1.0f));
+    float q = -0.2933092747783785f;
+    q = fmaf(q, t, 0.5490077620704806f);
+    q = fmaf(q, t, 0.33261061953102816f);
+    q = fmaf(q, t, 0.40756540382343115f);
+    const float hi = 1.0f - (g * t) * q;
+    return copysignf(x < 1.0f ? lo : hi, a);
+}
+
 PP_DEV Edge gauss_edge(float e, float t, float c2) {
     float a = (e - t - 0.5f) * c2;
     float g = exp_neg_sq(a);
-    return {erff(a), g, a * g};
+    return {erf_with_exp(a, g), g, a * g};
 }
 
 // Per-axis terms of one pixel from its left (e0) and right (e1) edges.  dE and S
@@ -152,6 +179,8 @@ struct GaussModel {
     static constexpr int iSx = (V == kSigmaXY) ? 4 : 6;        // sigma x, y columns
     static constexpr int iSy = iSx + 1;
     static constexpr bool kCrlbAbs = false;
+    // Models whose fit accumulates the normal equations per row (accumulate_rows).
+    static constexpr bool kRowFactored = (V == kFixed || V == kConstSigma || V == kSigma || V == kAstig);
 
     GaussCalib c;
 
@@ -302,16 +331,11 @@ struct GaussModel {
         for_rows<R, RL, G>(t, g * RL, [&](int r, int x, float mu, const float (&J)[K]) { fn(r * R + x, mu, J); });
     }
 
-    // Fit-kernel evaluation over a block of RL rows starting at y0 of an RxR ROI,
-    // called by every lane of an aligned group of G lanes.  The group shares the
-    // R+1 column edges: lane g evaluates edges g, g+G, ... and the others receive
-    // them by lane broadcast.  f(r, x, mu, J) is called for every pixel of rows
-    // y0+r < R.
-    template <int R, int RL, int G, class F>
-    PP_DEV void for_rows(const float (&t)[KT], int y0, F&& f) const {
-        AxisShape SX, SY;
-        shapes(t, SX, SY);
-        AxisTerms col[R];
+    // The R column terms (I folded into dE and S), the group sharing the R+1 column
+    // edges: lane g evaluates edges g, g+G, ... and the others receive them by lane
+    // broadcast.  Called by every lane of an aligned group of G lanes.
+    template <int R, int G>
+    PP_DEV void columns(const float (&t)[KT], const AxisShape& SX, AxisTerms (&col)[R]) const {
         if constexpr (G == 1) {
             Edge prev = gauss_edge(0.0f, t[0], SX.c2);
 #pragma unroll
@@ -339,6 +363,124 @@ struct GaussModel {
                 prev = nx;
             });
         }
+    }
+
+    // Fit-kernel normal equations of lane g's rows (RowLayout<R, G>), for the
+    // models whose Jacobian columns are sums of (row term) x (column term):
+    //   J_x = Dy a, J_y = Ey b, J_I = Dy b, J_bg = 1, J_s = Dy c + Sy b
+    // with column terms a = I dEx, b = Dx, c = I Sx and row terms Dy, Ey = I dEy,
+    // Sy = I Sy (J_s is the sigma / z column; absent for fixed sigma).  Per pixel
+    // it forms w = x/mu^2 and v = x/mu - 1 and accumulates the row sums
+    // sum w p q and sum v p over p, q in {a, b, c, 1}; per row it adds
+    // alpha_ij += sum_pq Ri_p Rj_q (sum w p q) and beta_i += sum_p Ri_p (sum v p) with
+    // Ri the row coefficients of J_i.  The sums are those of the per-pixel form,
+    // sum w J_i J_j and sum v J_i, regrouped.  acc (packed alpha, i <= j, then beta)
+    // is overwritten: the first row sets it.
+    template <int R, int G, int NS>
+    PP_DEV void accumulate_rows(const float (&t)[KT], int g, const float (&s)[NS],
+                                float (&acc)[K * (K + 1) / 2 + K]) const {
+        constexpr int RL = RowLayout<R, G>::RL;
+        constexpr bool SH = (V == kSigma || V == kAstig);  // has the c column
+        constexpr int NA = K * (K + 1) / 2;
+        // packed upper-triangle index of alpha(i, j), i <= j
+        auto q = [](int i, int j) { return i * K - i * (i - 1) / 2 + (j - i); };
+        AxisShape SX, SY;
+        shapes(t, SX, SY);
+        AxisTerms col[R];
+        columns<R, G>(t, SX, col);
+        const int y0 = g * RL;
+        const float I = t[iI], bg = t[iBg];
+        Edge prev = gauss_edge((float)y0, t[1], SY.c2);
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            const int y = y0 + r;
+            Edge nx = gauss_edge((float)(y + 1), t[1], SY.c2);
+            const AxisTerms row = scaled(axis_terms(prev, nx, SY.cp, SY.cs), I);
+            prev = nx;
+            if (RL * G == R || y < R) {
+                float Saa, Sab, Sa1, Sbb, Sb1, S11, Ta, Tb, T1;
+                float Sac = 0.0f, Sbc = 0.0f, Scc = 0.0f, Sc1 = 0.0f, Tc = 0.0f;
+#pragma unroll
+                for (int x = 0; x < R; x++) {
+                    const float a = col[x].dE, b = col[x].D;
+                    const float p = b * row.D;
+                    const float mu = bg + I * p;
+                    const float inv = rcpf(fmaxf(mu, 1e-6f));
+                    const float tv = s[r * R + x] * inv;  // samples clamped at load
+                    const float w = tv * inv;
+                    const float v = tv - 1.0f;
+                    const float wa = w * a, wb = w * b;
+                    if (x == 0) {
+                        Saa = wa * a; Sab = wa * b; Sa1 = wa;
+                        Sbb = wb * b; Sb1 = wb; S11 = w;
+                        Ta = v * a; Tb = v * b; T1 = v;
+                    } else {
+                        Saa += wa * a; Sab += wa * b; Sa1 += wa;
+                        Sbb += wb * b; Sb1 += wb; S11 += w;
+                        Ta += v * a; Tb += v * b; T1 += v;
+                    }
+                    if constexpr (SH) {
+                        const float c = col[x].S;
+                        const float wc = w * c;
+                        if (x == 0) {
+                            Sac = wa * c; Sbc = wb * c; Scc = wc * c; Sc1 = wc; Tc = v * c;
+                        } else {
+                            Sac += wa * c; Sbc += wb * c; Scc += wc * c; Sc1 += wc; Tc += v * c;
+                        }
+                    }
+                }
+                // the first row sets acc, later rows add to it
+                auto put = [&](int i, float val) {
+                    if (r == 0)
+                        acc[i] = val;
+                    else
+                        acc[i] += val;
+                };
+                const float D = row.D, E = row.dE;
+                const float DD = D * D, DE = D * E;
+                constexpr int X = 0, Y = 1;
+                put(q(X, X), DD * Saa);
+                put(q(X, Y), DE * Sab);
+                put(q(X, iI), DD * Sab);
+                put(q(X, iBg), D * Sa1);
+                put(q(Y, Y), (E * E) * Sbb);
+                put(q(Y, iI), DE * Sbb);
+                put(q(Y, iBg), E * Sb1);
+                put(q(iI, iI), DD * Sbb);
+                put(q(iI, iBg), D * Sb1);
+                put(q(iBg, iBg), S11);
+                put(NA + X, D * Ta);
+                put(NA + Y, E * Tb);
+                put(NA + iI, D * Tb);
+                put(NA + iBg, T1);
+                if constexpr (SH) {
+                    // J_s = D c + Sy b
+                    const float Sy = row.S;
+                    const float DS = D * Sy;
+                    constexpr int s0 = iS;
+                    put(q(X < s0 ? X : s0, X < s0 ? s0 : X), DS * Sab + DD * Sac);
+                    put(q(Y < s0 ? Y : s0, Y < s0 ? s0 : Y), (E * Sy) * Sbb + DE * Sbc);
+                    put(q(iI < s0 ? iI : s0, iI < s0 ? s0 : iI), DS * Sbb + DD * Sbc);
+                    put(q(iBg < s0 ? iBg : s0, iBg < s0 ? s0 : iBg), Sy * Sb1 + D * Sc1);
+                    put(q(s0, s0), (Sy * Sy) * Sbb + (2.0f * DS) * Sbc + DD * Scc);
+                    put(NA + s0, Sy * Tb + D * Tc);
+                }
+            } else if (r == 0) {
+#pragma unroll
+                for (int i = 0; i < NA + K; i++) acc[i] = 0.0f;
+            }
+        }
+    }
+
+    // Fit-kernel evaluation over a block of RL rows starting at y0 of an RxR ROI,
+    // called by every lane of an aligned group of G lanes.  f(r, x, mu, J) is called
+    // for every pixel of rows y0+r < R.
+    template <int R, int RL, int G, class F>
+    PP_DEV void for_rows(const float (&t)[KT], int y0, F&& f) const {
+        AxisShape SX, SY;
+        shapes(t, SX, SY);
+        AxisTerms col[R];
+        columns<R, G>(t, SX, col);
         Edge prev = gauss_edge((float)y0, t[1], SY.c2);
 #pragma unroll
         for (int r = 0; r < RL; r++) {
