@@ -717,7 +717,8 @@ struct SplineGridModelT : SplineCommon<BG> {
 
     template <int R, int G>
     using Layout = FlatLayout<R, G>;
-    static constexpr int lds_floats(int R) { return 2 * (R + 3) * (R + 3); }
+    // window of R + 4 grid samples per axis (see for_pixels), two tables
+    static constexpr int lds_floats(int R) { return 2 * (R + 4) * (R + 4); }
 
     // Lagrange weights on the nodes -1, 0, 1, 2 at t, and their derivatives.
     PP_DEV static void lagrange(float t, float (&w)[4], float (&dw)[4]) {
@@ -809,7 +810,11 @@ struct SplineGridModelT : SplineCommon<BG> {
     // lane of the group with the group's LDS region (lds_floats(R) floats).
     template <int R, int G, class Fn>
     PP_DEV void for_pixels(const float (&t)[K], int g, float* lds, Fn&& fn) const {
-        constexpr int W = R + 3;          // window width in grid samples
+        // Window width in grid samples.  A pixel's cell is (int)(x - tx) in FP32, which
+        // rounding can push one past (int)(-tx) + x (e.g. -tx = 2.9999990: x = 19 gives
+        // 19 + 2.9999990 = 22.0), so cells run from i0 to i0 + R and the 4x4 stencil to
+        // i0 + R + 3: R + 4 samples.
+        constexpr int W = R + 4;
         constexpr int NE = W * W;         // window entries
         constexpr int PE = (NE + G - 1) / G;
         using L = FlatLayout<R, G>;
@@ -826,7 +831,8 @@ struct SplineGridModelT : SplineCommon<BG> {
         float* E = lds;
         float* Ez = lds + NE;
         // z-contraction of the window: rows j0.., columns i0.. (clamped to the grid;
-        // entries past the last cell are never read by a pixel).  Two entries per
+        // entries past the last cell are never read by a pixel).  Every read offset
+        // (cell - origin) * W + stencil stays below W * W.  Two entries per
         // trip keep 8 loads in flight per lane without holding the whole window.
 #pragma unroll 2
         for (int u = 0; u < PE; u++) {
@@ -840,11 +846,12 @@ struct SplineGridModelT : SplineCommon<BG> {
                 Ez[e] = v0 * F.wdz[0] + v1 * F.wdz[1] + v2 * F.wdz[2] + v3 * F.wdz[3];
             }
         }
-        // the group is inside one wavefront, whose LDS operations complete in order:
-        // only the compiler has to be kept from moving the reads above the writes
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        // the group is inside one wavefront: its writes must have completed before any
+        // lane reads the tables (workgroup-scope fences wait for the wave's LDS
+        // operations; no workgroup barrier is needed)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 #pragma unroll
         for (int s = 0; s < L::NS; s++) {
             const int p = L::index(g, s);
@@ -875,10 +882,10 @@ struct SplineGridModelT : SplineCommon<BG> {
             // many pixels and runs out of VGPRs
             __builtin_amdgcn_sched_barrier(0);
         }
-        // the next call overwrites the tables: every lane's reads come first
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        // the next call overwrites the tables: every lane's reads complete first
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
 };
 

@@ -374,3 +374,41 @@ def test_error_against_fp64_fit(ctx, kind):
     assert p99[iI] <= TOL["rel"] and p99[iI + 1] <= TOL["rel"]
     if _zcol(kind):
         assert p99[2] <= TOL["z"]
+
+
+def test_grid_spline_window_covers_rounded_cells(ctx):
+    """A pixel's spline cell is (int)(y - ty) in FP32; for positions just above an
+    integer (e.g. y = 10.000001, so -ty = 2.9999990) the sum rounds up and the cell lands
+    one past the window origin + y.  The grid form's LDS window must cover that cell:
+    its fused CRLB (per-pixel evaluation from the window) agrees with the stand-alone
+    CRLB kernel (evaluation from the grid in HBM), and repeated fits are identical."""
+    import torch
+    est, _, _ = MD.make("spline", ctx)
+    assert est.spline_form() == "grid"
+    rng = np.random.default_rng(41)
+    vals = np.array([k + d for k in (8, 9, 10, 11) for d in (1e-7, 5e-7, 1e-6, 2e-6, 0.3)], np.float32)
+    n = 32768
+    th = np.zeros((n, 5), np.float32)
+    th[:, 0] = rng.choice(vals, n)
+    th[:, 1] = rng.choice(vals, n)
+    th[:, 2] = rng.uniform(-300, 300, n)
+    th[:, 3] = rng.uniform(1000, 5000, n)
+    th[:, 4] = rng.choice(np.array([1e-6, 2.0], np.float32), n)
+    smp = torch.poisson(est.expected_value_device(torch.from_numpy(th).cuda()))
+    d_th = torch.from_numpy(th).cuda()
+    est.SetLevMarParams(1e-18, 0)
+    fused = est.estimate_device(smp, initial=d_th, crlb=True)
+    c, x = est.crlb_device(d_th, smp)
+    torch.cuda.synchronize()
+    a = torch.cat([fused["crlb"], fused["chisq"][:, None]], 1).cpu().numpy()
+    b = torch.cat([c, x[:, None]], 1).cpu().numpy()
+    fin = np.all(np.isfinite(b), axis=1)
+    assert fin.mean() > 0.9
+    # the two kernels sum in different orders, which near-singular Fisher matrices
+    # (bg = 1e-6) amplify; a wrong window entry moves a spot's CRLB by orders of magnitude
+    close = np.all(np.isclose(a, b, rtol=1e-2, atol=1e-6), axis=1)
+    assert close[fin].mean() > 0.999, f"{(~close[fin]).sum()} of {fin.sum()} spots differ"
+    est.SetLevMarParams(1e-18, 50)
+    p1 = est.estimate_device(smp, crlb=False)["params"].cpu().numpy()
+    p2 = est.estimate_device(smp, crlb=False)["params"].cpu().numpy()
+    assert np.array_equal(p1, p2, equal_nan=True)
