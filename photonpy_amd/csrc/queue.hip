@@ -49,6 +49,77 @@ T* device_alloc(size_t n) {
     return static_cast<T*>(p);
 }
 
+// Copies large host buffers with a few helper threads (the calling thread helps):
+// Schedule's copy of the caller's ROIs into pinned batches is bound by one core's
+// memcpy rate otherwise.  One copy at a time; small copies stay on the caller.
+class CopyPool {
+public:
+    explicit CopyPool(int helpers) {
+        for (int i = 0; i < helpers; i++) threads_.emplace_back([this] { work(); });
+    }
+    ~CopyPool() {
+        {
+            std::lock_guard<std::mutex> l(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : threads_) t.join();
+    }
+    void copy(void* dst, const void* src, size_t bytes) {
+        if (threads_.empty() || bytes < kMinParallel) {
+            std::memcpy(dst, src, bytes);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> l(m_);
+            dst_ = static_cast<char*>(dst);
+            src_ = static_cast<const char*>(src);
+            bytes_ = bytes;
+            next_ = 0;
+            busy_ = (int)threads_.size();
+            gen_++;
+        }
+        cv_.notify_all();
+        run_chunks();
+        std::unique_lock<std::mutex> l(m_);
+        done_.wait(l, [&] { return busy_ == 0; });
+    }
+
+private:
+    static constexpr size_t kMinParallel = 1 << 20, kChunk = 256 << 10;
+    void run_chunks() {
+        for (;;) {
+            const size_t o = next_.fetch_add(kChunk);
+            if (o >= bytes_) return;
+            std::memcpy(dst_ + o, src_ + o, std::min(kChunk, bytes_ - o));
+        }
+    }
+    void work() {
+        unsigned long long seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> l(m_);
+                cv_.wait(l, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            run_chunks();
+            std::lock_guard<std::mutex> l(m_);
+            if (--busy_ == 0) done_.notify_all();
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    char* dst_ = nullptr;
+    const char* src_ = nullptr;
+    size_t bytes_ = 0;
+    std::atomic<size_t> next_{0};
+    int busy_ = 0;
+    unsigned long long gen_ = 0;
+    bool stop_ = false;
+};
+
 }  // namespace
 
 struct EstimationQueue {
@@ -104,6 +175,7 @@ struct EstimationQueue {
     std::condition_variable todoCv;  // todo or numActive changed, or stop (todoMtx)
     std::unique_ptr<Batch> next;
 
+    CopyPool copier{std::max(0, std::min(7, (int)std::thread::hardware_concurrency() / 2 - 1))};
     std::thread worker;
     std::atomic<bool> stop{false};
     std::string workerError;  // first launch failure, reported by later calls
@@ -293,15 +365,21 @@ struct EstimationQueue {
         }
         useInitial = initial != nullptr;
         std::lock_guard<std::mutex> l(scheduleMtx);
-        for (int i = 0; i < count; i++) {
+        // runs of spots that fit in the batch being filled, copied in bulk
+        for (int i = 0; i < count;) {
             Batch& b = *next;
-            const int j = b.numspots++;
-            b.ids[j] = ids ? ids[i] : 0;
-            std::memcpy(b.samples + (size_t)j * S, samples + (size_t)i * S, sizeof(float) * S);
-            b.roipos[2 * j] = roipos ? roipos[2 * i] : 0;
-            b.roipos[2 * j + 1] = roipos ? roipos[2 * i + 1] : 0;
-            if (initial) std::memcpy(b.params + (size_t)j * K, initial + (size_t)i * K, sizeof(float) * K);
-            if (NC > 0) std::memcpy(b.constants + (size_t)j * NC, constants + (size_t)i * NC, sizeof(float) * NC);
+            const int j = b.numspots;
+            const int m = std::min(count - i, batchSize - j);
+            for (int u = 0; u < m; u++) {
+                b.ids[j + u] = ids ? ids[i + u] : 0;
+                b.roipos[2 * (j + u)] = roipos ? roipos[2 * (i + u)] : 0;
+                b.roipos[2 * (j + u) + 1] = roipos ? roipos[2 * (i + u) + 1] : 0;
+            }
+            copier.copy(b.samples + (size_t)j * S, samples + (size_t)i * S, sizeof(float) * S * m);
+            if (initial) std::memcpy(b.params + (size_t)j * K, initial + (size_t)i * K, sizeof(float) * K * m);
+            if (NC > 0) copier.copy(b.constants + (size_t)j * NC, constants + (size_t)i * NC, sizeof(float) * NC * m);
+            b.numspots = j + m;
+            i += m;
             if (b.numspots == batchSize) flush_locked();
         }
     }
