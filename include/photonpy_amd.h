@@ -178,6 +178,11 @@ Estimator* GLRT_CreatePSF(Estimator* model, Context* ctx);
 /* ---------------------------------------------------------------------- */
 /* MI355X additions (no reference counterpart)                              */
 
+/* Lanes per spot the queue's batch fits use: the estimator's own group size, or for
+ * batches of a few thousand spots up to 16 (batchSize x lanes <= 65536) where a kernel is
+ * compiled, so a small batch fills more of the GPU.  Results then differ from a direct fit
+ * with the estimator's own group size in summation order only. */
+int EstimQueue_GroupSize(EstimationQueue* q);
 /* GLRT wrapper: 1 (default) = the reference's mean of the call's first spot, 0 = each
  * spot's own mean.  Returns 0, or 1 when e is not a GLRT wrapper. */
 int GLRT_SetMeanOfFirstSpot(Estimator* e, int enable);

@@ -154,9 +154,15 @@ static Estimator* make_gauss(int kind, int roisize, const GaussCalib& gc) {
 }
 
 // ---------------------------------------------------------------- launch helpers
+int fit_group(Estimator* e, int groupHint) {
+    for (int G = groupHint; G > e->G(); G /= 2)
+        if (fit_compiled(e->desc, e->roisize, G)) return G;
+    return e->G();
+}
+
 void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constants, const float* d_initial,
              float* d_params, int* d_iter, float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st,
-             bool timing) {
+             bool timing, int groupHint) {
     e->ensure_device();
     const int R = e->roisize;
     if (R < kMinFitR || R > kMaxFitR)
@@ -189,10 +195,11 @@ void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constant
         }
         events = e->phaseEvents.data() + e->phaseEvents.size() - 3;
     }
-    hipError_t err = launch_fit_any(e->desc, R, e->G(), a, events, st);
+    const int G = fit_group(e, groupHint);
+    hipError_t err = launch_fit_any(e->desc, R, G, a, events, st);
     if (err == hipErrorInvalidValue)
         throw std::runtime_error("photonpy_amd: no fit kernel compiled for roisize " + std::to_string(R) + ", group " +
-                                 std::to_string(e->G()));
+                                 std::to_string(G));
     hip_check(err, "fit kernel launch");
 }
 

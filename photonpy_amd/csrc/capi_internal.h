@@ -210,6 +210,12 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // Fused fit (+ CRLB / chi^2 when d_crlb / d_chisq are given) of n spots on st with the
 // estimator's model, LM settings, lane-group size, refill batch and next work counter;
 // timing = false keeps phase-timing events out (graph capture).
+// Lanes per spot of a fit: the largest compiled group size from groupHint down to
+// the estimator's own (its own when groupHint <= its own).
+int fit_group(Estimator* e, int groupHint);
+// groupHint > 0: try that many lanes per spot first (halving down to the estimator's
+// own group size while no kernel is compiled for it) -- small batches fill more of
+// the GPU with more lanes per spot.
 void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constants, const float* d_initial,
              float* d_params, int* d_iter, float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st,
-             bool timing = true);
+             bool timing = true, int groupHint = 0);

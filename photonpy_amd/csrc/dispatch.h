@@ -80,6 +80,10 @@ inline int default_group_for(const ModelDesc& d, int R) {
 
 constexpr int kMinFitR = 3, kMaxFitR = 32;
 
+// Whether a fit kernel is compiled for (R, G) for this model and form (the size
+// lists of fit_sizes.h).
+bool fit_compiled(const ModelDesc& d, int R, int G);
+
 // Per-model launchers, defined in fit_<model>_<part>.hip.  Return hipErrorInvalidValue
 // when no kernel is compiled for (R, G).  `events` is null or three events that are
 // recorded on the stream before the fit kernel, after it and after the CRLB kernel.

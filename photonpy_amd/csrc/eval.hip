@@ -1,6 +1,7 @@
 // Launchers of the any-size kernels: expected value / derivatives, stand-alone
 // CRLB + chi^2, and the spline coefficient solve.
 #include "dispatch.h"
+#include "fit_sizes.h"
 
 namespace ppa {
 
@@ -60,6 +61,26 @@ hipError_t launch_ev_any(const ModelDesc& d, int R, int n, const float* params, 
 hipError_t launch_crlb_any(const ModelDesc& d, int R, int n, const float* params, const float* constants,
                            const float* sample, float* crlb, float* chisq, hipStream_t st) {
     return with_model(d, [&](const auto& m) { return crlb_t(m, R, n, params, constants, sample, crlb, chisq, st); });
+}
+
+bool fit_compiled(const ModelDesc& d, int R, int G) {
+#define PPA_HAS(R_, G_) \
+    if (R == R_ && G == G_) return true;
+    const bool variant = !(d.kind == kKindFixed || d.kind == kKindSigma || d.kind == kKindAstig || d.kind == kKindSpline);
+    if (is_spline(d.kind) && d.useGrid) {
+        if (variant) {
+            PPA_GRID_VAR_SIZES_0(PPA_HAS) PPA_GRID_VAR_SIZES_1(PPA_HAS) PPA_GRID_VAR_SIZES_2(PPA_HAS)
+            PPA_GRID_VAR_SIZES_3(PPA_HAS)
+        } else {
+            PPA_GRID_SIZES_0(PPA_HAS) PPA_GRID_SIZES_1(PPA_HAS) PPA_GRID_SIZES_2(PPA_HAS) PPA_GRID_SIZES_3(PPA_HAS)
+        }
+    } else if (variant) {
+        PPA_VAR_SIZES_0(PPA_HAS) PPA_VAR_SIZES_1(PPA_HAS) PPA_VAR_SIZES_2(PPA_HAS) PPA_VAR_SIZES_3(PPA_HAS)
+    } else {
+        PPA_FIT_SIZES_0(PPA_HAS) PPA_FIT_SIZES_1(PPA_HAS) PPA_FIT_SIZES_2(PPA_HAS) PPA_FIT_SIZES_3(PPA_HAS)
+    }
+#undef PPA_HAS
+    return false;
 }
 
 hipError_t launch_glrt_any(const ModelDesc& d, int R, int n, const float* params, const float* constants,

@@ -165,6 +165,10 @@ struct EstimationQueue {
     Context* ctx;
     int device = 0;
     int batchSize, maxQueueLen, K, S, NC, D;
+    // Lanes per spot for the queue's batches: a batch of a few thousand spots fills a
+    // small part of the GPU, so each spot gets up to 16 lanes while batch x lanes stays
+    // within 64K (the fit falls back to the estimator's own size where none is compiled).
+    int groupHint = 0;
     bool keepSamples;
     std::atomic<bool> useInitial{false};
     std::vector<Lane> lanes;
@@ -204,6 +208,8 @@ struct EstimationQueue {
             ln.diag = device_alloc<float>((size_t)batchSize * D);
         }
         next = std::make_unique<Batch>(batchSize, S, K, NC, D);
+        for (groupHint = 16; groupHint > 1 && (long)batchSize * groupHint > 65536; groupHint /= 2) {
+        }
         worker = std::thread([this]() { run(); });
     }
 
@@ -239,8 +245,7 @@ struct EstimationQueue {
         {
             std::lock_guard<std::mutex> l(est->mtx);
             run_fit(est, n, ln.samples, NC > 0 ? ln.constants : nullptr, init ? ln.params : nullptr, ln.params,
-                    ln.iterations, nullptr, 0, ln.crlb,
-                    ln.chisq, ln.stream, false);
+                    ln.iterations, nullptr, 0, ln.crlb, ln.chisq, ln.stream, false, groupHint);
             if (D > 0)
                 hip_check(launch_glrt_any(est->desc, est->roisize, n, ln.params, NC > 0 ? ln.constants : nullptr,
                                           ln.samples, outer->glrtMeanOfSpot0, ln.diag, ln.stream),
@@ -477,6 +482,11 @@ bool EstimQueue_IsIdle(EstimationQueue* q) { return q ? q->is_idle() : true; }
 int EstimQueue_GetResultCount(EstimationQueue* q) { return q ? q->result_count() : 0; }
 
 int EstimQueue_GetQueueLength(EstimationQueue* q) { return q ? q->queue_length() : 0; }
+
+int EstimQueue_GroupSize(EstimationQueue* q) {
+    if (!q) return 0;
+    return guarded([&]() -> int { return fit_group(q->est, q->groupHint); }, 0);
+}
 
 int EstimQueue_GetResults(EstimationQueue* q, int maxresults, float* estim, float* diag, float* crlb, int* roipos,
                           float* samples, EstimatorResult* result) {

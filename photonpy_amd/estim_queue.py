@@ -142,6 +142,10 @@ class EstimQueue:
     def GetQueueLength(self):
         return self._lib.EstimQueue_GetQueueLength(self.inst)
 
+    def GroupSize(self):
+        """Lanes per spot of the queue's batch fits (no reference counterpart)."""
+        return self._lib.EstimQueue_GroupSize(self.inst)
+
     def GetResultCount(self):
         return self._lib.EstimQueue_GetResultCount(self.inst)
 

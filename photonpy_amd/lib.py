@@ -141,6 +141,7 @@ def _bind(lib):
         "EstimQueue_IsIdle": ([V], ct.c_bool),
         "EstimQueue_GetResultCount": ([V], ct.c_int),
         "EstimQueue_GetQueueLength": ([V], ct.c_int),
+        "EstimQueue_GroupSize": ([V], ct.c_int),
         "EstimQueue_GetResults": ([V, I, FloatArray, NullableFloatArrayType, NullableFloatArrayType, IntArray,
                                    NullableFloatArrayType, ctl.ndpointer(EstimResultDType, flags="aligned, c_contiguous")],
                                   ct.c_int),

@@ -34,11 +34,15 @@ def test_queue_matches_direct_fit(ctx, kind):
     th, smp = MD.samples(kind, n, 41)
     est, _, _ = MD.make(kind, ctx)
     est.SetLevMarParams(1e-18, 50)
-    ref = _direct(est, smp)
     rng = np.random.default_rng(0)
     ids = rng.permutation(n).astype(np.int32)
     roipos = np.stack([ids, 2 * ids], 1).astype(np.int32)
     with pa.EstimQueue(est, batchSize=512, numStreams=3) as q:
+        # small batches fit with more lanes per spot where compiled (R=11: 4, not 2)
+        G = q.GroupSize()
+        assert G == (4 if kind == "sigma" else est.GetGroupSize())
+        est.SetGroupSize(G)
+        ref = _direct(est, smp)
         pos = 0
         while pos < n:  # uneven chunks, so batches straddle Schedule calls
             m = int(rng.integers(1, 1200))
@@ -64,8 +68,9 @@ def test_queue_initial_values_and_samples(ctx):
     th, smp = MD.samples("fixed", 700, 43)
     est, _, _ = MD.make("fixed", ctx)
     initial = (th * 1.02).astype(np.float32)
-    ref = _direct(est, smp, initial)
     with pa.EstimQueue(est, batchSize=128, numStreams=2, keepSamples=True) as q:
+        est.SetGroupSize(q.GroupSize())
+        ref = _direct(est, smp, initial)
         q.Schedule(smp, ids=np.arange(700), initial=initial)
         q.WaitUntilDone()
         r = q.GetResults(getSampleData=True)
