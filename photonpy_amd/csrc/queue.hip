@@ -51,7 +51,8 @@ T* device_alloc(size_t n) {
 
 // Copies large host buffers with a few helper threads (the calling thread helps):
 // Schedule's copy of the caller's ROIs into pinned batches is bound by one core's
-// memcpy rate otherwise.  One copy at a time; small copies stay on the caller.
+// memcpy rate otherwise.  One pool per process (copy_pool()); copies from several
+// queues take turns; small copies stay on the caller.
 class CopyPool {
 public:
     explicit CopyPool(int helpers) {
@@ -70,6 +71,7 @@ public:
             std::memcpy(dst, src, bytes);
             return;
         }
+        std::lock_guard<std::mutex> one(turn_);
         {
             std::lock_guard<std::mutex> l(m_);
             dst_ = static_cast<char*>(dst);
@@ -109,6 +111,7 @@ private:
         }
     }
     std::vector<std::thread> threads_;
+    std::mutex turn_;  // one parallel copy at a time
     std::mutex m_;
     std::condition_variable cv_, done_;
     char* dst_ = nullptr;
@@ -119,6 +122,11 @@ private:
     unsigned long long gen_ = 0;
     bool stop_ = false;
 };
+
+CopyPool& copy_pool() {
+    static CopyPool pool(std::max(0, std::min(7, (int)std::thread::hardware_concurrency() / 2 - 1)));
+    return pool;
+}
 
 }  // namespace
 
@@ -179,7 +187,6 @@ struct EstimationQueue {
     std::condition_variable todoCv;  // todo or numActive changed, or stop (todoMtx)
     std::unique_ptr<Batch> next;
 
-    CopyPool copier{std::max(0, std::min(7, (int)std::thread::hardware_concurrency() / 2 - 1))};
     std::thread worker;
     std::atomic<bool> stop{false};
     std::string workerError;  // first launch failure, reported by later calls
@@ -380,9 +387,9 @@ struct EstimationQueue {
                 b.roipos[2 * (j + u)] = roipos ? roipos[2 * (i + u)] : 0;
                 b.roipos[2 * (j + u) + 1] = roipos ? roipos[2 * (i + u) + 1] : 0;
             }
-            copier.copy(b.samples + (size_t)j * S, samples + (size_t)i * S, sizeof(float) * S * m);
+            copy_pool().copy(b.samples + (size_t)j * S, samples + (size_t)i * S, sizeof(float) * S * m);
             if (initial) std::memcpy(b.params + (size_t)j * K, initial + (size_t)i * K, sizeof(float) * K * m);
-            if (NC > 0) copier.copy(b.constants + (size_t)j * NC, constants + (size_t)i * NC, sizeof(float) * NC * m);
+            if (NC > 0) copy_pool().copy(b.constants + (size_t)j * NC, constants + (size_t)i * NC, sizeof(float) * NC * m);
             b.numspots = j + m;
             i += m;
             if (b.numspots == batchSize) flush_locked();
