@@ -717,8 +717,9 @@ struct SplineGridModelT : SplineCommon<BG> {
 
     template <int R, int G>
     using Layout = FlatLayout<R, G>;
-    // window of R + 4 grid samples per axis (see for_pixels), two tables
-    static constexpr int lds_floats(int R) { return 2 * (R + 4) * (R + 4); }
+    // window of R + 4 grid samples per axis (see for_pixels), two tables, and the
+    // per-column and per-row cell offsets (2R ints)
+    static constexpr int lds_floats(int R) { return 2 * (R + 4) * (R + 4) + 2 * R; }
 
     // Lagrange weights on the nodes -1, 0, 1, 2 at t, and their derivatives.
     PP_DEV static void lagrange(float t, float (&w)[4], float (&dw)[4]) {
@@ -830,6 +831,17 @@ struct SplineGridModelT : SplineCommon<BG> {
         const size_t zs = (size_t)Dy * Dx;
         float* E = lds;
         float* Ez = lds + NE;
+        // cell offsets of the pixel columns and rows from the window origin, rows times W
+        int* OX = reinterpret_cast<int*>(lds + 2 * NE);
+        int* OY = OX + R;
+#pragma unroll
+        for (int u0 = 0; u0 < 2 * R; u0 += G) {
+            const int u = u0 + g;
+            if (u < R)
+                OX[u] = cell_x(F, u) - i0;
+            else if (u < 2 * R)
+                OY[u - R] = (cell_y(F, u - R) - j0) * W;
+        }
         // z-contraction of the window: rows j0.., columns i0.. (clamped to the grid;
         // entries past the last cell are never read by a pixel).  Every read offset
         // (cell - origin) * W + stencil stays below W * W.  Two entries per
@@ -856,8 +868,10 @@ struct SplineGridModelT : SplineCommon<BG> {
         for (int s = 0; s < L::NS; s++) {
             const int p = L::index(g, s);
             if (L::NS * G == R * R || p < R * R) {
-                const int y = p / R, x = p - (p / R) * R;
-                const int o = (cell_y(F, y) - j0) * W + (cell_x(F, x) - i0);
+                // p / R as a 20-bit multiply-shift, exact for p < 2048 and R <= 32
+                constexpr int kDivR = (1 << 20) / R + 1;
+                const int y = (p * kDivR) >> 20, x = p - y * R;
+                const int o = OY[y] + OX[x];
                 float f = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
 #pragma unroll
                 for (int n = 0; n < 4; n++) {
