@@ -23,13 +23,30 @@ PP_DEV float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 // G <= 64).  Partners combine partial sums commutatively, so every lane of the
 // group ends with bit-identical totals.  Steps inside a row of 16 lanes use DPP
 // moves (quad_perm xor 1 / xor 2, row_half_mirror, row_mirror: each pairs every
-// lane with one in the other half of its block); wider steps use ds_bpermute.
-// The moved value is written with old = 0 and bound_ctrl set (every lane of a
-// group is active, so no lane reads a disabled source); that form lets the
-// compiler fold the move into the add as its DPP source operand.
+// lane with one in the other half of its block); the steps across rows use the
+// gfx950 row swaps (v_permlane16_swap: rows 0-1 and 2-3, v_permlane32_swap: the
+// two halves of the wavefront), which leave both partners' values in every lane.
+// None of them goes through LDS.  The moved value is written with old = 0 and
+// bound_ctrl set (every lane of a group is active, so no lane reads a disabled
+// source); that form lets the compiler fold the move into the add as its DPP
+// source operand.
 template <int CTRL>
 PP_DEV float dpp_add(float v) {
     return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+// {value of the even row, value of the odd row} of this lane's pair of 16-lane rows.
+PP_DEV void row_pair(float v, float& even, float& odd) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    even = __uint_as_float(r[0]);
+    odd = __uint_as_float(r[1]);
+}
+
+// {value of the lower half, value of the upper half} of the wavefront at this lane.
+PP_DEV void half_pair(float v, float& lo, float& hi) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    lo = __uint_as_float(r[0]);
+    hi = __uint_as_float(r[1]);
 }
 
 template <int G>
@@ -38,8 +55,16 @@ PP_DEV float group_sum(float v) {
     if constexpr (G >= 4) v = dpp_add<0x4E>(v);   // quad_perm [2,3,0,1]
     if constexpr (G >= 8) v = dpp_add<0x141>(v);  // row_half_mirror
     if constexpr (G >= 16) v = dpp_add<0x140>(v); // row_mirror
-#pragma unroll
-    for (int m = 16; m < G; m <<= 1) v += __shfl_xor(v, m, kWave);
+    if constexpr (G >= 32) {
+        float a, b;
+        row_pair(v, a, b);
+        v = a + b;
+    }
+    if constexpr (G >= 64) {
+        float a, b;
+        half_pair(v, a, b);
+        v = a + b;
+    }
     return v;
 }
 
@@ -52,7 +77,9 @@ PP_DEV void group_sum_n(float (&v)[N]) {
 }
 
 // Value of v held by lane SRC of this lane's aligned group of G lanes.
-// G <= 4 uses a quad_perm DPP move (a full-rate VALU op), larger groups ds_bpermute.
+// G <= 4 uses a quad_perm DPP move (a full-rate VALU op); larger groups use
+// ds_bpermute, whose lane offset is an immediate, so the broadcasts of a kernel
+// that is bound by VALU issue take no VALU slot.
 template <int G, int SRC>
 PP_DEV float group_bcast(float v) {
     static_assert(SRC >= 0 && SRC < G, "source lane outside the group");

@@ -336,8 +336,21 @@ PP_DEV float* group_lds() {
 // Phase 1: fit.  Phase 2 (CRLB and chi^2 of the fitted spots) is a kernel of its
 // own, launched after it on the same stream, so that it sees every wavefront's
 // results and runs at the occupancy of its own, smaller register set.
+// Minimum wavefronts per SIMD the compiler must fit a model's fit kernel into
+// (its register budget): M::min_waves(R, G) where the model names one, else 1
+// (no bound).
+template <class M, int R, int G, class = void>
+struct MinWaves {
+    static constexpr int value = 1;
+};
 template <class M, int R, int G>
-__global__ void __launch_bounds__(256) fit_kernel(M model, FitArgs a) {
+struct MinWaves<M, R, G, std::void_t<decltype(M::min_waves(R, G))>> {
+    static constexpr int value = M::min_waves(R, G);
+};
+
+template <class M, int R, int G>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves<M, R, G>::value)))
+fit_kernel(M model, FitArgs a) {
     const int groupsPerBlock = blockDim.x / G;
     fit_spots<M, R, G>(model, a, blockIdx.x * groupsPerBlock + threadIdx.x / G, gridDim.x * groupsPerBlock,
                        group_lds<M, R, G>());

@@ -699,9 +699,9 @@ struct SplineModelT : SplineCommon<BG> {
 // the same polynomial as the coefficient form sum_p c_p w^pz v^py u^px.  The
 // grid is [Nz+3][Ny+3][Nx+3] floats (215 KB for a 61x26x26-cell table, which
 // stays in L2) against 64 floats per cell (10.5 MB).  Per spot and iteration the
-// lane group contracts the grid's four z-slices over the spot's window into two
-// LDS tables (value and z-derivative weights, I / du folded in); each pixel then
-// reads its 4x4 entries of both.  Cell indices, the truncating sub-pixel offset
+// lane group contracts the grid's four z-slices over the spot's window into an
+// LDS table of (value, z-derivative) weight pairs (I / du folded in); each pixel
+// then reads its 4x4 pairs.  Cell indices, the truncating sub-pixel offset
 // and the psf >= 0 clamp follow CubicSplinePSFModels.h:64-153 as in SplineModelT.
 template <int BG>
 struct SplineGridModelT : SplineCommon<BG> {
@@ -717,8 +717,16 @@ struct SplineGridModelT : SplineCommon<BG> {
 
     template <int R, int G>
     using Layout = FlatLayout<R, G>;
-    // window of R + 4 grid samples per axis (see for_pixels), two tables, and the
-    // per-column and per-row cell offsets (2R ints)
+    // Five wavefronts per SIMD (<= 96 VGPRs) hide the L2 latency of the window
+    // loads; that register budget holds the flat-background form with groups of
+    // >= 8 lanes and <= 13 pixels per lane without spilling.  Other sizes and the
+    // tilted / image backgrounds keep the compiler's own choice.
+    static constexpr int min_waves(int R, int G) {
+        return (BG == kSplineFlat && G >= 8 && (R * R + G - 1) / G <= 13) ? 5 : 1;
+    }
+    // window of R + 4 grid samples per axis (see for_pixels) as (value, z-derivative)
+    // pairs, and the per-column and per-row cell offsets (2R ints); an even count
+    // keeps every group's pairs 8-byte aligned
     static constexpr int lds_floats(int R) { return 2 * (R + 4) * (R + 4) + 2 * R; }
 
     // Lagrange weights on the nodes -1, 0, 1, 2 at t, and their derivatives.
@@ -829,8 +837,9 @@ struct SplineGridModelT : SplineCommon<BG> {
         const int Dy = Ny + 3, Dx = Nx + 3;
         const float* slab = grid + (size_t)F.k * Dy * Dx;
         const size_t zs = (size_t)Dy * Dx;
-        float* E = lds;
-        float* Ez = lds + NE;
+        // (value, z-derivative) weight pairs: one 8-byte LDS read per stencil entry,
+        // two entries per ds_read2_b64 at immediate offsets from one address
+        float2* T = reinterpret_cast<float2*>(lds);
         // cell offsets of the pixel columns and rows from the window origin, rows times W
         int* OX = reinterpret_cast<int*>(lds + 2 * NE);
         int* OY = OX + R;
@@ -854,8 +863,8 @@ struct SplineGridModelT : SplineCommon<BG> {
                 const int gy = min(j0 + r, Dy - 1), gx = min(i0 + c, Dx - 1);
                 const float* src = slab + (size_t)gy * Dx + gx;
                 const float v0 = src[0], v1 = src[zs], v2 = src[2 * zs], v3 = src[3 * zs];
-                E[e] = v0 * F.wz[0] + v1 * F.wz[1] + v2 * F.wz[2] + v3 * F.wz[3];
-                Ez[e] = v0 * F.wdz[0] + v1 * F.wdz[1] + v2 * F.wdz[2] + v3 * F.wdz[3];
+                T[e] = make_float2(v0 * F.wz[0] + v1 * F.wz[1] + v2 * F.wz[2] + v3 * F.wz[3],
+                                   v0 * F.wdz[0] + v1 * F.wdz[1] + v2 * F.wdz[2] + v3 * F.wdz[3]);
             }
         }
         // the group is inside one wavefront: its writes must have completed before any
@@ -878,7 +887,8 @@ struct SplineGridModelT : SplineCommon<BG> {
                     float a = 0.0f, ad = 0.0f, az = 0.0f;
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
-                        const float e = E[o + n * W + q], ez = Ez[o + n * W + q];
+                        const float2 v = T[o + n * W + q];
+                        const float e = v.x, ez = v.y;
                         a += F.wx[q] * e;
                         ad += F.wdx[q] * e;
                         az += F.wx[q] * ez;
