@@ -23,7 +23,7 @@ TABLE = os.path.join(_HERE, "tuning.json")
 
 _KIND = {"x,y,I,bg": "fixed", "x,y,\tI,bg,sigma": "sigma"}
 # group sizes compiled for each ROI size (csrc/fit_sizes.h)
-COMPILED_GROUPS = {3: [1], 4: [1], 5: [2], 6: [2], 7: [1, 2, 4], 8: [2], 9: [2, 4], 10: [2, 4], 11: [2, 4],
+COMPILED_GROUPS = {3: [1], 4: [1], 5: [2], 6: [2], 7: [1, 2, 4], 8: [2], 9: [2, 4], 10: [2, 4, 8, 16], 11: [2, 4],
                    12: [4], 13: [4, 8], 14: [4], 15: [4], 16: [4, 8], 17: [8], 18: [8], 19: [8], 20: [8, 16],
                    21: [8], 22: [8], 23: [8], 24: [8], 25: [8, 16], 26: [16], 27: [16], 28: [16], 29: [16],
                    30: [16], 31: [16], 32: [16]}

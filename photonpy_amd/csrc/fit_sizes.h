@@ -5,7 +5,7 @@
 #pragma once
 
 #define PPA_FIT_SIZES_0(X) X(3, 1) X(4, 1) X(5, 2) X(6, 2) X(7, 1) X(7, 2) X(7, 4) X(8, 2) X(9, 2) X(9, 4) X(10, 2)
-#define PPA_FIT_SIZES_1(X) X(10, 4) X(11, 2) X(11, 4) X(12, 4) X(13, 4) X(13, 8) X(14, 4) X(15, 4) X(16, 4)
+#define PPA_FIT_SIZES_1(X) X(10, 4) X(10, 8) X(10, 16) X(11, 2) X(11, 4) X(12, 4) X(13, 4) X(13, 8) X(14, 4) X(15, 4) X(16, 4)
 #define PPA_FIT_SIZES_2(X) X(16, 8) X(17, 8) X(18, 8) X(19, 8) X(20, 8) X(20, 16) X(21, 8) X(22, 8) X(23, 8) X(24, 8)
 #define PPA_FIT_SIZES_3(X) X(25, 16) X(25, 8) X(26, 16) X(27, 16) X(28, 16) X(29, 16) X(30, 16) X(31, 16) X(32, 16)
 

@@ -196,7 +196,7 @@ def test_unsupported_roisize_raises(ctx):
     assert est.ExpectedValue(np.array([[20.0, 20.0, 100.0, 1.0]])).shape == (1, 40, 40)
 
 
-@pytest.mark.parametrize("R,G", [(10, 4), (11, 4), (16, 8), (20, 16), (7, 2)])
+@pytest.mark.parametrize("R,G", [(10, 4), (10, 16), (11, 4), (16, 8), (20, 16), (7, 2)])
 def test_group_sizes_agree(ctx, R, G):
     th, smp = MD.samples("sigma", 800, 21, R=R)
     est, _, _ = MD.make("sigma", ctx, R=R)
