@@ -164,6 +164,12 @@ void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constant
              float* d_params, int* d_iter, float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st,
              bool timing, int groupHint) {
     e->ensure_device();
+    // The fit kernel's work counter and the spot strides are 32-bit and run past n by
+    // at most the grid's group count (< 2^21 lane groups on one device).
+    constexpr int kMaxSpots = INT_MAX - (1 << 21);
+    if (n > kMaxSpots)
+        throw std::runtime_error("photonpy_amd: at most " + std::to_string(kMaxSpots) + " spots per fit call, got " +
+                                 std::to_string(n));
     const int R = e->roisize;
     if (R < kMinFitR || R > kMaxFitR)
         throw std::runtime_error("photonpy_amd: fit kernels are compiled for roisize " + std::to_string(kMinFitR) +
