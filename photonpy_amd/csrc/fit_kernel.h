@@ -144,7 +144,10 @@ PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTrai
             for (int j = i; j < K; j++) fi[q++] += w * J[j];
         }
     };
-    model.template for_pixels<R, G>(th, g, lds, pixel_bg<M>(th, pb, body));
+    if constexpr (row_factored<M>())
+        model.template accumulate_rows<R, G, true>(th, g, s, fi);
+    else
+        model.template for_pixels<R, G>(th, g, lds, pixel_bg<M>(th, pb, body));
     group_sum_n<G>(fi);
     if (g == 0) {
         if (a.crlb) {

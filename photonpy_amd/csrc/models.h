@@ -370,18 +370,20 @@ struct GaussModel {
     //   J_x = Dy a, J_y = Ey b, J_I = Dy b, J_bg = 1, J_s = Dy c + Sy b
     // with column terms a = I dEx, b = Dx, c = I Sx and row terms Dy, Ey = I dEy,
     // Sy = I Sy (J_s is the sigma / z column; absent for fixed sigma).  Per pixel
-    // it forms w = x/mu^2 and v = x/mu - 1 and accumulates the row sums
-    // sum w p q and sum v p over p, q in {a, b, c, 1}; per row it adds
-    // alpha_ij += sum_pq Ri_p Rj_q (sum w p q) and beta_i += sum_p Ri_p (sum v p) with
-    // Ri the row coefficients of J_i.  The sums are those of the per-pixel form,
-    // sum w J_i J_j and sum v J_i, regrouped.  acc (packed alpha, i <= j, then beta)
-    // is overwritten: the first row sets it.
-    template <int R, int G, int NS>
-    PP_DEV void accumulate_rows(const float (&t)[KT], int g, const float (&s)[NS],
-                                float (&acc)[K * (K + 1) / 2 + K]) const {
+    // it forms a weight w and accumulates the row sums sum w p q over p, q in
+    // {a, b, c, 1}; per row it adds alpha_ij += sum_pq Ri_p Rj_q (sum w p q) with Ri
+    // the row coefficients of J_i.  The sums are those of the per-pixel form,
+    // sum w J_i J_j, regrouped.  acc is overwritten: the first row sets it.
+    //   FISHER = false (LM step): w = x/mu^2 with mu >= 1e-6, and also
+    //     beta_i = sum (x/mu - 1) J_i; acc = packed alpha (i <= j), then beta.
+    //   FISHER = true (CRLB): w = 1/mu with mu >= 1e-8 (the Fisher matrix), and
+    //     chi^2 = sum (x - mu)^2 / mu; acc = packed Fisher matrix, then chi^2.
+    template <int R, int G, bool FISHER = false, int NS, int NACC>
+    PP_DEV void accumulate_rows(const float (&t)[KT], int g, const float (&s)[NS], float (&acc)[NACC]) const {
         constexpr int RL = RowLayout<R, G>::RL;
         constexpr bool SH = (V == kSigma || V == kAstig);  // has the c column
         constexpr int NA = K * (K + 1) / 2;
+        static_assert(NACC == NA + (FISHER ? 1 : K), "accumulator layout");
         // packed upper-triangle index of alpha(i, j), i <= j
         auto q = [](int i, int j) { return i * K - i * (i - 1) / 2 + (j - i); };
         AxisShape SX, SY;
@@ -405,27 +407,46 @@ struct GaussModel {
                     const float a = col[x].dE, b = col[x].D;
                     const float p = b * row.D;
                     const float mu = bg + I * p;
-                    const float inv = rcpf(fmaxf(mu, 1e-6f));
-                    const float tv = s[r * R + x] * inv;  // samples clamped at load
-                    const float w = tv * inv;
-                    const float v = tv - 1.0f;
+                    float w, v = 0.0f, chi = 0.0f;
+                    if constexpr (FISHER) {
+                        const float m = fmaxf(1e-8f, mu);
+                        const float inv = rcpf(m);
+                        const float e = s[r * R + x] - m;
+                        w = inv;
+                        chi = e * e * inv;
+                    } else {
+                        const float inv = rcpf(fmaxf(mu, 1e-6f));
+                        const float tv = s[r * R + x] * inv;  // samples clamped at load
+                        w = tv * inv;
+                        v = tv - 1.0f;
+                    }
                     const float wa = w * a, wb = w * b;
                     if (x == 0) {
                         Saa = wa * a; Sab = wa * b; Sa1 = wa;
                         Sbb = wb * b; Sb1 = wb; S11 = w;
-                        Ta = v * a; Tb = v * b; T1 = v;
+                        if constexpr (FISHER) {
+                            T1 = chi;
+                        } else {
+                            Ta = v * a; Tb = v * b; T1 = v;
+                        }
                     } else {
                         Saa += wa * a; Sab += wa * b; Sa1 += wa;
                         Sbb += wb * b; Sb1 += wb; S11 += w;
-                        Ta += v * a; Tb += v * b; T1 += v;
+                        if constexpr (FISHER) {
+                            T1 += chi;
+                        } else {
+                            Ta += v * a; Tb += v * b; T1 += v;
+                        }
                     }
                     if constexpr (SH) {
                         const float c = col[x].S;
                         const float wc = w * c;
                         if (x == 0) {
-                            Sac = wa * c; Sbc = wb * c; Scc = wc * c; Sc1 = wc; Tc = v * c;
+                            Sac = wa * c; Sbc = wb * c; Scc = wc * c; Sc1 = wc;
+                            if constexpr (!FISHER) Tc = v * c;
                         } else {
-                            Sac += wa * c; Sbc += wb * c; Scc += wc * c; Sc1 += wc; Tc += v * c;
+                            Sac += wa * c; Sbc += wb * c; Scc += wc * c; Sc1 += wc;
+                            if constexpr (!FISHER) Tc += v * c;
                         }
                     }
                 }
@@ -449,10 +470,14 @@ struct GaussModel {
                 put(q(iI, iI), DD * Sbb);
                 put(q(iI, iBg), D * Sb1);
                 put(q(iBg, iBg), S11);
-                put(NA + X, D * Ta);
-                put(NA + Y, E * Tb);
-                put(NA + iI, D * Tb);
-                put(NA + iBg, T1);
+                if constexpr (FISHER) {
+                    put(NA, T1);
+                } else {
+                    put(NA + X, D * Ta);
+                    put(NA + Y, E * Tb);
+                    put(NA + iI, D * Tb);
+                    put(NA + iBg, T1);
+                }
                 if constexpr (SH) {
                     // J_s = D c + Sy b
                     const float Sy = row.S;
@@ -463,11 +488,11 @@ struct GaussModel {
                     put(q(iI < s0 ? iI : s0, iI < s0 ? s0 : iI), DS * Sbb + DD * Sbc);
                     put(q(iBg < s0 ? iBg : s0, iBg < s0 ? s0 : iBg), Sy * Sb1 + D * Sc1);
                     put(q(s0, s0), (Sy * Sy) * Sbb + (2.0f * DS) * Sbc + DD * Scc);
-                    put(NA + s0, Sy * Tb + D * Tc);
+                    if constexpr (!FISHER) put(NA + s0, Sy * Tb + D * Tc);
                 }
             } else if (r == 0) {
 #pragma unroll
-                for (int i = 0; i < NA + K; i++) acc[i] = 0.0f;
+                for (int i = 0; i < NACC; i++) acc[i] = 0.0f;
             }
         }
     }
