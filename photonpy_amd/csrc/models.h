@@ -53,6 +53,23 @@ struct RowLayout {
     PP_DEV static int y(int g, int s) { return g * RL + s / R; }
 };
 
+// SplitRowLayout (two lanes, odd R): RowLayout would give each lane (R+1)/2 rows
+// and leave one row of slots empty.  Here lane g owns H = (R-1)/2 whole rows (lane
+// 0 rows [0, H), lane 1 rows [H+1, R)) in slots [0, H*R), and half of the middle
+// row H in slots H*R + j: lane 0 columns j in [0, M), lane 1 columns M + j,
+// M = (R+1)/2.  Lane 1's last slot (column R) is padding: index() puts it past
+// the ROI.
+template <int R>
+struct SplitRowLayout {
+    static_assert(R % 2 == 1, "odd ROI sizes only");
+    static constexpr int H = (R - 1) / 2, M = (R + 1) / 2;
+    static constexpr int NS = H * R + M;
+    static constexpr bool kStrided = false;
+    PP_DEV static int y(int g, int s) { return s < H * R ? g * (H + 1) + s / R : H; }
+    PP_DEV static int x(int g, int s) { return s < H * R ? s % R : g * M + (s - H * R); }
+    PP_DEV static int index(int g, int s) { return x(g, s) < R ? y(g, s) * R + x(g, s) : R * R; }
+};
+
 template <int R, int G>
 struct FlatLayout {
     static constexpr int NS = (R * R + G - 1) / G;
@@ -319,14 +336,18 @@ struct GaussModel {
               (float)x, (float)y, (float)R * 0.5f, t, mu, J);
     }
 
+    // The row-factored models split the middle row of an odd ROI between two lanes.
     template <int R, int G>
-    using Layout = RowLayout<R, G>;
+    static constexpr bool kSplit = kRowFactored && G == 2 && R % 2 == 1;
+    template <int R, int G>
+    using Layout = std::conditional_t<kSplit<R, G>, SplitRowLayout<R>, RowLayout<R, G>>;
     static constexpr int lds_floats(int) { return 0; }
 
     // Fit-kernel evaluation of lane g's pixels: fn(slot, mu, J) for every slot of
     // RowLayout<R, G> that lies inside the ROI.
     template <int R, int G, class Fn>
     PP_DEV void for_pixels(const float (&t)[KT], int g, float*, Fn&& fn) const {
+        static_assert(!kSplit<R, G>, "the split layout is accumulated by accumulate_rows");
         constexpr int RL = RowLayout<R, G>::RL;
         for_rows<R, RL, G>(t, g * RL, [&](int r, int x, float mu, const float (&J)[K]) { fn(r * R + x, mu, J); });
     }
@@ -365,8 +386,121 @@ struct GaussModel {
         }
     }
 
-    // Fit-kernel normal equations of lane g's rows (RowLayout<R, G>), for the
-    // models whose Jacobian columns are sums of (row term) x (column term):
+    // Row sums of accumulate_rows: sum w p q over p, q in {a, b, c, 1} and sum v p
+    // (FISHER: T1 holds chi^2 and the other T sums are unused).
+    struct RowSums {
+        float Saa, Sab, Sa1, Sbb, Sb1, S11, Ta, Tb, T1;
+        float Sac = 0.0f, Sbc = 0.0f, Scc = 0.0f, Sc1 = 0.0f, Tc = 0.0f;
+    };
+
+    // Sums over the N pixels j of one row (or row part) with row term D; pix(j, a, b,
+    // c, x, ok) gives pixel j's column terms, sample and whether it is a pixel at all.
+    template <bool FISHER, int N, class Pix>
+    PP_DEV static void row_sums(float I, float bg, float D, Pix&& pix, RowSums& z) {
+        constexpr bool SH = (V == kSigma || V == kAstig);  // has the c column
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            float a, b, c, xs;
+            bool ok;
+            pix(j, a, b, c, xs, ok);
+            const float p = b * D;
+            const float mu = bg + I * p;
+            float w, v = 0.0f, chi = 0.0f;
+            if constexpr (FISHER) {
+                const float m = fmaxf(1e-8f, mu);
+                const float inv = rcpf(m);
+                const float e = xs - m;
+                w = inv;
+                chi = e * e * inv;
+            } else {
+                const float inv = rcpf(fmaxf(mu, 1e-6f));
+                const float tv = xs * inv;  // samples clamped at load
+                w = tv * inv;
+                v = tv - 1.0f;
+            }
+            if (!ok) w = v = chi = 0.0f;  // padding slot (folds away where ok is constant)
+            const float wa = w * a, wb = w * b;
+            if (j == 0) {
+                z.Saa = wa * a; z.Sab = wa * b; z.Sa1 = wa;
+                z.Sbb = wb * b; z.Sb1 = wb; z.S11 = w;
+                if constexpr (FISHER) {
+                    z.T1 = chi;
+                } else {
+                    z.Ta = v * a; z.Tb = v * b; z.T1 = v;
+                }
+            } else {
+                z.Saa += wa * a; z.Sab += wa * b; z.Sa1 += wa;
+                z.Sbb += wb * b; z.Sb1 += wb; z.S11 += w;
+                if constexpr (FISHER) {
+                    z.T1 += chi;
+                } else {
+                    z.Ta += v * a; z.Tb += v * b; z.T1 += v;
+                }
+            }
+            if constexpr (SH) {
+                const float wc = w * c;
+                if (j == 0) {
+                    z.Sac = wa * c; z.Sbc = wb * c; z.Scc = wc * c; z.Sc1 = wc;
+                    if constexpr (!FISHER) z.Tc = v * c;
+                } else {
+                    z.Sac += wa * c; z.Sbc += wb * c; z.Scc += wc * c; z.Sc1 += wc;
+                    if constexpr (!FISHER) z.Tc += v * c;
+                }
+            }
+        }
+    }
+
+    // alpha_ij += sum_pq Ri_p Rj_q (sum w p q), beta_i += sum_p Ri_p (sum v p) for one
+    // row (FISHER: the Fisher matrix and chi^2); the first row sets acc.
+    template <bool FISHER, int NACC>
+    PP_DEV static void fold_row(const AxisTerms& row, const RowSums& z, bool first, float (&acc)[NACC]) {
+        constexpr bool SH = (V == kSigma || V == kAstig);
+        constexpr int NA = K * (K + 1) / 2;
+        // packed upper-triangle index of alpha(i, j), i <= j
+        auto q = [](int i, int j) { return i * K - i * (i - 1) / 2 + (j - i); };
+        auto put = [&](int i, float val) {
+            if (first)
+                acc[i] = val;
+            else
+                acc[i] += val;
+        };
+        const float D = row.D, E = row.dE;
+        const float DD = D * D, DE = D * E;
+        constexpr int X = 0, Y = 1;
+        put(q(X, X), DD * z.Saa);
+        put(q(X, Y), DE * z.Sab);
+        put(q(X, iI), DD * z.Sab);
+        put(q(X, iBg), D * z.Sa1);
+        put(q(Y, Y), (E * E) * z.Sbb);
+        put(q(Y, iI), DE * z.Sbb);
+        put(q(Y, iBg), E * z.Sb1);
+        put(q(iI, iI), DD * z.Sbb);
+        put(q(iI, iBg), D * z.Sb1);
+        put(q(iBg, iBg), z.S11);
+        if constexpr (FISHER) {
+            put(NA, z.T1);
+        } else {
+            put(NA + X, D * z.Ta);
+            put(NA + Y, E * z.Tb);
+            put(NA + iI, D * z.Tb);
+            put(NA + iBg, z.T1);
+        }
+        if constexpr (SH) {
+            // J_s = D c + Sy b
+            const float Sy = row.S;
+            const float DS = D * Sy;
+            constexpr int s0 = iS;
+            put(q(X < s0 ? X : s0, X < s0 ? s0 : X), DS * z.Sab + DD * z.Sac);
+            put(q(Y < s0 ? Y : s0, Y < s0 ? s0 : Y), (E * Sy) * z.Sbb + DE * z.Sbc);
+            put(q(iI < s0 ? iI : s0, iI < s0 ? s0 : iI), DS * z.Sbb + DD * z.Sbc);
+            put(q(iBg < s0 ? iBg : s0, iBg < s0 ? s0 : iBg), Sy * z.Sb1 + D * z.Sc1);
+            put(q(s0, s0), (Sy * Sy) * z.Sbb + (2.0f * DS) * z.Sbc + DD * z.Scc);
+            if constexpr (!FISHER) put(NA + s0, Sy * z.Tb + D * z.Tc);
+        }
+    }
+
+    // Fit-kernel normal equations of lane g's rows (Layout<R, G>), for the models
+    // whose Jacobian columns are sums of (row term) x (column term):
     //   J_x = Dy a, J_y = Ey b, J_I = Dy b, J_bg = 1, J_s = Dy c + Sy b
     // with column terms a = I dEx, b = Dx, c = I Sx and row terms Dy, Ey = I dEy,
     // Sy = I Sy (J_s is the sigma / z column; absent for fixed sigma).  Per pixel
@@ -380,119 +514,70 @@ struct GaussModel {
     //     chi^2 = sum (x - mu)^2 / mu; acc = packed Fisher matrix, then chi^2.
     template <int R, int G, bool FISHER = false, int NS, int NACC>
     PP_DEV void accumulate_rows(const float (&t)[KT], int g, const float (&s)[NS], float (&acc)[NACC]) const {
-        constexpr int RL = RowLayout<R, G>::RL;
-        constexpr bool SH = (V == kSigma || V == kAstig);  // has the c column
         constexpr int NA = K * (K + 1) / 2;
         static_assert(NACC == NA + (FISHER ? 1 : K), "accumulator layout");
-        // packed upper-triangle index of alpha(i, j), i <= j
-        auto q = [](int i, int j) { return i * K - i * (i - 1) / 2 + (j - i); };
+        static_assert(NS == Layout<R, G>::NS, "sample slots of the layout");
         AxisShape SX, SY;
         shapes(t, SX, SY);
         AxisTerms col[R];
         columns<R, G>(t, SX, col);
-        const int y0 = g * RL;
         const float I = t[iI], bg = t[iBg];
-        Edge prev = gauss_edge((float)y0, t[1], SY.c2);
+        if constexpr (kSplit<R, G>) {
+            constexpr int H = SplitRowLayout<R>::H, M = SplitRowLayout<R>::M;
+            const int y0 = g * (H + 1);
+            const Edge first = gauss_edge((float)y0, t[1], SY.c2);
+            Edge prev = first;
 #pragma unroll
-        for (int r = 0; r < RL; r++) {
-            const int y = y0 + r;
-            Edge nx = gauss_edge((float)(y + 1), t[1], SY.c2);
-            const AxisTerms row = scaled(axis_terms(prev, nx, SY.cp, SY.cs), I);
-            prev = nx;
-            if (RL * G == R || y < R) {
-                float Saa, Sab, Sa1, Sbb, Sb1, S11, Ta, Tb, T1;
-                float Sac = 0.0f, Sbc = 0.0f, Scc = 0.0f, Sc1 = 0.0f, Tc = 0.0f;
+            for (int r = 0; r < H; r++) {
+                const Edge nx = gauss_edge((float)(y0 + r + 1), t[1], SY.c2);
+                const AxisTerms row = scaled(axis_terms(prev, nx, SY.cp, SY.cs), I);
+                prev = nx;
+                RowSums z;
+                row_sums<FISHER, R>(I, bg, row.D, [&](int x, float& a, float& b, float& c, float& xs, bool& ok) {
+                    a = col[x].dE; b = col[x].D; c = col[x].S; xs = s[r * R + x]; ok = true;
+                }, z);
+                fold_row<FISHER>(row, z, r == 0, acc);
+            }
+            // the middle row H: lane 0 holds edge H (its last) and lane 1 edge H + 1 (its
+            // first); each evaluates the other one
+            const Edge en = gauss_edge((float)(g == 0 ? H + 1 : H), t[1], SY.c2);
+            const Edge lo = g == 0 ? prev : en, hi = g == 0 ? en : first;
+            const AxisTerms row = scaled(axis_terms(lo, hi, SY.cp, SY.cs), I);
+            RowSums z;
+            row_sums<FISHER, M>(I, bg, row.D, [&](int j, float& a, float& b, float& c, float& xs, bool& ok) {
+                if (j < M - 1) {
+                    const bool hiHalf = g != 0;
+                    a = hiHalf ? col[M + j].dE : col[j].dE;
+                    b = hiHalf ? col[M + j].D : col[j].D;
+                    c = hiHalf ? col[M + j].S : col[j].S;
+                    ok = true;
+                } else {  // column M - 1 for lane 0, padding for lane 1
+                    a = col[j].dE; b = col[j].D; c = col[j].S;
+                    ok = g == 0;
+                }
+                xs = s[H * R + j];
+            }, z);
+            fold_row<FISHER>(row, z, H == 0, acc);
+        } else {
+            constexpr int RL = RowLayout<R, G>::RL;
+            const int y0 = g * RL;
+            Edge prev = gauss_edge((float)y0, t[1], SY.c2);
 #pragma unroll
-                for (int x = 0; x < R; x++) {
-                    const float a = col[x].dE, b = col[x].D;
-                    const float p = b * row.D;
-                    const float mu = bg + I * p;
-                    float w, v = 0.0f, chi = 0.0f;
-                    if constexpr (FISHER) {
-                        const float m = fmaxf(1e-8f, mu);
-                        const float inv = rcpf(m);
-                        const float e = s[r * R + x] - m;
-                        w = inv;
-                        chi = e * e * inv;
-                    } else {
-                        const float inv = rcpf(fmaxf(mu, 1e-6f));
-                        const float tv = s[r * R + x] * inv;  // samples clamped at load
-                        w = tv * inv;
-                        v = tv - 1.0f;
-                    }
-                    const float wa = w * a, wb = w * b;
-                    if (x == 0) {
-                        Saa = wa * a; Sab = wa * b; Sa1 = wa;
-                        Sbb = wb * b; Sb1 = wb; S11 = w;
-                        if constexpr (FISHER) {
-                            T1 = chi;
-                        } else {
-                            Ta = v * a; Tb = v * b; T1 = v;
-                        }
-                    } else {
-                        Saa += wa * a; Sab += wa * b; Sa1 += wa;
-                        Sbb += wb * b; Sb1 += wb; S11 += w;
-                        if constexpr (FISHER) {
-                            T1 += chi;
-                        } else {
-                            Ta += v * a; Tb += v * b; T1 += v;
-                        }
-                    }
-                    if constexpr (SH) {
-                        const float c = col[x].S;
-                        const float wc = w * c;
-                        if (x == 0) {
-                            Sac = wa * c; Sbc = wb * c; Scc = wc * c; Sc1 = wc;
-                            if constexpr (!FISHER) Tc = v * c;
-                        } else {
-                            Sac += wa * c; Sbc += wb * c; Scc += wc * c; Sc1 += wc;
-                            if constexpr (!FISHER) Tc += v * c;
-                        }
-                    }
-                }
-                // the first row sets acc, later rows add to it
-                auto put = [&](int i, float val) {
-                    if (r == 0)
-                        acc[i] = val;
-                    else
-                        acc[i] += val;
-                };
-                const float D = row.D, E = row.dE;
-                const float DD = D * D, DE = D * E;
-                constexpr int X = 0, Y = 1;
-                put(q(X, X), DD * Saa);
-                put(q(X, Y), DE * Sab);
-                put(q(X, iI), DD * Sab);
-                put(q(X, iBg), D * Sa1);
-                put(q(Y, Y), (E * E) * Sbb);
-                put(q(Y, iI), DE * Sbb);
-                put(q(Y, iBg), E * Sb1);
-                put(q(iI, iI), DD * Sbb);
-                put(q(iI, iBg), D * Sb1);
-                put(q(iBg, iBg), S11);
-                if constexpr (FISHER) {
-                    put(NA, T1);
-                } else {
-                    put(NA + X, D * Ta);
-                    put(NA + Y, E * Tb);
-                    put(NA + iI, D * Tb);
-                    put(NA + iBg, T1);
-                }
-                if constexpr (SH) {
-                    // J_s = D c + Sy b
-                    const float Sy = row.S;
-                    const float DS = D * Sy;
-                    constexpr int s0 = iS;
-                    put(q(X < s0 ? X : s0, X < s0 ? s0 : X), DS * Sab + DD * Sac);
-                    put(q(Y < s0 ? Y : s0, Y < s0 ? s0 : Y), (E * Sy) * Sbb + DE * Sbc);
-                    put(q(iI < s0 ? iI : s0, iI < s0 ? s0 : iI), DS * Sbb + DD * Sbc);
-                    put(q(iBg < s0 ? iBg : s0, iBg < s0 ? s0 : iBg), Sy * Sb1 + D * Sc1);
-                    put(q(s0, s0), (Sy * Sy) * Sbb + (2.0f * DS) * Sbc + DD * Scc);
-                    if constexpr (!FISHER) put(NA + s0, Sy * Tb + D * Tc);
-                }
-            } else if (r == 0) {
+            for (int r = 0; r < RL; r++) {
+                const int y = y0 + r;
+                const Edge nx = gauss_edge((float)(y + 1), t[1], SY.c2);
+                const AxisTerms row = scaled(axis_terms(prev, nx, SY.cp, SY.cs), I);
+                prev = nx;
+                if (RL * G == R || y < R) {
+                    RowSums z;
+                    row_sums<FISHER, R>(I, bg, row.D, [&](int x, float& a, float& b, float& c, float& xs, bool& ok) {
+                        a = col[x].dE; b = col[x].D; c = col[x].S; xs = s[r * R + x]; ok = true;
+                    }, z);
+                    fold_row<FISHER>(row, z, r == 0, acc);
+                } else if (r == 0) {
 #pragma unroll
-                for (int i = 0; i < NACC; i++) acc[i] = 0.0f;
+                    for (int i = 0; i < NACC; i++) acc[i] = 0.0f;
+                }
             }
         }
     }
