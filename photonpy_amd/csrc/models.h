@@ -341,6 +341,8 @@ struct GaussModel {
     static constexpr bool kSplit = kRowFactored && G == 2 && R % 2 == 1;
     template <int R, int G>
     using Layout = std::conditional_t<kSplit<R, G>, SplitRowLayout<R>, RowLayout<R, G>>;
+    // three wavefronts per SIMD (<= 168 VGPRs) for the two-lane split fits
+    static constexpr int min_waves(int R, int G) { return (kRowFactored && G == 2 && R % 2 == 1) ? 3 : 1; }
     static constexpr int lds_floats(int) { return 0; }
 
     // Fit-kernel evaluation of lane g's pixels: fn(slot, mu, J) for every slot of
@@ -395,16 +397,17 @@ struct GaussModel {
 
     // Sums over the N pixels j of one row (or row part) with row term D; pix(j, a, b,
     // c, x, ok) gives pixel j's column terms, sample and whether it is a pixel at all.
+    // mu = bg + (I D) b: one FMA per pixel.
     template <bool FISHER, int N, class Pix>
     PP_DEV static void row_sums(float I, float bg, float D, Pix&& pix, RowSums& z) {
+        const float ID = I * D;
         constexpr bool SH = (V == kSigma || V == kAstig);  // has the c column
 #pragma unroll
         for (int j = 0; j < N; j++) {
             float a, b, c, xs;
             bool ok;
             pix(j, a, b, c, xs, ok);
-            const float p = b * D;
-            const float mu = bg + I * p;
+            const float mu = bg + ID * b;
             float w, v = 0.0f, chi = 0.0f;
             if constexpr (FISHER) {
                 const float m = fmaxf(1e-8f, mu);
