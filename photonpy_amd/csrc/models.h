@@ -343,12 +343,14 @@ struct GaussModel {
     using Layout = std::conditional_t<kSplit<R, G>, SplitRowLayout<R>, RowLayout<R, G>>;
     // Register budgets that measured faster than the compiler's own choice, a few
     // spilled registers included: three wavefronts per SIMD (<= 168 VGPRs) for the
-    // two-lane split fits (11x11 free sigma: fit kernel 4.27 -> 4.20 ms) and four
-    // (<= 128) for the 16x16 astigmatic fit at 8 lanes (13.1 -> 12.6 ms).  One more
-    // wavefront than that spills several times as much and runs 27-50% slower.
+    // two-lane split fits (11x11 free sigma: fit kernel 4.27 -> 4.20 ms) and the
+    // 16x16 astigmatic fit at 4 lanes (11.4 -> 11.2 ms), four (<= 128) for it at 8
+    // lanes (13.1 -> 12.6 ms).  One more wavefront than that spills several times as
+    // much and runs 27-50% slower.
     static constexpr int min_waves(int R, int G) {
         if (kRowFactored && G == 2 && R % 2 == 1) return 3;
         if (V == kAstig && R == 16 && G == 8) return 4;
+        if (V == kAstig && R == 16 && G == 4) return 3;
         return 1;
     }
     static constexpr int lds_floats(int) { return 0; }
