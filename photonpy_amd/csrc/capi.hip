@@ -718,10 +718,36 @@ void Estimator_Estimate(Estimator* e, int n, const float* sample, const float* c
             float* dt = tl ? sl.take<float>(n * tl * K) : nullptr;
             const float* dcn = upload_constants(e, sl, n, constants);
             float* dd = D ? sl.take<float>(n * D) : nullptr;
-            hip_check(hipMemcpy(ds, sample, n * S * 4, hipMemcpyHostToDevice), "H2D samples");
             if (di) hip_check(hipMemcpy(di, initial, n * K * 4, hipMemcpyHostToDevice), "H2D initial");
-            if (dt) hip_check(hipMemset(dt, 0, n * tl * K * 4), "clear trace");
-            run_fit(e, n, ds, dcn, di, dp, dit, dt, (int)tl, nullptr, nullptr, 0);
+            // Large calls without trace or diagnostics upload the samples in chunks of
+            // up to 32 MB: chunk k is copied by the process's copy threads into pinned
+            // staging buffer k % 2, sent on the copy stream and fitted on the fit stream
+            // while chunk k + 1 is being copied (each spot's fit is independent of the
+            // batch it is in, so the results equal one fit over all spots).
+            const int chunk = std::max(4096, (int)((32u << 20) / (S * 4)));
+            if (!dt && !dd && n >= 2 * chunk) {
+                e->ensure_upload((size_t)chunk * S);
+                auto& u = e->up;
+                for (int i0 = 0, k = 0; i0 < n; i0 += chunk, k++) {
+                    const int b = k & 1, m = std::min(chunk, n - i0);
+                    // staging[b] was last read by the H2D copy of chunk k - 2
+                    if (k >= 2) hip_check(hipEventSynchronize(u.copied[b]), "hipEventSynchronize(staging)");
+                    parallel_copy(u.staging[b], sample + (size_t)i0 * S, (size_t)m * S * 4);
+                    hip_check(hipMemcpyAsync(ds + (size_t)i0 * S, u.staging[b], (size_t)m * S * 4, hipMemcpyHostToDevice,
+                                             u.copy),
+                              "H2D samples");
+                    hip_check(hipEventRecord(u.copied[b], u.copy), "hipEventRecord");
+                    hip_check(hipStreamWaitEvent(u.fit, u.copied[b], 0), "hipStreamWaitEvent");
+                    run_fit(e, m, ds + (size_t)i0 * S, dcn ? dcn + (size_t)i0 * C : nullptr,
+                            di ? di + (size_t)i0 * K : nullptr, dp + (size_t)i0 * K, dit + i0, nullptr, 0, nullptr,
+                            nullptr, u.fit);
+                }
+                hip_check(hipStreamSynchronize(u.fit), "hipStreamSynchronize(fit)");
+            } else {
+                hip_check(hipMemcpy(ds, sample, n * S * 4, hipMemcpyHostToDevice), "H2D samples");
+                if (dt) hip_check(hipMemset(dt, 0, n * tl * K * 4), "clear trace");
+                run_fit(e, n, ds, dcn, di, dp, dit, dt, (int)tl, nullptr, nullptr, 0);
+            }
             if (dd) {
                 hip_check(launch_glrt_any(e->desc, e->roisize, n, dp, dcn, ds, outer->glrtMeanOfSpot0, dd, 0),
                           "glrt kernel");

@@ -40,6 +40,8 @@ inline void hip_check(hipError_t e, const char* what) {
 // ---------------------------------------------------------------- objects
 struct EstimationQueue;
 void queue_destroy(EstimationQueue* q);  // queue.hip
+// memcpy over the process's copy threads (queue.hip); host memory only
+void parallel_copy(void* dst, const void* src, size_t bytes);
 
 struct Context {
     std::mutex mtx;
@@ -72,6 +74,45 @@ struct Estimator {
     float* devGrid = nullptr;
     bool gridForm = false;     // table verified to be a grid interpolant
     int splineMode = 0;        // 0: grid form when available, 1: always coefficients
+    // Pipelined upload of a large host call's samples (Estimator_Estimate): two
+    // pinned staging buffers, a copy stream and a fit stream, made on first use.
+    struct Upload {
+        float* staging[2] = {nullptr, nullptr};
+        size_t floats = 0;  // per staging buffer
+        hipStream_t copy = nullptr, fit = nullptr;
+        hipEvent_t copied[2] = {nullptr, nullptr};  // staging buffer's H2D done
+    } up;
+
+    void ensure_upload(size_t floats) {
+        if (!up.copy) {
+            hip_check(hipStreamCreateWithFlags(&up.copy, hipStreamNonBlocking), "hipStreamCreate(copy)");
+            hip_check(hipStreamCreateWithFlags(&up.fit, hipStreamNonBlocking), "hipStreamCreate(fit)");
+            for (auto& ev : up.copied) hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+        }
+        if (floats > up.floats) {
+            for (auto& b : up.staging) {
+                if (b) hip_check(hipHostFree(b), "hipHostFree(staging)");
+                b = nullptr;
+            }
+            up.floats = 0;
+            for (auto& b : up.staging)
+                hip_check(hipHostMalloc((void**)&b, floats * sizeof(float), hipHostMallocDefault), "hipHostMalloc(staging)");
+            up.floats = floats;
+        }
+    }
+
+    void release_upload() {
+        if (up.copy) (void)hipStreamSynchronize(up.copy);
+        if (up.fit) (void)hipStreamSynchronize(up.fit);
+        for (auto& b : up.staging)
+            if (b) (void)hipHostFree(b);
+        for (auto& ev : up.copied)
+            if (ev) (void)hipEventDestroy(ev);
+        if (up.copy) (void)hipStreamDestroy(up.copy);
+        if (up.fit) (void)hipStreamDestroy(up.fit);
+        up = Upload{};
+    }
+
     // grow-only device workspace for the host-pointer API
     void* ws = nullptr;
     size_t wsBytes = 0;
@@ -135,6 +176,7 @@ struct Estimator {
     int G() const { return group > 0 ? group : default_group_for(desc, roisize); }
 
     ~Estimator() {
+        release_upload();
         if (devCoefs) (void)hipFree(devCoefs);
         if (devGrid) (void)hipFree(devGrid);
         if (devShared) (void)hipFree(devShared);

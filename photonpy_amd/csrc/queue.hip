@@ -130,6 +130,8 @@ CopyPool& copy_pool() {
 
 }  // namespace
 
+void parallel_copy(void* dst, const void* src, size_t bytes) { copy_pool().copy(dst, src, bytes); }
+
 struct EstimationQueue {
     struct Batch {
         std::vector<int> ids;
