@@ -134,7 +134,10 @@ void Estimator_ComputeExpectedValue(Estimator* e, int numspots, const float* par
                                     const int* spotpos, float* ev);
 /* Estimators/Estimator.cpp:115-128 — initial may be NULL (centre-of-mass start),
  * trace [numspots, traceBufLen, K] may be NULL; diagnostics [numspots, numDiag] (may be NULL):
- * numDiag is 0 for the models and 3 for a GLRT wrapper */
+ * numDiag is 0 for the models and 3 for a GLRT wrapper.  A call with at least two 32 MB
+ * chunks of samples and no trace or diagnostics buffer uploads the chunks while earlier
+ * ones are fitted (same results), through two 32 MB pinned buffers that the estimator
+ * keeps until it is deleted. */
 void Estimator_Estimate(Estimator* e, int numspots, const float* sample, const float* constants, const int* spotpos,
                         const float* initial, float* params, float* diagnostics, int* iterations, float* trace,
                         int traceBufLen);
