@@ -53,20 +53,25 @@ struct RowLayout {
     PP_DEV static int y(int g, int s) { return g * RL + s / R; }
 };
 
-// SplitRowLayout (two lanes, odd R): RowLayout would give each lane (R+1)/2 rows
-// and leave one row of slots empty.  Here lane g owns H = (R-1)/2 whole rows (lane
-// 0 rows [0, H), lane 1 rows [H+1, R)) in slots [0, H*R), and half of the middle
-// row H in slots H*R + j: lane 0 columns j in [0, M), lane 1 columns M + j,
-// M = (R+1)/2.  Lane 1's last slot (column R) is padding: index() puts it past
-// the ROI.
-template <int R>
+// SplitRowLayout (row-factored models where the rows do not divide evenly): the
+// lanes work in pairs, pair b owning the band of B = 2R/G rows starting at row b*B,
+// with B odd.  RowLayout would give each lane (B+1)/2 rows and leave slots empty;
+// here the pair's lane l owns H = (B-1)/2 whole rows (lane 0 band rows [0, H), lane 1
+// [H+1, B)) in slots [0, H*R), and half of the band's middle row H in slots H*R + j:
+// lane 0 columns j in [0, M), lane 1 columns M + j, M = (R+1)/2.  Lane 1's slot past
+// column R - 1 (odd R) is padding: index() puts it past the ROI.
+template <int R, int G>
 struct SplitRowLayout {
-    static_assert(R % 2 == 1, "odd ROI sizes only");
-    static constexpr int H = (R - 1) / 2, M = (R + 1) / 2;
+    static constexpr int B = 2 * R / G;  // rows per lane pair
+    static_assert(G >= 2 && 2 * R % G == 0 && B % 2 == 1, "an odd band of rows per lane pair");
+    static constexpr int H = (B - 1) / 2, M = (R + 1) / 2;
     static constexpr int NS = H * R + M;
     static constexpr bool kStrided = false;
-    PP_DEV static int y(int g, int s) { return s < H * R ? g * (H + 1) + s / R : H; }
-    PP_DEV static int x(int g, int s) { return s < H * R ? s % R : g * M + (s - H * R); }
+    PP_DEV static int y(int g, int s) {
+        const int y0 = (g >> 1) * B;
+        return s < H * R ? y0 + (g & 1) * (H + 1) + s / R : y0 + H;
+    }
+    PP_DEV static int x(int g, int s) { return s < H * R ? s % R : (g & 1) * M + (s - H * R); }
     PP_DEV static int index(int g, int s) { return x(g, s) < R ? y(g, s) * R + x(g, s) : R * R; }
 };
 
@@ -336,11 +341,13 @@ struct GaussModel {
               (float)x, (float)y, (float)R * 0.5f, t, mu, J);
     }
 
-    // The row-factored models split the middle row of an odd ROI between two lanes.
+    // The row-factored models split the middle row of each lane pair's band when
+    // the band has an odd number of rows (2R/G odd: odd R at 2 lanes, R = 10, 14, ...
+    // at 4, R = 12, 20, 28 at 8, R = 8, 24 at 16).
     template <int R, int G>
-    static constexpr bool kSplit = kRowFactored && G == 2 && R % 2 == 1;
+    static constexpr bool kSplit = kRowFactored && G >= 2 && 2 * R % G == 0 && (2 * R / G) % 2 == 1;
     template <int R, int G>
-    using Layout = std::conditional_t<kSplit<R, G>, SplitRowLayout<R>, RowLayout<R, G>>;
+    using Layout = std::conditional_t<kSplit<R, G>, SplitRowLayout<R, G>, RowLayout<R, G>>;
     // Register budgets that measured faster than the compiler's own choice, a few
     // spilled registers included: three wavefronts per SIMD (<= 168 VGPRs) for the
     // two-lane split fits (11x11 free sigma: fit kernel 4.27 -> 4.20 ms) and the
@@ -536,8 +543,11 @@ struct GaussModel {
         columns<R, G>(t, SX, col);
         const float I = t[iI], bg = t[iBg];
         if constexpr (kSplit<R, G>) {
-            constexpr int H = SplitRowLayout<R>::H, M = SplitRowLayout<R>::M;
-            const int y0 = g * (H + 1);
+            using SL = SplitRowLayout<R, G>;
+            constexpr int H = SL::H, M = SL::M;
+            const int l = g & 1;  // lane in the pair; the pair's band starts at row yb
+            const int yb = (g >> 1) * SL::B;
+            const int y0 = yb + l * (H + 1);
             const Edge first = gauss_edge((float)y0, t[1], SY.c2);
             Edge prev = first;
 #pragma unroll
@@ -551,22 +561,22 @@ struct GaussModel {
                 }, z);
                 fold_row<FISHER>(row, z, r == 0, acc);
             }
-            // the middle row H: lane 0 holds edge H (its last) and lane 1 edge H + 1 (its
-            // first); each evaluates the other one
-            const Edge en = gauss_edge((float)(g == 0 ? H + 1 : H), t[1], SY.c2);
-            const Edge lo = g == 0 ? prev : en, hi = g == 0 ? en : first;
+            // the band's middle row yb + H: lane 0 holds its upper edge (its last) and lane
+            // 1 its lower edge (its first); each evaluates the other one
+            const Edge en = gauss_edge((float)(yb + (l == 0 ? H + 1 : H)), t[1], SY.c2);
+            const Edge lo = l == 0 ? prev : en, hi = l == 0 ? en : first;
             const AxisTerms row = scaled(axis_terms(lo, hi, SY.cp, SY.cs), I);
             RowSums z;
             row_sums<FISHER, M>(I, bg, row.D, [&](int j, float& a, float& b, float& c, float& xs, bool& ok) {
-                if (j < M - 1) {
-                    const bool hiHalf = g != 0;
+                if (M + j < R) {  // lane 0 column j, lane 1 column M + j
+                    const bool hiHalf = l != 0;
                     a = hiHalf ? col[M + j].dE : col[j].dE;
                     b = hiHalf ? col[M + j].D : col[j].D;
                     c = hiHalf ? col[M + j].S : col[j].S;
                     ok = true;
-                } else {  // column M - 1 for lane 0, padding for lane 1
+                } else {  // odd R: column M - 1 for lane 0, padding for lane 1
                     a = col[j].dE; b = col[j].D; c = col[j].S;
-                    ok = g == 0;
+                    ok = l == 0;
                 }
                 xs = s[H * R + j];
             }, z);

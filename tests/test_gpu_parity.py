@@ -218,21 +218,23 @@ def test_group_sizes_agree(ctx, R, G):
     assert bad.sum() <= max(2, 1e-2 * conv.sum()), f"{bad.sum()} of {conv.sum()} differ"
 
 
-@pytest.mark.parametrize("kind", ["fixed", "astig"])
-def test_split_middle_row_agrees_with_row_layout(ctx, kind):
-    # 9x9 ROIs at 2 lanes per spot split the middle row between the two lanes; at 4
-    # lanes every lane owns whole rows.  Both pixel partitions must give the same fits
-    # (astig: the z column's row terms go through the split row too).
-    th, smp = MD.samples(kind, 800, 29, R=9)
-    est, _, _ = MD.make(kind, ctx, R=9)
+@pytest.mark.parametrize("kind,R,Gs,Gw", [("fixed", 9, 2, 4), ("astig", 9, 2, 4), ("fixed", 10, 4, 2),
+                                           ("astig", 10, 4, 2)])
+def test_split_middle_row_agrees_with_row_layout(ctx, kind, R, Gs, Gw):
+    # Gs lanes per spot split the middle row of each lane pair's band of 2R/Gs rows
+    # between the pair (9x9 at 2 lanes, 10x10 at 4); at Gw lanes every lane owns whole
+    # rows.  Both pixel partitions must give the same fits (astig: the z column's row
+    # terms go through the split row too).
+    th, smp = MD.samples(kind, 800, 29, R=R)
+    est, _, _ = MD.make(kind, ctx, R=R)
     lo, hi = est.ParamLimits()
-    est.SetGroupSize(4)
-    p4, _, _ = est.Estimate(smp)
-    est.SetGroupSize(2)
-    p2, _, _ = est.Estimate(smp)
-    ok = np.all(np.isfinite(p4), axis=1) & ~_on_bounds(p4, lo, hi)
+    est.SetGroupSize(Gw)
+    pw, _, _ = est.Estimate(smp)
+    est.SetGroupSize(Gs)
+    ps, _, _ = est.Estimate(smp)
+    ok = np.all(np.isfinite(pw), axis=1) & ~_on_bounds(pw, lo, hi)
     assert ok.mean() > 0.5
-    bad = ~_compare(kind, p2[ok], p4[ok])
+    bad = ~_compare(kind, ps[ok], pw[ok])
     assert bad.sum() <= max(2, 1e-2 * ok.sum()), f"{bad.sum()} of {ok.sum()} differ"
 
 
