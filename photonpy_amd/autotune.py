@@ -133,7 +133,7 @@ def batch_streams(default=4):
 
 
 def main():
-    """python -m photonpy_amd.autotune [--configs c1,c2,c3,c4] [--n 200000] [--batch-n 400000]"""
+    """python -m photonpy_amd.autotune [--configs c1,c2,c3,c4] [--n 200000] [--bins] [--batch-n 400000]"""
     import argparse
     import torch
     from . import Context
@@ -142,6 +142,8 @@ def main():
     ap.add_argument("--configs", default="c1,c2,c3,c4")
     ap.add_argument("--n", type=int, default=200_000)
     ap.add_argument("--batch-n", type=int, default=0, help="also tune the mixed batch's stream count on this many ROIs")
+    ap.add_argument("--bins", action="store_true",
+                    help="tune every (model, ROI size) bin of the mixed workload not yet in the table, on its own ROIs")
     args = ap.parse_args()
     with Context() as ctx:
         for c in [c for c in args.configs.split(",") if c]:
@@ -154,6 +156,19 @@ def main():
                   flush=True)
             del smp
             torch.cuda.empty_cache()
+        if args.bins:
+            # the keys of the configs above keep their own workload's choice
+            tuned = set(load())
+            for b in synthetic.mixed_bins(ctx):
+                k = key(b["est"])
+                if k in tuned or not compiled_groups(b["est"]):
+                    continue
+                best, res = tune_group(b["est"], b["samples"])
+                tuned.add(k)
+                print("bin", k, b["n"], "best", best,
+                      {f"{g}/{r}": round(b["n"] / v / 1e6, 2) for (g, r), v in res.items()}, flush=True)
+                del b
+                torch.cuda.empty_cache()
         if args.batch_n:
             bins = synthetic.mixed_bins(ctx, n=args.batch_n)
             for b in bins:
