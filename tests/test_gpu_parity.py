@@ -236,6 +236,19 @@ def test_split_middle_row_agrees_with_row_layout(ctx, kind, R, Gs, Gw):
     assert ok.mean() > 0.5
     bad = ~_compare(kind, ps[ok], pw[ok])
     assert bad.sum() <= max(2, 1e-2 * ok.sum()), f"{bad.sum()} of {ok.sum()} differ"
+    # the fused CRLB and chi^2 (Fisher matrix from the same row sums) agree as well
+    import torch
+    d = torch.from_numpy(smp).cuda()
+    est.SetGroupSize(Gw)
+    ow = est.estimate_device(d, crlb=True)
+    est.SetGroupSize(Gs)
+    os_ = est.estimate_device(d, crlb=True)
+    cw, cs = ow["crlb"].cpu().numpy(), os_["crlb"].cpu().numpy()
+    xw, xs = ow["chisq"].cpu().numpy(), os_["chisq"].cpu().numpy()
+    m = ok & np.all(np.isfinite(cw), axis=1) & (cw[:, 0] < 0.5)
+    assert m.mean() > 0.5
+    same = np.all(np.isclose(cs[m], cw[m], rtol=1e-3), axis=1) & np.isclose(xs[m], xw[m], rtol=1e-3, atol=1e-3)
+    assert same.mean() > 0.98, f"{(~same).sum()} of {m.sum()} CRLB/chi^2 differ"
 
 
 def test_refill_batches_do_not_change_results(ctx):
