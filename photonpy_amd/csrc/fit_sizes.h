@@ -13,7 +13,7 @@
 #define PPA_GRID_SIZES_0(X) X(3, 4) X(4, 4) X(5, 4) X(6, 4) X(7, 8) X(7, 4) X(8, 8) X(9, 8) X(9, 16) X(10, 8)
 #define PPA_GRID_SIZES_1(X) X(11, 8) X(11, 16) X(12, 8) X(13, 16) X(13, 8) X(14, 16) X(15, 16) X(16, 16) X(16, 32)
 #define PPA_GRID_SIZES_2(X) X(17, 16) X(18, 16) X(19, 16) X(20, 16) X(20, 32) X(20, 64) X(21, 16) X(22, 16) X(23, 16)
-#define PPA_GRID_SIZES_3(X) X(24, 16) X(25, 16) X(25, 32) X(26, 16) X(27, 16) X(28, 16) X(29, 16) X(30, 16) X(31, 16) X(32, 16)
+#define PPA_GRID_SIZES_3(X) X(24, 16) X(25, 16) X(25, 32) X(25, 64) X(26, 16) X(27, 16) X(28, 16) X(29, 16) X(30, 16) X(31, 16) X(32, 16)
 
 // The reference's other Gaussian variants and the intensity/background model:
 // default_group(R) for every R.

@@ -300,7 +300,7 @@ def test_spline_grid_form_matches_coefficient_form(ctx):
     # the grid form also matches the oracle (coefficient form) through test_estimate_parity
 
 
-@pytest.mark.parametrize("R,G", [(20, 16), (20, 32), (16, 32), (11, 16), (7, 4)])
+@pytest.mark.parametrize("R,G", [(20, 16), (20, 32), (16, 32), (11, 16), (7, 4), (25, 64)])
 def test_spline_grid_group_sizes_agree(ctx, R, G):
     th, smp = MD.samples("spline", 400, 33, R=R)
     est, _, _ = MD.make("spline", ctx, R=R)
