@@ -9,15 +9,21 @@
  * (inputs already resident in HBM, asynchronous on a caller-supplied HIP
  * stream) that have no counterpart in the reference.
  *
- * Errors: reference-style calls report failures through
- * SMLM_GetLastError() (creators also return NULL); device-pointer calls
- * return 0 on success and a nonzero code otherwise.  numspots == 0 is a
- * no-op everywhere, as in the reference.
+ * Errors: every caught failure goes to the debug-print callback
+ * (SetDebugPrintCallback) and is kept for SMLM_GetLastError(); creators also
+ * return NULL and device-pointer calls return a nonzero code.  numspots == 0
+ * is a no-op everywhere, as in the reference.
+ *
+ * The `cuda` flag of the creators: true gives the reference's GPU estimator
+ * interface (NumDiag 0), false its CPU estimator interface (NumDiag 1, a
+ * diagnostics column that Estimate leaves untouched, EstimatorImpl.h:108-185).
+ * Both compute on the GPU; there is no CPU code path.
  */
 #ifndef PHOTONPY_AMD_H
 #define PHOTONPY_AMD_H
 
 #include <stdbool.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -177,6 +183,33 @@ int EstimQueue_GetResults(EstimationQueue* q, int maxresults, float* estim, floa
  * As in the reference, `mean` is the mean sample of the first spot of each call (each
  * batch for a queue); GLRT_SetMeanOfFirstSpot(e, 0) makes it each spot's own. */
 Estimator* GLRT_CreatePSF(Estimator* model, Context* ctx);
+
+/* ---------------------------------------------------------------------- */
+/* Process-level functions the reference's Python loader binds at load time  */
+/* (photonpy/cpp/lib.py:71-153, gaussian.py:76-84).                          */
+
+/* CudaUtils.cu:12-16 — number of visible HIP devices */
+int CudaGetNumDevices(void);
+/* CudaUtils.cu:18-21 — makes device `index` current for the calling thread */
+bool CudaSetDevice(int index);
+/* CudaUtils.cu:23-32 — compute-unit count and device name (truncated to namelen - 1) */
+bool CudaGetDeviceInfo(int index, int* numMultiprocessors, char* name, int namelen);
+/* CudaUtils.cu:38-46 — bytes held by this library's device and pinned host allocations
+ * (pitched allocations do not exist here: those two tallies are 0) */
+void CudaGetMemoryUse(uint64_t* pinnedBytes, uint64_t* devicePitchedBytes, uint64_t* deviceBytes,
+                      uint64_t* pitchedNumAllocs);
+/* StringUtils.cpp:45-49 — receives every error message a C-ABI call catches (also kept for
+ * SMLM_GetLastError); a callback returning nonzero lets the library echo it to stderr */
+void SetDebugPrintCallback(int (*cb)(const char* msg));
+/* DebugImageCallback.cpp:14-26 — ShowDebugImage passes its arguments to the callback */
+void SetDebugImageCallback(void (*cb)(int width, int height, int numImages, const float* data, const char* title));
+void ShowDebugImage(int w, int h, int numImages, const float* data, const char* title);
+/* FFT.cpp:151-172, Rendering.cpp:12-59: bound by the loader but outside the spot-fitting
+ * path; each reports "not provided" through the error channel and does nothing else */
+void FFT(const void* src, void* dst, int batchsize, int siglen, int forward);
+void FFT2(const void* src, void* dst, int batchsize, int sigw, int sigh, int forward);
+void DrawROIs(float* image, int width, int height, const float* rois, int numrois, int roisize, const int* roiposYX);
+void Gauss2D_Draw(double* image, int imgw, int imgh, const float* spotList, int nspots, float addSigma);
 
 /* ---------------------------------------------------------------------- */
 /* MI355X additions (no reference counterpart)                              */

@@ -84,7 +84,7 @@ MixedBatch* MixedBatch_Create(int nbins, Estimator* const* ests, const int* coun
                 for (int b = 0; b < nbins; b++) {
                     Estimator* e = ests[b];
                     if (!e) throw std::runtime_error("photonpy_amd: NULL estimator in mixed batch");
-                    if (e->numDiag > 0)
+                    if (e->inner)
                         throw std::runtime_error("photonpy_amd: a mixed batch returns no diagnostics; pass the "
                                                  "wrapped estimator instead of its GLRT wrapper");
                     if (e->NC() > 0)

@@ -162,8 +162,15 @@ int fit_group(Estimator* e, int groupHint) {
 
 void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constants, const float* d_initial,
              float* d_params, int* d_iter, float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st,
-             bool timing, int groupHint) {
-    e->ensure_device();
+             bool timing, int groupHint, int* counter) {
+    if (counter) {
+        if (!e->device_ready())
+            throw std::runtime_error("photonpy_amd: the spline tables of this estimator are not on the device yet; "
+                                     "an allocation-free call cannot upload them (any other fit or evaluation call "
+                                     "on the estimator does)");
+    } else {
+        e->ensure_device();
+    }
     // The fit kernel's work counter and the spot strides are 32-bit and run past n by
     // at most the grid's group count (< 2^21 lane groups on one device).
     constexpr int kMaxSpots = INT_MAX - (1 << 21);
@@ -187,13 +194,13 @@ void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constant
     a.maxIter = e->lm.iterations;
     a.lambda = e->lm.lambda;
     a.refill = e->refill;
-    a.counter = e->next_counter();
+    a.counter = counter ? counter : e->next_counter();
     a.constants = d_constants;
     if (e->NC() > 0 && !d_constants)
         throw std::runtime_error("photonpy_amd: this estimator needs " + std::to_string(e->NC()) +
                                  " constants per spot");
     hipEvent_t* events = nullptr;
-    if (e->phaseTiming && timing) {
+    if (e->phaseTiming && timing && !counter) {
         for (int i = 0; i < 3; i++) {
             hipEvent_t ev;
             hip_check(hipEventCreate(&ev), "hipEventCreate");
@@ -226,6 +233,66 @@ struct Slicer {
         return t;
     }
 };
+
+// CSpline_CreatePSF (throws on failure).
+static Estimator* create_spline(int roisize, const CSpline_Calibration* calib, const float* coefs, int mode, bool cuda,
+                                Context* ctx) {
+    if (!calib) throw std::runtime_error("photonpy_amd: calibration is NULL");
+    // CSPLINE_FLATBG / TILTEDBG / BGIMAGE (CubicSplinePSF.cu:72-97)
+    if (mode < 0 || mode > 2) throw std::runtime_error("Invalid model background mode");
+    if (roisize < 1) throw std::runtime_error("photonpy_amd: roisize must be positive");
+    const float* src = coefs ? coefs : calib->coefs;
+    if (!src) throw std::runtime_error("photonpy_amd: spline coefficients are NULL");
+    const int Nz = calib->shape[0], Ny = calib->shape[1], Nx = calib->shape[2];
+    if (Nz < 1 || Ny < 1 || Nx < 1) throw std::runtime_error("photonpy_amd: invalid spline shape");
+    auto* e = new Estimator();
+    e->roisize = roisize;
+    e->desc.kind = mode == 1 ? kKindSplineTilted : mode == 2 ? kKindSplineImage : kKindSpline;
+    e->desc.sm.Nz = Nz;
+    e->desc.sm.Ny = Ny;
+    e->desc.sm.Nx = Nx;
+    e->desc.sm.zmin = calib->z_min;
+    e->desc.sm.zmax = calib->z_max;
+    e->desc.sm.coefs = nullptr;
+    e->hostCoefs.assign(src, src + (size_t)Nz * Ny * Nx * 64);  // copied, as CubicSplinePSF.cu:48-52
+    e->desc.sg.Nz = Nz;
+    e->desc.sg.Ny = Ny;
+    e->desc.sg.Nx = Nx;
+    e->desc.sg.zmin = calib->z_min;
+    e->desc.sg.zmax = calib->z_max;
+    e->desc.sg.grid = nullptr;
+    e->gridForm = spline_grid_from_coefs(Nz, Ny, Nx, e->hostCoefs.data(), e->hostGrid);
+    if (!e->gridForm) e->hostGrid.clear();
+    e->desc.useGrid = e->gridForm;
+    e->fmt = mode == 1 ? "x,y,z,I,bg,bgx,bgy" : "x,y,z,I,bg";
+    const float b = 1.5f, hi = (float)roisize - 1 - b;
+    e->limits = {{b, hi}, {b, hi}, {calib->z_min, calib->z_max}, {10.0f, 1e9f}, {1e-6f, 1e9f}};
+    if (mode == 1) {
+        e->limits.push_back({-1e6f, 1e6f});
+        e->limits.push_back({-1e6f, 1e6f});
+    }
+    set_flavour(e, cuda);
+    attach(e, ctx);
+    return e;
+}
+
+// CSpline_Compute on the GPU (throws on failure).
+static void compute_spline_coefs(const int* shape, const float* data, float* splineCoefs, int border) {
+    if (!shape || !data || !splineCoefs) throw std::runtime_error("photonpy_amd: NULL argument");
+    const size_t nin = (size_t)shape[0] * shape[1] * shape[2];
+    const int o0 = shape[0] - 3 + 2 * border, o1 = shape[1] - 3 + 2 * border, o2 = shape[2] - 3 + 2 * border;
+    if (o0 <= 0 || o1 <= 0 || o2 <= 0) throw std::runtime_error("photonpy_amd: z-stack too small for a spline");
+    const size_t nout = (size_t)o0 * o1 * o2 * 64;
+    float *din = nullptr, *dout = nullptr;
+    hip_check(dev_alloc(&din, nin * sizeof(float)), "hipMalloc");
+    hipError_t err = dev_alloc(&dout, nout * sizeof(float));
+    if (err == hipSuccess) err = hipMemcpy(din, data, nin * sizeof(float), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = launch_spline_coefs(shape, din, dout, border, 0);
+    if (err == hipSuccess) err = hipMemcpy(splineCoefs, dout, nout * sizeof(float), hipMemcpyDeviceToHost);
+    (void)dev_release(din);
+    (void)dev_release(dout);
+    hip_check(err, "CSpline_Compute");
+}
 
 extern "C" {
 
@@ -262,11 +329,11 @@ void Context_Destroy(Context* ctx) {
 Estimator* Gauss2D_CreatePSF_XYIBg(int roisize, float sigmaX, float sigmaY, bool cuda, Context* ctx) {
     return guarded(
         [&]() -> Estimator* {
-            require_gpu(cuda);
             GaussCalib gc{};
             gc.sx = sigmaX;
             gc.sy = sigmaY;
             Estimator* e = make_gauss(kKindFixed, roisize, gc);
+            set_flavour(e, cuda);
             attach(e, ctx);
             return e;
         },
@@ -276,10 +343,10 @@ Estimator* Gauss2D_CreatePSF_XYIBg(int roisize, float sigmaX, float sigmaY, bool
 Estimator* Gauss2D_CreatePSF_XYIBgSigma(int roisize, float initialSigma, bool cuda, Context* ctx) {
     return guarded(
         [&]() -> Estimator* {
-            require_gpu(cuda);
             GaussCalib gc{};
             gc.initialSigma = initialSigma;
             Estimator* e = make_gauss(kKindSigma, roisize, gc);
+            set_flavour(e, cuda);
             attach(e, ctx);
             return e;
         },
@@ -290,8 +357,8 @@ Estimator* Gauss2D_CreatePSF_XYIBgSigma(int roisize, float initialSigma, bool cu
 Estimator* Gauss2D_CreatePSF_XYIBgConstSigma(int roisize, bool cuda, Context* ctx) {
     return guarded(
         [&]() -> Estimator* {
-            require_gpu(cuda);
             Estimator* e = make_gauss(kKindConstSigma, roisize, GaussCalib{});
+            set_flavour(e, cuda);
             attach(e, ctx);
             return e;
         },
@@ -302,8 +369,8 @@ Estimator* Gauss2D_CreatePSF_XYIBgConstSigma(int roisize, bool cuda, Context* ct
 Estimator* Gauss2D_CreatePSF_XYITiltedBg(int roisize, bool cuda, Context* ctx) {
     return guarded(
         [&]() -> Estimator* {
-            require_gpu(cuda);
             Estimator* e = make_gauss(kKindTiltedBg, roisize, GaussCalib{});
+            set_flavour(e, cuda);
             attach(e, ctx);
             return e;
         },
@@ -311,11 +378,11 @@ Estimator* Gauss2D_CreatePSF_XYITiltedBg(int roisize, bool cuda, Context* ctx) {
 }
 
 static Estimator* create_sigmaxy(int kind, int roisize, float isx, float isy, bool cuda, Context* ctx) {
-    require_gpu(cuda);
     GaussCalib gc{};
     gc.initialSigma = isx;
     gc.initialSigmaY = isy;
     Estimator* e = make_gauss(kind, roisize, gc);
+    set_flavour(e, cuda);
     attach(e, ctx);
     return e;
 }
@@ -379,7 +446,6 @@ void Gauss2D_EstimateIntensityBg(const float* imageData, float* IBg, float* IBg_
 
 static Estimator* create_astig(int roisize, const float* x, const float* y, float minz, float maxz, float Bx, float By,
                                bool cuda, Context* ctx) {
-    require_gpu(cuda);
     GaussCalib gc{};
     for (int i = 0; i < 4; i++) {
         gc.ax[i] = x[i];
@@ -390,6 +456,7 @@ static Estimator* create_astig(int roisize, const float* x, const float* y, floa
     gc.minz = minz;
     gc.maxz = maxz;
     Estimator* e = make_gauss(kKindAstig, roisize, gc);
+    set_flavour(e, cuda);
     attach(e, ctx);
     return e;
 }
@@ -414,70 +481,13 @@ Estimator* Gauss2D_CreatePSF_XYZIBgQ(int roisize, const Gauss3D_CalibrationQ* c,
 
 Estimator* CSpline_CreatePSF(int roisize, const CSpline_Calibration* calib, const float* coefs, int mode, bool cuda,
                              Context* ctx) {
-    return guarded(
-        [&]() -> Estimator* {
-            require_gpu(cuda);
-            if (!calib) throw std::runtime_error("photonpy_amd: calibration is NULL");
-            // CSPLINE_FLATBG / TILTEDBG / BGIMAGE (CubicSplinePSF.cu:72-97)
-            if (mode < 0 || mode > 2) throw std::runtime_error("Invalid model background mode");
-            if (roisize < 1) throw std::runtime_error("photonpy_amd: roisize must be positive");
-            const float* src = coefs ? coefs : calib->coefs;
-            if (!src) throw std::runtime_error("photonpy_amd: spline coefficients are NULL");
-            const int Nz = calib->shape[0], Ny = calib->shape[1], Nx = calib->shape[2];
-            if (Nz < 1 || Ny < 1 || Nx < 1) throw std::runtime_error("photonpy_amd: invalid spline shape");
-            auto* e = new Estimator();
-            e->roisize = roisize;
-            e->desc.kind = mode == 1 ? kKindSplineTilted : mode == 2 ? kKindSplineImage : kKindSpline;
-            e->desc.sm.Nz = Nz;
-            e->desc.sm.Ny = Ny;
-            e->desc.sm.Nx = Nx;
-            e->desc.sm.zmin = calib->z_min;
-            e->desc.sm.zmax = calib->z_max;
-            e->desc.sm.coefs = nullptr;
-            e->hostCoefs.assign(src, src + (size_t)Nz * Ny * Nx * 64);  // copied, as CubicSplinePSF.cu:48-52
-            e->desc.sg.Nz = Nz;
-            e->desc.sg.Ny = Ny;
-            e->desc.sg.Nx = Nx;
-            e->desc.sg.zmin = calib->z_min;
-            e->desc.sg.zmax = calib->z_max;
-            e->desc.sg.grid = nullptr;
-            e->gridForm = spline_grid_from_coefs(Nz, Ny, Nx, e->hostCoefs.data(), e->hostGrid);
-            if (!e->gridForm) e->hostGrid.clear();
-            e->desc.useGrid = e->gridForm;
-            e->fmt = mode == 1 ? "x,y,z,I,bg,bgx,bgy" : "x,y,z,I,bg";
-            const float b = 1.5f, hi = (float)roisize - 1 - b;
-            e->limits = {{b, hi}, {b, hi}, {calib->z_min, calib->z_max}, {10.0f, 1e9f}, {1e-6f, 1e9f}};
-            if (mode == 1) {
-                e->limits.push_back({-1e6f, 1e6f});
-                e->limits.push_back({-1e6f, 1e6f});
-            }
-            attach(e, ctx);
-            return e;
-        },
-        nullptr);
+    return guarded([&]() -> Estimator* { return create_spline(roisize, calib, coefs, mode, cuda, ctx); }, nullptr);
 }
 
 void CSpline_Compute(const int* shape, const float* data, float* splineCoefs, int border) {
     guarded(
         [&]() -> int {
-            if (!shape || !data || !splineCoefs) throw std::runtime_error("photonpy_amd: NULL argument");
-            const size_t nin = (size_t)shape[0] * shape[1] * shape[2];
-            const int o0 = shape[0] - 3 + 2 * border, o1 = shape[1] - 3 + 2 * border, o2 = shape[2] - 3 + 2 * border;
-            if (o0 <= 0 || o1 <= 0 || o2 <= 0) throw std::runtime_error("photonpy_amd: z-stack too small for a spline");
-            const size_t nout = (size_t)o0 * o1 * o2 * 64;
-            float *din = nullptr, *dout = nullptr;
-            hip_check(hipMalloc(&din, nin * sizeof(float)), "hipMalloc");
-            hipError_t e1 = hipMalloc(&dout, nout * sizeof(float));
-            if (e1 != hipSuccess) {
-                (void)hipFree(din);
-                hip_check(e1, "hipMalloc");
-            }
-            hipError_t err = hipMemcpy(din, data, nin * sizeof(float), hipMemcpyHostToDevice);
-            if (err == hipSuccess) err = launch_spline_coefs(shape, din, dout, border, 0);
-            if (err == hipSuccess) err = hipMemcpy(splineCoefs, dout, nout * sizeof(float), hipMemcpyDeviceToHost);
-            (void)hipFree(din);
-            (void)hipFree(dout);
-            hip_check(err, "CSpline_Compute");
+            compute_spline_coefs(shape, data, splineCoefs, border);
             return 0;
         },
         0);
@@ -492,9 +502,7 @@ Estimator* CSpline_CreatePSF_FromFile(int roisize, const char* filename, int mod
             if (cells[0] < 1 || cells[1] < 1 || cells[2] < 1)
                 throw std::runtime_error("photonpy_amd: z-stack too small for a spline");
             std::vector<float> coefs((size_t)cells[0] * cells[1] * cells[2] * 64);
-            SMLM_ClearError();
-            CSpline_Compute(h.shape, data.data(), coefs.data(), 0);
-            if (!g_lastError.empty()) throw std::runtime_error(g_lastError);
+            compute_spline_coefs(h.shape, data.data(), coefs.data(), 0);
             // the coefficient table's shape (the reference passes the z-stack's own
             // shape here, CubicSplinePSF.cu:141, and so reads past its table)
             CSpline_Calibration calib;
@@ -504,9 +512,7 @@ Estimator* CSpline_CreatePSF_FromFile(int roisize, const char* filename, int mod
             calib.z_min = h.zmin;
             calib.z_max = h.zmax;
             calib.coefs = coefs.data();
-            Estimator* e = CSpline_CreatePSF(roisize, &calib, nullptr, mode, cuda, ctx);
-            if (!e) throw std::runtime_error(g_lastError);
-            return e;
+            return create_spline(roisize, &calib, nullptr, mode, cuda, ctx);
         },
         nullptr);
 }
@@ -521,13 +527,13 @@ void CSpline_Evaluate(const int* shape, const float* splineCoefs, const float* s
             const size_t ncoef = (size_t)shape[0] * shape[1] * shape[2] * 64;
             const size_t nout = (size_t)o0 * o1 * o2 * 4;
             float *dc = nullptr, *dout = nullptr;
-            hip_check(hipMalloc(&dc, ncoef * sizeof(float)), "hipMalloc");
-            hipError_t err = hipMalloc(&dout, nout * sizeof(float));
+            hip_check(dev_alloc(&dc, ncoef * sizeof(float)), "hipMalloc");
+            hipError_t err = dev_alloc(&dout, nout * sizeof(float));
             if (err == hipSuccess) err = hipMemcpy(dc, splineCoefs, ncoef * sizeof(float), hipMemcpyHostToDevice);
             if (err == hipSuccess) err = launch_spline_evaluate(shape, dc, shift, border, dout, 0);
             if (err == hipSuccess) err = hipMemcpy(values, dout, nout * sizeof(float), hipMemcpyDeviceToHost);
-            (void)hipFree(dc);
-            if (dout) (void)hipFree(dout);
+            (void)dev_release(dc);
+            if (dout) (void)dev_release(dout);
             hip_check(err, "CSpline_Evaluate");
             return 0;
         },
@@ -708,7 +714,9 @@ void Estimator_Estimate(Estimator* e, int n, const float* sample, const float* c
             std::lock_guard<std::mutex> l(e->mtx);
             const size_t K = e->K(), S = e->S(), C = e->NC();
             const size_t tl = (trace && traceBufLen > 0) ? (size_t)traceBufLen : 0;
-            const size_t D = diagnostics ? (size_t)outer->numDiag : 0;
+            // diagnostics are computed for a GLRT wrapper; a cuda = false estimator's
+            // column is left as the caller passed it (set_flavour)
+            const size_t D = (diagnostics && outer->inner) ? (size_t)outer->numDiag : 0;
             Slicer sl{(char*)e->workspace(Slicer::bytes(
                 {n * S * 4, initial ? n * K * 4 : 0, n * K * 4, (size_t)n * 4, n * tl * K * 4, n * C * 4, n * D * 4}))};
             float* ds = sl.take<float>(n * S);
@@ -866,7 +874,7 @@ int Estimator_ChiSquareAndCRLBDevice(Estimator* e, int n, const float* d_params,
 int Estimator_DiagnosticsDevice(Estimator* e, int n, const float* d_sample, const float* d_constants,
                                 const float* d_params, float* d_diag, void* stream) {
     if (!e) return 1;
-    if (n <= 0 || e->numDiag == 0) return 0;
+    if (n <= 0 || !e->inner) return 0;
     Estimator* outer = e;
     e = model_of(e);
     return guarded(

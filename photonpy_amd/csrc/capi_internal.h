@@ -27,7 +27,30 @@ using namespace ppa;
 // ---------------------------------------------------------------- errors
 inline thread_local std::string g_lastError;
 
-inline void set_error(const std::string& msg) { g_lastError = msg; }
+// Passes a message to the debug-print callback (runtime.hip).
+void deliver_message(const std::string& msg);
+
+// Records the calling thread's last error and delivers it through the debug-print
+// callback, as the reference reports the exceptions its C-ABI calls catch.
+inline void set_error(const std::string& msg) {
+    g_lastError = msg;
+    deliver_message(msg);
+}
+
+// Device and pinned host allocations of the library, tallied for CudaGetMemoryUse
+// (runtime.hip).  The release functions accept NULL.
+hipError_t dev_alloc(void** p, size_t bytes);
+hipError_t dev_release(void* p);
+hipError_t pinned_alloc(void** p, size_t bytes);
+hipError_t pinned_release(void* p);
+template <class T>
+inline hipError_t dev_alloc(T** p, size_t bytes) {
+    return dev_alloc(reinterpret_cast<void**>(p), bytes);
+}
+template <class T>
+inline hipError_t pinned_alloc(T** p, size_t bytes) {
+    return pinned_alloc(reinterpret_cast<void**>(p), bytes);
+}
 
 struct HipError : std::runtime_error {
     using std::runtime_error::runtime_error;
@@ -62,6 +85,7 @@ struct Estimator {
     // numDiag diagnostics per spot.
     Estimator* inner = nullptr;
     int numDiag = 0;
+    bool hostFlavour = false;  // created with cuda = false (see set_flavour)
     int glrtMeanOfSpot0 = 1;  // the reference's GLRT mean (first spot of the call)
     int group = 0;                  // 0: default_group(roisize)
     int refill = 0;                 // stopped groups per batched refill (0: kernel default)
@@ -91,12 +115,12 @@ struct Estimator {
         }
         if (floats > up.floats) {
             for (auto& b : up.staging) {
-                if (b) hip_check(hipHostFree(b), "hipHostFree(staging)");
+                if (b) hip_check(pinned_release(b), "hipHostFree(staging)");
                 b = nullptr;
             }
             up.floats = 0;
             for (auto& b : up.staging)
-                hip_check(hipHostMalloc((void**)&b, floats * sizeof(float), hipHostMallocDefault), "hipHostMalloc(staging)");
+                hip_check(pinned_alloc(&b, floats * sizeof(float)), "hipHostMalloc(staging)");
             up.floats = floats;
         }
     }
@@ -105,7 +129,7 @@ struct Estimator {
         if (up.copy) (void)hipStreamSynchronize(up.copy);
         if (up.fit) (void)hipStreamSynchronize(up.fit);
         for (auto& b : up.staging)
-            if (b) (void)hipHostFree(b);
+            if (b) (void)pinned_release(b);
         for (auto& ev : up.copied)
             if (ev) (void)hipEventDestroy(ev);
         if (up.copy) (void)hipStreamDestroy(up.copy);
@@ -135,7 +159,7 @@ struct Estimator {
     }
 
     void ensure_counters() {
-        if (!devCounters) hip_check(hipMalloc(&devCounters, kCounterSlots * sizeof(int)), "hipMalloc(counters)");
+        if (!devCounters) hip_check(dev_alloc(&devCounters, kCounterSlots * sizeof(int)), "hipMalloc(counters)");
     }
 
     int* next_counter() {
@@ -167,7 +191,7 @@ struct Estimator {
         std::vector<float> v = shared_values();
         if (v.empty()) return nullptr;
         if (!devShared) {
-            hip_check(hipMalloc(&devShared, v.size() * sizeof(float)), "hipMalloc(shared)");
+            hip_check(dev_alloc(&devShared, v.size() * sizeof(float)), "hipMalloc(shared)");
             hip_check(hipMemcpy(devShared, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice), "upload shared");
         }
         return devShared;
@@ -177,35 +201,41 @@ struct Estimator {
 
     ~Estimator() {
         release_upload();
-        if (devCoefs) (void)hipFree(devCoefs);
-        if (devGrid) (void)hipFree(devGrid);
-        if (devShared) (void)hipFree(devShared);
-        if (devCounters) (void)hipFree(devCounters);
-        if (ws) (void)hipFree(ws);
+        if (devCoefs) (void)dev_release(devCoefs);
+        if (devGrid) (void)dev_release(devGrid);
+        if (devShared) (void)dev_release(devShared);
+        if (devCounters) (void)dev_release(devCounters);
+        if (ws) (void)dev_release(ws);
         release_phase_events();
     }
 
     void ensure_device() {
         if (is_spline(desc.kind) && !devCoefs) {
             const size_t bytes = hostCoefs.size() * sizeof(float);
-            hip_check(hipMalloc(&devCoefs, bytes), "hipMalloc(spline coefficients)");
+            hip_check(dev_alloc(&devCoefs, bytes), "hipMalloc(spline coefficients)");
             hip_check(hipMemcpy(devCoefs, hostCoefs.data(), bytes, hipMemcpyHostToDevice), "upload spline coefficients");
             desc.sm.coefs = devCoefs;
         }
         if (is_spline(desc.kind) && desc.useGrid && !devGrid) {
             const size_t bytes = hostGrid.size() * sizeof(float);
-            hip_check(hipMalloc(&devGrid, bytes), "hipMalloc(spline grid)");
+            hip_check(dev_alloc(&devGrid, bytes), "hipMalloc(spline grid)");
             hip_check(hipMemcpy(devGrid, hostGrid.data(), bytes, hipMemcpyHostToDevice), "upload spline grid");
             desc.sg.grid = devGrid;
         }
     }
 
+    // Whether the device tables a fit reads are resident (nothing to upload).
+    bool device_ready() const {
+        if (!is_spline(desc.kind)) return true;
+        return devCoefs && (!desc.useGrid || devGrid);
+    }
+
     void* workspace(size_t bytes) {
         if (bytes > wsBytes) {
-            if (ws) hip_check(hipFree(ws), "hipFree(workspace)");
+            if (ws) hip_check(dev_release(ws), "hipFree(workspace)");
             ws = nullptr;
             wsBytes = 0;
-            hip_check(hipMalloc(&ws, bytes), "hipMalloc(workspace)");
+            hip_check(dev_alloc(&ws, bytes), "hipMalloc(workspace)");
             wsBytes = bytes;
         }
         return ws;
@@ -242,9 +272,13 @@ inline auto guarded(F fn, decltype(fn()) fallback) -> decltype(fn()) {
     }
 }
 
-inline void require_gpu(bool cuda) {
-    if (!cuda)
-        throw std::runtime_error("photonpy_amd: CPU estimators are not provided; create the estimator with cuda=True");
+// cuda = false asks the reference for its CPU estimator (PSFImpl, EstimatorImpl.h:108-185).
+// Such an estimator computes on the GPU like the others; it keeps the CPU estimator's
+// interface: NumDiag = 1 (EstimatorImpl.h:120), a diagnostics column its Estimate
+// never writes (EstimatorImpl.h:166-182).
+inline void set_flavour(Estimator* e, bool cuda) {
+    e->hostFlavour = !cuda;
+    e->numDiag = cuda ? 0 : 1;
 }
 
 // ---------------------------------------------------------------- launch helpers
@@ -259,6 +293,8 @@ int fit_group(Estimator* e, int groupHint);
 // groupHint > 0: try that many lanes per spot first (halving down to the estimator's
 // own group size while no kernel is compiled for it) -- small batches fill more of
 // the GPU with more lanes per spot.
+// counter != null: use that work counter and allocate nothing (the estimator's device
+// tables must already be resident; no timing events are recorded).
 void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constants, const float* d_initial,
              float* d_params, int* d_iter, float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st,
-             bool timing = true, int groupHint = 0);
+             bool timing = true, int groupHint = 0, int* counter = nullptr);

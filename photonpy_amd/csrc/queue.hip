@@ -37,7 +37,7 @@ template <class T>
 T* pinned(size_t n) {
     void* p = nullptr;
     if (n == 0) return nullptr;
-    hip_check(hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault), "hipHostMalloc(queue batch)");
+    hip_check(pinned_alloc(&p, n * sizeof(T)), "hipHostMalloc(queue batch)");
     return static_cast<T*>(p);
 }
 
@@ -45,7 +45,7 @@ template <class T>
 T* device_alloc(size_t n) {
     void* p = nullptr;
     if (n == 0) return nullptr;
-    hip_check(hipMalloc(&p, n * sizeof(T)), "hipMalloc(queue stream buffers)");
+    hip_check(dev_alloc(&p, n * sizeof(T)), "hipMalloc(queue stream buffers)");
     return static_cast<T*>(p);
 }
 
@@ -148,6 +148,8 @@ struct EstimationQueue {
         Batch(int n, int S, int K, int NC, int D) : ids(n), roipos((size_t)n * 2) {
             constants = pinned<float>((size_t)n * NC);
             diag = pinned<float>((size_t)n * D);
+            // written by the GLRT pass only; a cuda = false estimator's column stays 0
+            if (diag) std::memset(diag, 0, sizeof(float) * (size_t)n * D);
             samples = pinned<float>((size_t)n * S);
             params = pinned<float>((size_t)n * K);
             crlb = pinned<float>((size_t)n * K);
@@ -157,7 +159,7 @@ struct EstimationQueue {
         ~Batch() {
             for (void* p : {(void*)samples, (void*)params, (void*)crlb, (void*)chisq, (void*)iterations,
                             (void*)constants, (void*)diag})
-                if (p) (void)hipHostFree(p);
+                if (p) (void)pinned_release(p);
         }
     };
 
@@ -233,7 +235,7 @@ struct EstimationQueue {
             if (ln.stream) (void)hipStreamSynchronize(ln.stream);
             for (void* p : {(void*)ln.samples, (void*)ln.params, (void*)ln.crlb, (void*)ln.chisq, (void*)ln.iterations,
                             (void*)ln.constants, (void*)ln.diag})
-                if (p) (void)hipFree(p);
+                if (p) (void)dev_release(p);
             if (ln.done) (void)hipEventDestroy(ln.done);
             if (ln.stream) (void)hipStreamDestroy(ln.stream);
         }
@@ -255,12 +257,12 @@ struct EstimationQueue {
             std::lock_guard<std::mutex> l(est->mtx);
             run_fit(est, n, ln.samples, NC > 0 ? ln.constants : nullptr, init ? ln.params : nullptr, ln.params,
                     ln.iterations, nullptr, 0, ln.crlb, ln.chisq, ln.stream, false, groupHint);
-            if (D > 0)
+            if (D > 0 && outer->inner)
                 hip_check(launch_glrt_any(est->desc, est->roisize, n, ln.params, NC > 0 ? ln.constants : nullptr,
                                           ln.samples, outer->glrtMeanOfSpot0, ln.diag, ln.stream),
                           "queue: glrt kernel");
         }
-        if (D > 0)
+        if (D > 0 && outer->inner)
             hip_check(hipMemcpyAsync(b->diag, ln.diag, sizeof(float) * n * D, hipMemcpyDeviceToHost, ln.stream),
                       "queue: diagnostics to host");
         hip_check(hipMemcpyAsync(b->params, ln.params, sizeof(float) * n * K, hipMemcpyDeviceToHost, ln.stream),

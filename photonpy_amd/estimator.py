@@ -45,131 +45,143 @@ class Estimator:
     def _check(self):
         self._smlm.check()
 
-    def _checkparams(self, params, constants, roipos):
-        """estimator.py:155-187"""
-        params = np.array(params)
-        if len(params.shape) == 1:
-            params = [params]
-        params = np.ascontiguousarray(params, dtype=np.float32)
-        numspots = len(params)
+    # ------------------------------------------------------------ validation
+    # One validator per argument kind.  Shapes and the ValueError behaviour follow
+    # the reference API (estimator.py:155-187, 317-337): a single parameter row is
+    # promoted to [1, K]; constants may be None only when the model has none and
+    # otherwise are broadcast to [n, NumConstants]; roi positions default to zeros.
+    def _spot_params(self, params):
+        p = np.asarray(params, dtype=np.float32)
+        if p.ndim == 1:
+            p = p[np.newaxis]
+        if p.ndim != 2 or p.shape[1] != self.numparams:
+            raise ValueError(f"parameters must be [n, {self.numparams}] in the order {self.fmt!r}; "
+                             f"got shape {p.shape}")
+        return np.ascontiguousarray(p)
+
+    def _spot_constants(self, constants, n):
+        nc = self.numconst
         if constants is None:
-            if self.numconst != 0:
-                raise ValueError(f'Estimator is expecting constant array with shape {[len(params), self.numconst]} (None given) ')
-            constants = np.zeros((numspots, self.numconst), dtype=np.float32)
-        else:
-            constants = np.ascontiguousarray(constants, dtype=np.float32)
-            if constants.size != numspots * self.numconst:
-                try:
-                    constants_ = np.zeros([numspots, self.numconst], dtype=np.float32)
-                    constants_[:, :] = constants
-                    constants = constants_
-                except Exception:
-                    raise ValueError(f'Provided constants with shape {constants.shape}, expecting {[numspots, self.numconst]}')
+            if nc:
+                raise ValueError(f"this estimator needs {nc} constants per spot: pass an array of shape [{n}, {nc}]")
+            return np.zeros((n, 0), np.float32)
+        c = np.asarray(constants, dtype=np.float32)
+        if c.size == n * nc:
+            return np.ascontiguousarray(c.reshape(n, nc))
+        try:
+            return np.ascontiguousarray(np.broadcast_to(c, (n, nc)))
+        except ValueError:
+            raise ValueError(f"constants of shape {c.shape} cannot be broadcast to [{n}, {nc}]") from None
+
+    def _spot_roipos(self, roipos, n):
         if roipos is None:
-            roipos = np.zeros((numspots, self.indexdims), dtype=np.int32)
-        else:
-            roipos = np.ascontiguousarray(roipos, dtype=np.int32)
-            if not np.array_equal(roipos.shape, [numspots, self.indexdims]):
-                raise ValueError(f'Provided roi positions have incorrect shape: {roipos.shape} should be {[numspots, self.indexdims]}')
-        if params.ndim != 2 or params.shape[1] != self.numparams:
-            raise ValueError(f"{self.fmt} expected, {params.shape[-1]} parameters given")
-        return params, constants, roipos
+            return np.zeros((n, self.indexdims), np.int32)
+        r = np.ascontiguousarray(roipos, dtype=np.int32)
+        if r.shape != (n, self.indexdims):
+            raise ValueError(f"roi positions must be [{n}, {self.indexdims}]; got shape {r.shape}")
+        return r
+
+    def _spot_stack(self, imgs, n=None, what="samples"):
+        s = np.ascontiguousarray(imgs, dtype=np.float32)
+        want = tuple(int(v) for v in self.sampleshape)
+        if s.ndim != 1 + len(want) or tuple(s.shape[1:]) != want or (n is not None and len(s) != n):
+            lead = "n" if n is None else str(n)
+            raise ValueError(f"{what} must be [{lead}, {', '.join(map(str, want))}]; got shape {s.shape}")
+        return s
+
+    def _checkparams(self, params, constants, roipos):
+        """(params [n,K], constants [n,NumConstants], roipos [n,indexdims]) as contiguous arrays."""
+        p = self._spot_params(params)
+        return p, self._spot_constants(constants, len(p)), self._spot_roipos(roipos, len(p))
+
+    @staticmethod
+    def _fisher(deriv, ev):
+        """Fisher matrices sum_px J_k J_l / mu of [n,K,...] derivatives and [n,...] expected
+        values, mu floored at 1e-9 as in the reference's numpy route."""
+        n, K = deriv.shape[:2]
+        J = deriv.reshape(n, K, -1).astype(np.float64)
+        w = 1.0 / np.maximum(ev.reshape(n, -1).astype(np.float64), 1e-9)
+        return np.einsum("nkp,nlp,np->nkl", J, J, w)
+
+    @staticmethod
+    def _sd_of(cov):
+        return np.sqrt(np.abs(np.diagonal(cov, axis1=1, axis2=2)))
 
     # ------------------------------------------------------------------ model
     def ExpectedValue(self, params, roipos=None, constants=None):
         """Expected value of each pixel of the ROIs: [len(params), *sampleshape]."""
-        params, constants, roipos = self._checkparams(params, constants, roipos)
-        ev = np.zeros((len(params), *self.sampleshape), dtype=np.float32)
-        if len(params) > 0:
-            self._lib.Estimator_ComputeExpectedValue(self.inst, len(params), params, constants, roipos, ev)
+        p, c, r = self._checkparams(params, constants, roipos)
+        ev = np.zeros((len(p), *self.sampleshape), dtype=np.float32)
+        if len(p):
+            self._lib.Estimator_ComputeExpectedValue(self.inst, len(p), p, c, r, ev)
             self._check()
         return ev
 
     def GenerateSample(self, params, roipos=None, constants=None):
-        ev = self.ExpectedValue(params, roipos, constants)
-        return np.random.poisson(ev)
+        return np.random.poisson(self.ExpectedValue(params, roipos, constants))
 
     def Derivatives(self, params, roipos=None, constants=None):
         """(deriv [N,K,*shape], ev [N,*shape])"""
-        params, constants, roipos = self._checkparams(params, constants, roipos)
-        deriv = np.zeros((len(params), self.numparams, *self.sampleshape), dtype=np.float32)
-        ev = np.zeros((len(params), *self.sampleshape), dtype=np.float32)
-        if len(params) > 0:
-            self._lib.Estimator_ComputeDerivatives(self.inst, len(params), params, constants, roipos, deriv, ev)
+        p, c, r = self._checkparams(params, constants, roipos)
+        deriv = np.zeros((len(p), self.numparams, *self.sampleshape), dtype=np.float32)
+        ev = np.zeros((len(p), *self.sampleshape), dtype=np.float32)
+        if len(p):
+            self._lib.Estimator_ComputeDerivatives(self.inst, len(p), p, c, r, deriv, ev)
             self._check()
         return deriv, ev
 
     def FisherMatrix(self, params, roipos=None, constants=None):
-        """estimator.py:205-220 (ev clamped to >= 1e-9)."""
-        params, constants, roipos = self._checkparams(params, constants, roipos)
-        fi = np.zeros((len(params), self.numparams, self.numparams), dtype=np.float32)
-        if len(params) > 0:
-            deriv, ev = self.Derivatives(params, roipos, constants)
-            K = self.numparams
-            ev = np.reshape(ev, (len(params), self.samplecount))
-            deriv = np.reshape(deriv, (len(params), K, self.samplecount))
-            ev[ev < 1e-9] = 1e-9
-            for i in range(K):
-                for j in range(K):
-                    fi[:, i, j] = np.sum(1 / ev * (deriv[:, i] * deriv[:, j]), axis=1)
-        return fi
+        """[N,K,K] Fisher matrices from the analytic derivatives (float32)."""
+        p, c, r = self._checkparams(params, constants, roipos)
+        if not len(p):
+            return np.zeros((0, self.numparams, self.numparams), np.float32)
+        deriv, ev = self.Derivatives(p, r, c)
+        return self._fisher(deriv, ev).astype(np.float32)
 
     def CRLBAndChiSquare(self, params, samples=None, roipos=None, constants=None):
-        """(crlb, chisq) if samples are given, otherwise crlb (estimator.py:222-244)."""
-        params, constants, roipos = self._checkparams(params, constants, roipos)
-        crlb = np.zeros((len(params), self.numparams), dtype=np.float32)
-        chisq = np.zeros(len(params), dtype=np.float32)
-        if samples is not None:
-            samples = np.ascontiguousarray(samples, dtype=np.float32)
-            if not np.array_equal(samples.shape, [len(params), *self.sampleshape]):
-                raise ValueError(f"Sample data was expected to have shape {[len(params), *self.sampleshape]}. Given: {samples.shape}")
-        if len(params) > 0:
-            self._lib.Estimator_ChiSquareAndCRLB(self.inst, params, samples, constants, roipos, len(params), crlb, chisq)
+        """(crlb, chisq) when samples are given, otherwise crlb; computed by the library."""
+        p, c, r = self._checkparams(params, constants, roipos)
+        n = len(p)
+        smp = None if samples is None else self._spot_stack(samples, n)
+        crlb = np.zeros((n, self.numparams), dtype=np.float32)
+        chisq = np.zeros(n, dtype=np.float32)
+        if n:
+            self._lib.Estimator_ChiSquareAndCRLB(self.inst, p, smp, c, r, n, crlb, chisq)
             self._check()
-        if samples is not None:
-            return crlb, chisq
-        return crlb
+        return crlb if smp is None else (crlb, chisq)
 
     def ChiSquare(self, params, samples, roipos=None, constants=None):
         return self.CRLBAndChiSquare(params, samples, roipos, constants)[1]
 
     def CRLB(self, params, roipos=None, constants=None):
-        """Pseudo-inverse route of estimator.py:249-253."""
-        fisher = self.FisherMatrix(params, roipos, constants)
-        var = np.linalg.pinv(fisher)
-        return np.sqrt(np.abs(var[:, np.arange(self.numparams), np.arange(self.numparams)]))
+        """CRLB from the pseudo-inverse of FisherMatrix (the reference's numpy route)."""
+        return self._sd_of(np.linalg.pinv(self.FisherMatrix(params, roipos, constants)))
 
     def NumDeriv(self, params, roipos=None, constants=None, eps=1e-6):
-        """Central differences of ExpectedValue (estimator.py:267-284)."""
-        params, constants, roipos = self._checkparams(params, constants, roipos)
-        ev = self.ExpectedValue(params, roipos, constants)
-        deriv = np.zeros((len(params), self.numparams, *self.sampleshape), dtype=np.float32)
-        for k in range(self.numparams):
-            params_min = params * 1
-            params_min[:, k] -= eps
-            params_max = params * 1
-            params_max[:, k] += eps
-            ev_min = self.ExpectedValue(params_min, roipos, constants)
-            ev_max = self.ExpectedValue(params_max, roipos, constants)
-            deriv[:, k] = (ev_max - ev_min) / (2 * eps)
-        return deriv, ev
+        """Central differences of ExpectedValue, (deriv [N,K,*shape], ev [N,*shape]).
+        All 2K shifted parameter sets go to the library in one call."""
+        p, c, r = self._checkparams(params, constants, roipos)
+        n, K = p.shape
+        ev = self.ExpectedValue(p, r, c)
+        if not n:
+            return np.zeros((0, K, *self.sampleshape), np.float32), ev
+        shift = np.zeros((2, K, 1, K), np.float32)
+        k = np.arange(K)
+        shift[0, k, 0, k] = eps
+        shift[1, k, 0, k] = -eps
+        stacked = (p[np.newaxis, np.newaxis] + shift).reshape(2 * K * n, K)
+        evs = self.ExpectedValue(stacked, np.tile(r, (2 * K, 1)), np.tile(c, (2 * K, 1)))
+        evs = evs.reshape(2, K, n, *self.sampleshape)
+        deriv = np.moveaxis((evs[0] - evs[1]) / (2 * eps), 0, 1)
+        return np.ascontiguousarray(deriv, dtype=np.float32), ev
 
     def NumCRLB(self, params, roipos=None, constants=None, eps=1e-4, useNumDeriv=True):
-        """estimator.py:286-300"""
+        """CRLB from the inverse Fisher matrix of numerical (or analytic) derivatives."""
         if useNumDeriv:
             deriv, ev = self.NumDeriv(params, roipos, constants, eps)
         else:
             deriv, ev = self.Derivatives(params, roipos, constants)
-        K = self.numparams
-        deriv = np.reshape(deriv, (len(params), K, self.samplecount))
-        ev = np.reshape(ev, (len(params), self.samplecount))
-        ev[ev < 1e-9] = 1e-9
-        fi = np.zeros((len(params), K, K))
-        for i in range(K):
-            for j in range(K):
-                fi[:, i, j] = np.sum(1 / ev * (deriv[:, i] * deriv[:, j]), axis=1)
-        var = np.linalg.inv(fi)
-        return np.sqrt(np.abs(var[:, np.arange(self.numparams), np.arange(self.numparams)]))
+        return self._sd_of(np.linalg.inv(self._fisher(deriv, ev)))
 
     # ---------------------------------------------------------------- fitting
     def GetLevMarParams(self):
@@ -179,35 +191,29 @@ class Estimator:
         return lmp
 
     def SetLevMarParams(self, lambda_=1e-15, iterations=50, lmFactor=0.0):
-        self.lmparams = LevMarParams()
-        self.lmparams.lambda_ = lambda_
-        self.lmparams.iterations = iterations
-        self.lmparams.factor = lmFactor
+        self.lmparams = LevMarParams(iterations=int(iterations), lambda_=lambda_, factor=lmFactor)
         self._lib.Estimator_SetLMParams(self.inst, self.lmparams)
 
     def Estimate(self, imgs, roipos=None, constants=None, initial=None):
-        """(params [N,K], diag [N,numdiag], traces: list of [iterations_k, K]) — estimator.py:317-361."""
-        numspots = len(imgs)
-        traceBufLen = self.lmparams.iterations
-        imgs = np.ascontiguousarray(imgs, dtype=np.float32)
-        if not np.array_equal(imgs.shape[1:], self.sampleshape):
-            raise ValueError(f"Sample images are expected to have shape {self.sampleshape}. Given: {imgs.shape[1:]}")
-        iterations = np.zeros(numspots, np.int32)
-        params = np.zeros((numspots, self.numparams), dtype=np.float32)
-        trace = np.zeros((numspots, traceBufLen, self.numparams), dtype=np.float32)
-        diag = np.zeros((numspots, self.numdiag), dtype=np.float32)
-        params, constants, roipos = self._checkparams(params, constants, roipos)
+        """Fit ROIs [N,*sampleshape]: (params [N,K], diag [N,numdiag], traces), traces[i]
+        being spot i's parameters before each iteration it ran ([iterations_i, K])."""
+        smp = self._spot_stack(imgs, what="sample images")
+        n, K = len(smp), self.numparams
+        c, r = self._spot_constants(constants, n), self._spot_roipos(roipos, n)
         if initial is not None:
             initial = np.ascontiguousarray(initial, dtype=np.float32)
-            if not np.array_equal(initial.shape, [numspots, self.numparams]):
-                raise ValueError(f'Initial value array is expected to have shape {[numspots, self.numparams]}. Given: {initial.shape}')
-        if numspots > 0:
-            self._lib.Estimator_Estimate(self.inst, numspots, imgs, constants, roipos, initial, params, diag,
-                                         iterations, trace, traceBufLen)
+            if initial.shape != (n, K):
+                raise ValueError(f"initial values must be [{n}, {K}]; got shape {initial.shape}")
+        tl = self.lmparams.iterations
+        fitted = np.zeros((n, K), np.float32)
+        diag = np.zeros((n, self.numdiag), np.float32)
+        its = np.zeros(n, np.int32)
+        trace = np.zeros((n, tl, K), np.float32)
+        if n:
+            self._lib.Estimator_Estimate(self.inst, n, smp, c, r, initial, fitted, diag, its, trace, tl)
             self._check()
-        self.last_iterations = iterations
-        traces = [trace[k, 0:iterations[k], :] for k in range(numspots)]
-        return params, diag, traces
+        self.last_iterations = its
+        return fitted, diag, [trace[i, :its[i]] for i in range(n)]
 
     # ------------------------------------------------------- device tensors
     def _stream(self, stream):
@@ -267,7 +273,7 @@ class Estimator:
             self._check()
             raise RuntimeError(f"Estimator_EstimateDevice failed ({rc})")
         if self.numdiag > 0:
-            out["diag"] = torch.empty((n, self.numdiag), dtype=torch.float32, device=dev)
+            out["diag"] = torch.zeros((n, self.numdiag), dtype=torch.float32, device=dev)
             rc = self._lib.Estimator_DiagnosticsDevice(self.inst, n, ps, pc, out["params"].data_ptr(),
                                                        out["diag"].data_ptr(), self._stream(stream))
             if rc != 0:
@@ -392,8 +398,8 @@ class Estimator:
         return self._lib.Estimator_ParamFormat(self.inst).decode("utf-8")
 
     def ParamIndex(self, paramName):
-        if type(paramName) == list:
-            return [self.param_names.index(n) for n in paramName]
+        if isinstance(paramName, (list, tuple)):
+            return [self.ParamIndex(p) for p in paramName]
         return self.param_names.index(paramName)
 
     def NumParams(self):

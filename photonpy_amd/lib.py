@@ -81,6 +81,24 @@ class SMLM:
     def device_count(self):
         return self.lib.SMLM_DeviceCount()
 
+    def GetCudaMemUse(self):
+        """(pinned bytes, pitched bytes, device bytes, pitched allocations) held by the library."""
+        vals = [ct.c_uint64() for _ in range(4)]
+        self.lib.CudaGetMemoryUse(*[ct.byref(v) for v in vals])
+        return tuple(v.value for v in vals)
+
+    def GetCudaDevices(self):
+        """Names of the visible HIP devices."""
+        names = []
+        for d in range(self.lib.CudaGetNumDevices()):
+            cus, buf = ct.c_int32(), ct.create_string_buffer(256)
+            if self.lib.CudaGetDeviceInfo(d, ct.byref(cus), buf, len(buf)):
+                names.append(buf.value.decode("utf-8", "replace"))
+        return names
+
+    def SetCudaDevice(self, idx):
+        return bool(self.lib.CudaSetDevice(int(idx)))
+
 
 def _bind(lib):
     V, I, F, B = ct.c_void_p, ct.c_int32, ct.c_float, ct.c_bool
@@ -153,6 +171,10 @@ def _bind(lib):
         "MixedBatch_RunPlain": ([V, ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), V],
                                 ct.c_int),
         "Estimator_GetShared": ([V, V], ct.c_int),
+        "CudaGetNumDevices": ([], ct.c_int32),
+        "CudaSetDevice": ([I], ct.c_bool),
+        "CudaGetDeviceInfo": ([I, ct.POINTER(ct.c_int32), ct.c_char_p, I], ct.c_bool),
+        "CudaGetMemoryUse": ([ct.POINTER(ct.c_uint64)] * 4, None),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

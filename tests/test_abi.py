@@ -82,9 +82,13 @@ def test_spline_properties_and_modes(ctx):
         m.CreatePSF_XYZIBg(12, pa.CSplineCalibration(-1.0, 1.0, coefs), 7, True)
 
 
-def test_cpu_estimators_refused(ctx):
-    with pytest.raises(pa.NativeError, match="cuda=True"):
-        pa.GaussianPSFMethods(ctx).CreatePSF_XYIBg(10, 1.5, False)
+def test_cuda_false_keeps_the_cpu_estimator_interface(ctx):
+    # EstimatorImpl.h:120: the reference's CPU estimators report one diagnostics column
+    g = pa.GaussianPSFMethods(ctx)
+    cpu, gpu = g.CreatePSF_XYIBgSigma(11, 1.6, False), g.CreatePSF_XYIBgSigma(11, 1.6, True)
+    assert cpu.numdiag == 1 and gpu.numdiag == 0
+    assert cpu.numparams == gpu.numparams and cpu.fmt == gpu.fmt
+    assert np.array_equal(cpu.limits, gpu.limits)
 
 
 def test_shape_validation(ctx):
