@@ -1,11 +1,14 @@
 """Throughput benchmark of the MI355X spot fitter.
 
-`python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2]`
+`python bench.py [--gpus 1] [--steps K] [--warmup W] [--config all|c1|c2|c3|c4|c4adj|c5|queue]`
 
 One step = one fused launch (Levenberg-Marquardt fit + CRLB + chi^2) over the
 whole batch of synthetic ROIs, already resident in HBM.  Prints one JSON line.
 
-Workloads (the default is c2):
+The default run (`--config all`) reports the headline workload c2 in the
+top-level fields (K timed steps after W warm-up steps) and times every other
+workload of the metric under detail.configs, each with its own roofline and
+(c1-c4) CPU baseline:
   c1  2D Gaussian fixed sigma=1.5, 10x10 px, 10,000 spots, 30 iterations + CRLB
   c2  2D Gaussian free sigma, 11x11 px, 1,000,000 spots, 50 iterations + CRLB
   c3  astigmatic 3D Gaussian (quartic sigma(z)), 16x16 px, 2,000,000 spots, 50 iterations + CRLB
@@ -30,10 +33,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "MLE localizations/s per model (vs CPU + plain GPU); PSF-adjoint grad evals/s"
-PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 (vector and matrix)
-# The roofline's "bound" names the compute roofline ("mfma" in the output contract);
-# the fit issues FP32 vector instructions, no matrix instructions.
-COMPUTE_NOTE = "compute-bound at the FP32 peak; FP32 VALU instructions (no MFMA), packed FP32 disabled"
+PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 vector peak
+# These kernels issue FP32 vector (VALU) instructions and no matrix instructions: the
+# bound is the FP32 VALU roofline, labelled "valu".
+BOUND = "valu"
+COMPUTE_NOTE = "FP32 VALU roofline (no MFMA is issued); packed FP32 disabled"
+# Algorithmic FLOP per ROI of the spline adjoint's backward pass (per pixel: the spline
+# value and derivatives, the NLL gradient terms and the 64 coefficient-gradient terms).
+ADJ_FLOP_PER_ROI = 276_400
+# ROI gradient evaluations per 65,536-ROI step: 20x20 px x 64 coefficients each.
+ADJ_DIRECT_ATOMICS_PER_ROI = 400 * 64
 
 # ---------------------------------------------------------------------------
 # Algorithmic FLOP count per localization (FP32 model + normal equations, FP64
@@ -92,37 +101,87 @@ def hbm_bytes_per_loc(kind, R):
     return 4 * R * R + 4 * (2 * K + 2)  # ROI in; params, CRLB, chi^2, iterations out
 
 
-def cpu_baseline(cfg, smp_host, est, target_s=12.0):
-    """The oracle (FP32 restatement of the reference estimator) on host cores, over a
-    bounded prefix of the same ROIs: Estimate + CRLB + chi^2 per spot."""
+def host_info():
+    """Host the CPU baseline runs on: logical CPUs, the CPUs this process may use, the model."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "model": model}
+
+
+# The GPU box grants one GPU's share of its host: worker pools stay within 16 threads.
+CPU_SHARE = 16
+
+
+def oracle_model(cfg, est):
     from oracle import oracle as O
     kind, R = cfg["kind"], cfg["R"]
     if kind == "fixed":
-        om = O.gauss2d(R, 1.5)
-    elif kind == "sigma":
-        om = O.gauss2d_sigma(R, 1.6)
-    elif kind == "astig":
-        om = O.astig(R, [1.3, 200.0, 400.0, 0.1 / 400], [1.3, -200.0, 400.0, 0.1 / 400], [-600.0, 600.0],
-                     B=(0.05 / 400 ** 2, 0.05 / 400 ** 2))
-    else:
-        coefs = est.shared_parameters("cpu").numpy().reshape(61, 26, 26, 64)
-        lo, hi = est.ParamLimits()
-        om = O.spline(R, coefs, float(lo[2]), float(hi[2]))
-    threads = min(16, os.cpu_count() or 1)
+        return O.gauss2d(R, 1.5)
+    if kind == "sigma":
+        return O.gauss2d_sigma(R, 1.6)
+    if kind == "astig":
+        a = synthetic_astig()
+        return O.astig(R, a[0], a[1], [-600.0, 600.0], B=(a[2], a[2]))
+    coefs = est.shared_parameters("cpu").numpy().reshape(61, 26, 26, 64)
+    lo, hi = est.ParamLimits()
+    return O.spline(R, coefs, float(lo[2]), float(hi[2]))
 
-    def run(m):
-        s = smp_host[:m]
+
+def synthetic_astig():
+    """Reference parametrisation of the astigmatic workload: {s0, gamma, d, A/d} per axis, B/d^2."""
+    return [1.3, 200.0, 400.0, 0.1 / 400], [1.3, -200.0, 400.0, 0.1 / 400], 0.05 / 400 ** 2
+
+
+def cpu_baseline(cfg, smp_host, est, run_s=1.0, repeats=5):
+    """The oracle (FP32 restatement of the reference estimator) on the host: Estimate + CRLB
+    + chi^2 over the first m ROIs of the same batch (the batch repeated when it is smaller),
+    m sized so that one run takes about run_s seconds; one warm-up run, then the median of
+    `repeats` timed runs."""
+    om = oracle_model(cfg, est)
+    host = host_info()
+    threads = max(1, min(CPU_SHARE, host["usable_cpus"]))
+    total = len(smp_host)
+
+    def rois(m):
+        if m <= total:
+            return smp_host[:m]
+        return np.concatenate([smp_host] * (m // total) + [smp_host[:m % total]])
+
+    def run(s):
         t = time.perf_counter()
         p, _ = om.estimate(s, iterations=cfg["iters"], lambda_=cfg["lam"], nthreads=threads)
         om.crlb(p, s, nthreads=threads)
         return time.perf_counter() - t
 
-    m = min(len(smp_host), 4096)
-    dt = run(m)
-    m2 = int(min(len(smp_host), max(m, m * target_s / max(dt, 1e-3))))
-    dt2 = run(m2)
-    return {"value": m2 / dt2, "unit": "localizations/s", "cores": threads, "kind": "port",
-            "sample": f"first {m2} of the same ROIs, oracle Estimate+CRLB+chi^2, {threads} threads, {dt2:.1f} s"}
+    m = min(total, 512)
+    dt = run(rois(m))
+    for _ in range(4):  # grow the sample until a run takes at least half of run_s
+        if dt >= 0.5 * run_s:
+            break
+        m = int(m * min(64.0, 1.1 * run_s / max(dt, 1e-4)))
+        dt = run(rois(m))
+    s = rois(m)
+    run(s)  # warm-up
+    times = sorted(run(s) for _ in range(repeats))
+    med = times[len(times) // 2]
+    print(f"bench: cpu baseline {cfg['kind']} R={cfg['R']}: {m} ROIs, median {med:.3f} s on {threads} threads",
+          file=sys.stderr, flush=True)
+    what = f"first {m} ROIs of the same batch" if m <= total else f"the {total} ROIs of the batch, {m} in all"
+    return {"value": m / med, "unit": "localizations/s", "cores": threads, "kind": "port",
+            "sample": f"{what}; oracle Estimate + CRLB + chi^2 on {threads} threads (one GPU's {CPU_SHARE}-core "
+                      f"share of the host); median of {repeats} runs after one warm-up",
+            "runs_s": [round(t, 4) for t in times], "host": host}
 
 
 def host_api_rate(est, smp_host):
@@ -140,16 +199,105 @@ def host_api_rate(est, smp_host):
     return m / (time.perf_counter() - t1)
 
 
-def bench_mixed(args):
+def line(metric_value, unit, steps, warmup, ms, workload, extra_config, roofline, cpu, detail):
+    return {"metric": METRIC, "value": metric_value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": {"workload": workload, **extra_config}, "roofline": roofline,
+            "cpu_baseline": cpu, "detail": detail}
+
+
+def bench_fit(name, steps, warmup, n=0, group=0, refill=0, cpu=True, e2e=True):
+    """One single-model workload: `steps` timed fused launches after `warmup` untimed ones."""
+    import torch
+    import photonpy_amd as pa
+    from photonpy_amd import autotune, synthetic
+    cfg = synthetic.WORKLOADS[name]
+    n = n or cfg["n"]
+    torch.cuda.set_device(0)
+    with pa.Context() as ctx:
+        est = synthetic.make_estimator(ctx, cfg["kind"], cfg["R"])
+        est.SetLevMarParams(cfg["lam"], cfg["iters"])
+        autotune.apply(est)
+        if group:
+            est.SetGroupSize(group)
+        if refill:
+            est.SetRefill(refill)
+        th, smp = synthetic.make_samples(est, cfg["kind"], n, seed=1)
+        torch.cuda.synchronize()
+        for _ in range(warmup):
+            out = est.estimate_device(smp, crlb=True)
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        # HIP events on the launch stream around each kernel (fit, then CRLB)
+        est.set_phase_timing(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ev0[k].record(stream)
+            out = est.estimate_device(smp, crlb=True)
+            ev1[k].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        launches, fit_ms, crlb_ms = est.phase_times()
+        est.set_phase_timing(False)
+        assert launches == steps, (launches, steps)
+        step_gpu_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev0, ev1)]))
+        ms_per_step = wall * 1e3 / steps
+
+        it = out["iterations"].cpu().numpy()
+        p = out["params"].cpu().numpy()
+        executed = np.minimum(it, cfg["iters"]).astype(np.float64)
+        fkind = "splinegrid" if est.spline_form() == "grid" else cfg["kind"]
+        per0 = fit_flops_per_loc(fkind, cfg["R"], 0)
+        per1 = fit_flops_per_loc(fkind, cfg["R"], 1) - per0
+        fit_flops = n * per0 + per1 * executed.sum()
+        crlb_flops = n * crlb_flops_per_loc(fkind, cfg["R"])
+        achieved = fit_flops / (fit_ms * 1e-3) / 1e12
+        traffic = hbm_traffic(name) if n == cfg["n"] else None
+        e2e_rate = host_api_rate(est, smp[:min(n, 200_000)].cpu().numpy()) if e2e else None
+        cpu_line = cpu_baseline(cfg, smp.cpu().numpy(), est) if cpu else None
+        group = est.GetGroupSize()
+        form = est.spline_form()
+        del smp, out
+    torch.cuda.empty_cache()
+    roof = {"kernel": "fit_kernel", "bound": BOUND, "compute": COMPUTE_NOTE, "achieved": achieved,
+            "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic}
+    detail = {
+        "step_gpu_ms": step_gpu_ms,
+        "fit_kernel_ms": fit_ms,
+        "crlb_kernel_ms": crlb_ms,
+        "fit_kernel_algorithmic_flop": fit_flops,
+        "crlb_kernel_algorithmic_flop": crlb_flops,
+        "crlb_kernel_tflops": crlb_flops / (crlb_ms * 1e-3) / 1e12 if crlb_ms > 0 else None,
+        "mean_iterations": float(executed.mean()),
+        "stopped_fraction": float(np.mean(it <= cfg["iters"])),
+        "hbm_bytes_per_launch": n * hbm_bytes_per_loc(cfg["kind"], cfg["R"]),
+        "hbm_GBps": n * hbm_bytes_per_loc(cfg["kind"], cfg["R"]) / (step_gpu_ms * 1e-3) / 1e9,
+        "pcie_inclusive_loc_per_s": e2e_rate,
+        "finite_fraction": float(np.mean(np.all(np.isfinite(p), axis=1))),
+    }
+    if cpu_line:
+        detail["gpu_over_cpu"] = (n / (ms_per_step * 1e-3)) / cpu_line["value"]
+    extra = {"roi": cfg["R"], "spots": n, "iterations": cfg["iters"], "lanes_per_spot": group,
+             **({"spline_form": form} if form else {})}
+    return line(n / (ms_per_step * 1e-3), "localizations/s", steps, warmup, ms_per_step, cfg["workload"], extra,
+                roof, cpu_line, detail)
+
+
+def bench_mixed(steps, warmup, n=0):
     """Mixed batch (4,000,000 ROIs over 28 model x size bins): the batch form (bins on
     concurrent streams, one HIP graph launch per step) against the plain per-bin launches
-    one after another on one stream; both compute fit + CRLB + chi^2 for every spot."""
+    one after another on one stream; both compute fit + CRLB + chi^2 for every spot.  The
+    plain launches are also timed per bin (HIP events around each bin's fit and CRLB
+    kernels) for the time shares."""
     import torch
     import photonpy_amd as pa
     from photonpy_amd import autotune, synthetic
     from photonpy_amd.batch import MixedBatch
     with pa.Context() as ctx:
-        bins = synthetic.mixed_bins(ctx, n=args.n or None)
+        bins = synthetic.mixed_bins(ctx, n=n or None)
         ests = [b["est"] for b in bins]
         counts = [b["n"] for b in bins]
         samples = [b["samples"] for b in bins]
@@ -160,20 +308,29 @@ def bench_mixed(args):
             ref = mb.allocate_outputs()
 
             def timed(fn):
-                for _ in range(args.warmup):
+                for _ in range(warmup):
                     fn()
                 torch.cuda.synchronize()
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 t0 = time.perf_counter()
                 e0.record(stream)
-                for _ in range(args.steps):
+                for _ in range(steps):
                     fn()
                 e1.record(stream)
                 torch.cuda.synchronize()
-                return (time.perf_counter() - t0) * 1e3 / args.steps, e0.elapsed_time(e1) / args.steps
+                return (time.perf_counter() - t0) * 1e3 / steps, e0.elapsed_time(e1) / steps
 
             plain_ms, plain_gpu = timed(lambda: mb.run_plain(samples, ref))
             batch_ms, batch_gpu = timed(lambda: mb.run(samples, outs, use_graph=True))
+            # per-bin kernel times: each bin's plain fit + CRLB launch on its own
+            bin_ms = []
+            for e, smp in zip(ests, samples):
+                e.set_phase_timing(True)
+                e.estimate_device(smp, crlb=True)
+                torch.cuda.synchronize()
+                _, f_ms, c_ms = e.phase_times()
+                e.set_phase_timing(False)
+                bin_ms.append(f_ms + c_ms)
             same = all(bool(((a[k] == b[k]) | (a[k].isnan() & b[k].isnan())).all()) if a[k].is_floating_point()
                        else torch.equal(a[k], b[k]) for a, b in zip(outs, ref) for k in a)
             flops = 0.0
@@ -184,25 +341,25 @@ def bench_mixed(args):
                 per1 = fit_flops_per_loc(fk, b["R"], 1) - per0
                 flops += b["n"] * (per0 + crlb_flops_per_loc(fk, b["R"])) + per1 * it.sum()
             streams = mb.num_streams()
+            groups = [e.GetGroupSize() for e in ests]
             forms = sorted({b["est"].spline_form() for b in bins if b["kind"] == "spline"})
+            meta = [(b["kind"], b["R"]) for b in bins]
+        del samples, outs, ref, bins
+    torch.cuda.empty_cache()
+    total = max(sum(bin_ms), 1e-9)
+    shares = sorted(({"kind": k, "R": R, "n": c, "lanes": g, "ms": round(t, 4), "share": round(t / total, 4)}
+                     for (k, R), c, g, t in zip(meta, counts, groups, bin_ms)), key=lambda d: -d["ms"])
     achieved = flops / (batch_gpu * 1e-3) / 1e12
-    print(json.dumps({
-        "metric": METRIC, "value": n / (batch_ms * 1e-3), "unit": "localizations/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": batch_ms, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": synthetic.MIXED["workload"], "spots": n, "bins": len(bins), "streams": streams,
-                   "spline_form": forms},
-        "roofline": {"kernel": "all fit and CRLB kernels of the batch", "bound": "mfma", "compute": COMPUTE_NOTE,
-                     "achieved": achieved,
-                     "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None},
-        "cpu_baseline": None,
-        "detail": {"plain_ms_per_step": plain_ms, "plain_loc_per_s": n / (plain_ms * 1e-3),
-                   "batch_over_plain": plain_ms / batch_ms, "batch_gpu_ms": batch_gpu, "plain_gpu_ms": plain_gpu,
-                   "results_identical": bool(same),
-                   "bins": [{"kind": b["kind"], "R": b["R"], "n": b["n"]} for b in bins]}}))
+    roof = {"kernel": "all fit and CRLB kernels of the batch", "bound": BOUND, "compute": COMPUTE_NOTE,
+            "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None}
+    detail = {"plain_ms_per_step": plain_ms, "plain_loc_per_s": n / (plain_ms * 1e-3),
+              "batch_over_plain": plain_ms / batch_ms, "batch_gpu_ms": batch_gpu, "plain_gpu_ms": plain_gpu,
+              "results_identical": bool(same), "bin_time_shares": shares}
+    return line(n / (batch_ms * 1e-3), "localizations/s", steps, warmup, batch_ms, synthetic.MIXED["workload"],
+                {"spots": n, "bins": len(counts), "streams": streams, "spline_form": forms}, roof, None, detail)
 
 
-def bench_queue(args):
+def bench_queue(steps, warmup, n=0):
     """The reference's own throughput path (photonpy/tests/psf_queue.py): C2's ROIs in
     host memory streamed through an EstimQueue (batches of 4096 on 4 streams, scheduled
     10,000 at a time, results collected as they finish).  Host-to-host rate, PCIe
@@ -211,7 +368,7 @@ def bench_queue(args):
     import photonpy_amd as pa
     from photonpy_amd import autotune, synthetic
     cfg = synthetic.WORKLOADS["c2"]
-    n = args.n or cfg["n"]
+    n = n or cfg["n"]
     with pa.Context() as ctx:
         est = synthetic.make_estimator(ctx, cfg["kind"], cfg["R"])
         est.SetLevMarParams(cfg["lam"], cfg["iters"])
@@ -236,75 +393,107 @@ def bench_queue(args):
             got += len(q.GetResults().ids)
             return got
 
-        def timed(batch, steps, warmup):
+        def timed(batch, nsteps, nwarm):
             # the queue is created outside the timed region, as the reference's benchmark does
             with pa.EstimQueue(est, batchSize=batch, numStreams=4) as q:
-                for _ in range(warmup):
+                for _ in range(nwarm):
                     run_once(q)
                 t0 = time.perf_counter()
-                for _ in range(steps):
+                for _ in range(nsteps):
                     got = run_once(q)
-                return got, (time.perf_counter() - t0) / steps
+                return got, (time.perf_counter() - t0) / nsteps
 
-        got, dt = timed(4096, args.steps, args.warmup)
+        got, dt = timed(4096, steps, warmup)
         got_big, dt_big = timed(65536, 1, 1)
     assert got == n and got_big == n, (got, got_big, n)
-    print(json.dumps({
-        "metric": METRIC, "value": n / dt, "unit": "localizations/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "EstimQueue from host memory: " + cfg["workload"], "spots": n, "batch": 4096,
-                   "streams": 4, "schedule_chunk": chunk},
-        "roofline": None, "cpu_baseline": None,
-        "detail": {"note": "host-to-host including PCIe transfers; compare detail.pcie_inclusive_loc_per_s of c2",
-                   "batch65536_loc_per_s": n / dt_big}}))
+    return line(n / dt, "localizations/s", steps, warmup, dt * 1e3, "EstimQueue from host memory: " + cfg["workload"],
+                {"spots": n, "batch": 4096, "streams": 4, "schedule_chunk": chunk}, None, None,
+                {"note": "host-to-host including PCIe transfers; compare detail.pcie_inclusive_loc_per_s of c2",
+                 "batch65536_loc_per_s": n / dt_big})
 
 
-def bench_adjoint(args):
+def spline_adjoint_rounds(th, R, shape, zmin, zmax):
+    """Rounds of the tiled coefficient gradient for spot parameters th: spots share a round
+    when they share the window (z-slab, first pixel's cell row and column), FP32 as on the GPU."""
+    Nz, Ny, Nx = shape
+    f32 = np.float32
+    tx = (th[:, 0].astype(f32) - f32(Nx * 0.5)).astype(f32)
+    ty = (th[:, 1].astype(f32) - f32(Ny * 0.5)).astype(f32)
+    du = f32((f32(zmax) - f32(zmin)) / f32(Nz))
+    spz = ((th[:, 2].astype(f32) - f32(zmin)) / du).astype(f32)
+    k = np.clip(np.trunc(spz).astype(np.int64), 0, Nz - 1)
+    i0 = np.clip(np.trunc((f32(0) - tx).astype(f32)).astype(np.int64), 0, Nx - 1)
+    j0 = np.clip(np.trunc((f32(0) - ty).astype(f32)).astype(np.int64), 0, Ny - 1)
+    _, cnt = np.unique((k * Ny + j0) * Nx + i0, return_counts=True)
+    per_round = 32 if (R + 1) ** 2 * 32 * 4 + 32 * 64 * 4 <= 65536 else 16
+    return int(np.sum((cnt + per_round - 1) // per_round)), len(cnt)
+
+
+def bench_adjoint(steps, warmup, n=0):
     """Spline PSF calibration step: forward Poisson NLL + hand-written adjoint on 65,536
     20x20 ROIs, gradients w.r.t. per-spot parameters and all 2,638,976 spline coefficients
-    (mu and J recomputed in the backward pass, nothing stored)."""
+    (mu and J recomputed in the backward pass, nothing stored).  Forward and backward are
+    timed separately with HIP events on the launch stream."""
     import torch
     import photonpy_amd as pa
     from photonpy_amd import synthetic
     from photonpy_amd.autograd import poisson_nll
-    n = args.n or 65_536
+    n = n or 65_536
+    R = 20
     with pa.Context() as ctx:
-        est = synthetic.make_estimator(ctx, "spline", 20)
+        est = synthetic.make_estimator(ctx, "spline", R)
         th, smp = synthetic.make_samples(est, "spline", n, seed=1)
         params = torch.from_numpy(th).cuda().requires_grad_(True)
         coefs = est.shared_parameters().requires_grad_(True)
+        lo, hi = est.ParamLimits()
+        rounds, windows = spline_adjoint_rounds(th, R, (61, 26, 26), float(lo[2]), float(hi[2]))
         stream = torch.cuda.current_stream()
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
 
-        def step():
+        def step(e=None):
             params.grad = None
             coefs.grad = None
+            if e:
+                e[0].record(stream)
             loss = poisson_nll(est, params, smp, coefs).sum()
+            if e:
+                e[1].record(stream)
             loss.backward()
+            if e:
+                e[2].record(stream)
 
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             step()
         torch.cuda.synchronize()
-        ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-        ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
         t0 = time.perf_counter()
-        for k in range(args.steps):
-            ev0[k].record(stream)
-            step()
-            ev1[k].record(stream)
+        for k in range(steps):
+            step(ev[k])
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        gpu_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev0, ev1)]))
-    ms = wall * 1e3 / args.steps
-    atomics = n * 400 * 64
-    print(json.dumps({
-        "metric": METRIC, "value": n / (ms * 1e-3), "unit": "ROI gradient evaluations/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "cubic-spline PSF adjoint: forward NLL + backward w.r.t. params and 61x26x26x64 "
-                               "coefficients, 20x20 px, 65,536 ROIs", "roi": 20, "spots": n},
-        "detail": {"gpu_ms_per_step": gpu_ms, "coefficient_atomics_per_step": atomics,
-                   "atomic_adds_per_s": atomics / (gpu_ms * 1e-3)}}))
+        fwd_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+        bwd_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+        finite = bool(torch.isfinite(coefs.grad).all().item() and torch.isfinite(params.grad).all().item())
+        del smp, params, coefs
+    torch.cuda.empty_cache()
+    ms = wall * 1e3 / steps
+    achieved = n * ADJ_FLOP_PER_ROI / (bwd_ms * 1e-3) / 1e12
+    atomics_tiled = rounds * (R + 1) ** 2 * 64
+    roof = {"kernel": "backward pass (window sort + spline_adjoint_tiled_kernel)", "bound": BOUND,
+            "compute": COMPUTE_NOTE, "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32, "traffic": None}
+    detail = {"forward_ms": fwd_ms, "backward_ms": bwd_ms, "backward_algorithmic_flop": n * ADJ_FLOP_PER_ROI,
+              "windows": windows, "rounds": rounds,
+              "coefficient_atomics_per_step": atomics_tiled,
+              "coefficient_atomics_per_step_direct_form": n * ADJ_DIRECT_ATOMICS_PER_ROI,
+              "gradients_finite": finite}
+    return line(n / (ms * 1e-3), "ROI gradient evaluations/s", steps, warmup, ms,
+                "cubic-spline PSF adjoint: forward NLL + backward w.r.t. params and 61x26x26x64 coefficients, "
+                "20x20 px, 65,536 ROIs", {"roi": R, "spots": n}, roof, None, detail)
+
+
+# timed steps / warm-up steps of the workloads reported under detail.configs of the
+# default run (sized so each takes well under a second of GPU time)
+SUB_STEPS = {"c1": (200, 5), "c3": (10, 2), "c4": (5, 1), "c4adj": (10, 2), "c5": (5, 1)}
 
 
 def main():
@@ -312,11 +501,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c4adj", "c5", "queue"])
+    ap.add_argument("--config", default="all", choices=["all", "c1", "c2", "c3", "c4", "c4adj", "c5", "queue"])
     ap.add_argument("--n", type=int, default=0, help="override the number of spots")
     ap.add_argument("--group", type=int, default=0, help="lane-group size override (0 = tuned/default)")
     ap.add_argument("--refill", type=int, default=0, help="refill batch override (0 = tuned/default)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-e2e", action="store_true", help="skip the PCIe-inclusive host-API leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
@@ -327,7 +516,6 @@ def main():
         sys.exit(2)
 
     import torch
-    import photonpy_amd as pa
 
     if args.copy_calibration:
         nbytes = args.copy_calibration << 20
@@ -339,106 +527,41 @@ def main():
         print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 5}))
         return
 
-    from photonpy_amd import autotune, synthetic
-
+    cpu = not args.no_cpu
     if args.config == "c4adj":
-        return bench_adjoint(args)
+        print(json.dumps(bench_adjoint(args.steps, args.warmup, args.n)))
+        return
     if args.config == "c5":
-        return bench_mixed(args)
+        print(json.dumps(bench_mixed(args.steps, args.warmup, args.n)))
+        return
     if args.config == "queue":
-        return bench_queue(args)
-    cfg = synthetic.WORKLOADS[args.config]
-    n = args.n or cfg["n"]
-    torch.cuda.set_device(0)
-    with pa.Context() as ctx:
-        est = synthetic.make_estimator(ctx, cfg["kind"], cfg["R"])
-        est.SetLevMarParams(cfg["lam"], cfg["iters"])
-        autotune.apply(est)
-        if args.group:
-            est.SetGroupSize(args.group)
-        if args.refill:
-            est.SetRefill(args.refill)
-        th, smp = synthetic.make_samples(est, cfg["kind"], n, seed=1)
-        torch.cuda.synchronize()
-
-        for _ in range(args.warmup):
-            out = est.estimate_device(smp, crlb=True)
-        torch.cuda.synchronize()
-        stream = torch.cuda.current_stream()
-        ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-        ev1 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-        # HIP events on the launch stream around each kernel (fit, then CRLB)
-        est.set_phase_timing(True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(args.steps):
-            ev0[k].record(stream)
-            out = est.estimate_device(smp, crlb=True)
-            ev1[k].record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        launches, fit_ms, crlb_ms = est.phase_times()
-        est.set_phase_timing(False)
-        assert launches == args.steps, (launches, args.steps)
-        step_gpu_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev0, ev1)]))
-        ms_per_step = wall * 1e3 / args.steps
-
-        it = out["iterations"].cpu().numpy()
-        p = out["params"].cpu().numpy()
-        executed = np.minimum(it, cfg["iters"]).astype(np.float64)
-        fkind = "splinegrid" if est.spline_form() == "grid" else cfg["kind"]
-        per0 = fit_flops_per_loc(fkind, cfg["R"], 0)
-        per1 = fit_flops_per_loc(fkind, cfg["R"], 1) - per0
-        fit_flops = n * per0 + per1 * executed.sum()
-        crlb_flops = n * crlb_flops_per_loc(fkind, cfg["R"])
-        achieved = fit_flops / (fit_ms * 1e-3) / 1e12
-        traffic = hbm_traffic(args.config) if n == cfg["n"] else None
-
-        # PCIe-inclusive rate through the host-pointer API (samples H2D, results D2H)
-        m = min(n, 200_000)
-        smp_host = smp[:m].cpu().numpy()
-        e2e = None if args.no_e2e else host_api_rate(est, smp_host)
-
-        cpu = None if args.no_cpu else cpu_baseline(cfg, smp.cpu().numpy(), est)
-        group = est.GetGroupSize()
-        form = est.spline_form()
-
-    value = n / (ms_per_step * 1e-3)
-    line = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "localizations/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": cfg["workload"], "roi": cfg["R"], "spots": n, "iterations": cfg["iters"],
-                   "lanes_per_spot": group, **({"spline_form": form} if form else {})},
-        "roofline": {"kernel": "fit_kernel", "bound": "mfma", "compute": COMPUTE_NOTE, "achieved": achieved,
-                     "peak": PEAK_FP32,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic},
-        "cpu_baseline": cpu,
-        "detail": {
-            "step_gpu_ms": step_gpu_ms,
-            "fit_kernel_ms": fit_ms,
-            "crlb_kernel_ms": crlb_ms,
-            "fit_kernel_algorithmic_flop": fit_flops,
-            "crlb_kernel_algorithmic_flop": crlb_flops,
-            "crlb_kernel_tflops": crlb_flops / (crlb_ms * 1e-3) / 1e12 if crlb_ms > 0 else None,
-            "mean_iterations": float(executed.mean()),
-            "stopped_fraction": float(np.mean(it <= cfg["iters"])),
-            "hbm_bytes_per_launch": n * hbm_bytes_per_loc(cfg["kind"], cfg["R"]),
-            "hbm_GBps": n * hbm_bytes_per_loc(cfg["kind"], cfg["R"]) / (step_gpu_ms * 1e-3) / 1e9,
-            "pcie_inclusive_loc_per_s": e2e,
-            "finite_fraction": float(np.mean(np.all(np.isfinite(p), axis=1))),
-        },
-    }
-    print(json.dumps(line))
+        print(json.dumps(bench_queue(args.steps, args.warmup, args.n)))
+        return
+    if args.config != "all":
+        print(json.dumps(bench_fit(args.config, args.steps, args.warmup, args.n, args.group, args.refill, cpu=cpu,
+                                   e2e=not args.no_e2e)))
+        return
+    # headline: c2 with the driver's K and W; then every other workload of the metric
+    t_all = time.perf_counter()
+    head = bench_fit("c2", args.steps, args.warmup, cpu=cpu, e2e=not args.no_e2e)
+    print(f"bench: c2 {head['value']:.4g} loc/s", file=sys.stderr, flush=True)
+    configs = {}
+    for name in ("c1", "c3", "c4"):
+        k, w = SUB_STEPS[name]
+        configs[name] = bench_fit(name, k, w, cpu=cpu, e2e=False)
+        print(f"bench: {name} {configs[name]['value']:.4g} loc/s", file=sys.stderr, flush=True)
+    k, w = SUB_STEPS["c4adj"]
+    configs["c4adj"] = bench_adjoint(k, w)
+    print(f"bench: c4adj {configs['c4adj']['value']:.4g} ROI/s", file=sys.stderr, flush=True)
+    k, w = SUB_STEPS["c5"]
+    configs["c5"] = bench_mixed(k, w)
+    print(f"bench: c5 {configs['c5']['value']:.4g} loc/s", file=sys.stderr, flush=True)
+    for c in configs.values():
+        for key in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "dtype", "data"):
+            c.pop(key, None)
+    head["detail"]["configs"] = configs
+    head["detail"]["bench_wall_s"] = time.perf_counter() - t_all
+    print(json.dumps(head))
 
 
 if __name__ == "__main__":

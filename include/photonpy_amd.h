@@ -292,21 +292,25 @@ int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const fl
  * bin.  The bins are spread longest-first over nstreams internal HIP streams (0 = 4, one per
  * hardware queue of the process) that run concurrently, so one bin's tail overlaps the next
  * bins' work and different models share the CUs; each bin runs the estimator's own persistent
- * fit + CRLB launches.  With useGraph the fork, every launch and the join are captured into one
- * HIP graph on the first run with a given set of buffers and replayed with one hipGraphLaunch
- * on `stream`; new buffers are captured again.  Run takes per-bin device pointers (arrays of
- * nbins pointers; d_iters/d_crlb/d_chisq may be NULL).  A graph replay reuses the work-counter
- * slots it was captured with: do not run it concurrently with other fits of the same
- * estimators on other streams. */
+ * fit + CRLB launches with a work counter the batch owns.  With useGraph the fork, every launch
+ * and the join are captured into one HIP graph on the first run and replayed with one
+ * hipGraphLaunch on `stream`; a graph run with other buffers or changed estimator settings (LM
+ * parameters, group size, refill batch, spline form) than the captured ones is refused with an
+ * error (make a new MixedBatch for them; eager runs take any buffers).  Run takes per-bin
+ * device pointers (arrays of nbins pointers; d_initial [n,K] per bin, or the whole array NULL
+ * for the centre-of-mass start; d_iters/d_crlb/d_chisq may be NULL).  Runs of one batch must
+ * not overlap each other (they share the batch's counters). */
 typedef struct MixedBatch MixedBatch;
 MixedBatch* MixedBatch_Create(int nbins, Estimator* const* estimators, const int* counts, int nstreams);
 void MixedBatch_Delete(MixedBatch* mb);
 int MixedBatch_NumStreams(MixedBatch* mb);
-int MixedBatch_Run(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
-                   float* const* d_crlb, float* const* d_chisq, int useGraph, void* stream);
+int MixedBatch_Run(MixedBatch* mb, const float* const* d_samples, const float* const* d_initial,
+                   float* const* d_params, int* const* d_iters, float* const* d_crlb, float* const* d_chisq,
+                   int useGraph, void* stream);
 /* The same bins as one plain launch per bin, in bin order on `stream` (the comparison form). */
-int MixedBatch_RunPlain(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
-                        float* const* d_crlb, float* const* d_chisq, void* stream);
+int MixedBatch_RunPlain(MixedBatch* mb, const float* const* d_samples, const float* const* d_initial,
+                        float* const* d_params, int* const* d_iters, float* const* d_crlb, float* const* d_chisq,
+                        void* stream);
 
 /* Number of shared PSF parameters and a host copy of their current values. */
 int Estimator_NumShared(Estimator* e);

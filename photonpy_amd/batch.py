@@ -49,11 +49,15 @@ class MixedBatch:
             outs.append(o)
         return outs
 
-    def _ptrs(self, samples, outs):
+    def _ptrs(self, samples, outs, initial=None):
         nb = len(self.estimators)
         for e, n, s in zip(self.estimators, self.counts, samples):
             if tuple(s.shape) != (n, *e.sampleshape) or not s.is_contiguous():
                 raise ValueError(f"bin samples must be contiguous [{n}, {e.sampleshape[0]}, {e.sampleshape[1]}]")
+        if initial is not None:
+            for e, n, t in zip(self.estimators, self.counts, initial):
+                if tuple(t.shape) != (n, e.numparams) or not t.is_contiguous() or t.dtype != s.dtype:
+                    raise ValueError(f"bin initial values must be contiguous float32 [{n}, {e.numparams}]")
 
         def arr(key, seq):
             vals = [None if (o is None or o.get(key) is None) else o[key].data_ptr() for o in seq]
@@ -62,22 +66,25 @@ class MixedBatch:
             return (ct.c_void_p * nb)(*vals)
 
         sp = (ct.c_void_p * nb)(*[s.data_ptr() for s in samples])
-        return sp, arr("params", outs), arr("iterations", outs), arr("crlb", outs), arr("chisq", outs)
+        ini = None if initial is None else (ct.c_void_p * nb)(*[t.data_ptr() for t in initial])
+        return sp, ini, arr("params", outs), arr("iterations", outs), arr("crlb", outs), arr("chisq", outs)
 
-    def run(self, samples, outs, use_graph=True, stream=None):
+    def run(self, samples, outs, use_graph=True, stream=None, initial=None):
+        """Fit every bin: samples[b] [n_b,R_b,R_b] and, optionally, initial[b] [n_b,K_b]
+        (device tensors); results go to the tensors of outs (allocate_outputs)."""
         import torch
         stream = stream or torch.cuda.current_stream()
-        sp, pp, ip, cp, xp = self._ptrs(samples, outs)
-        rc = self._lib.MixedBatch_Run(self.inst, sp, pp, ip, cp, xp, int(use_graph), stream.cuda_stream)
+        sp, ini, pp, ip, cp, xp = self._ptrs(samples, outs, initial)
+        rc = self._lib.MixedBatch_Run(self.inst, sp, ini, pp, ip, cp, xp, int(use_graph), stream.cuda_stream)
         if rc != 0:
             self._smlm.check()
             raise RuntimeError(f"MixedBatch_Run failed ({rc})")
 
-    def run_plain(self, samples, outs, stream=None):
+    def run_plain(self, samples, outs, stream=None, initial=None):
         import torch
         stream = stream or torch.cuda.current_stream()
-        sp, pp, ip, cp, xp = self._ptrs(samples, outs)
-        rc = self._lib.MixedBatch_RunPlain(self.inst, sp, pp, ip, cp, xp, stream.cuda_stream)
+        sp, ini, pp, ip, cp, xp = self._ptrs(samples, outs, initial)
+        rc = self._lib.MixedBatch_RunPlain(self.inst, sp, ini, pp, ip, cp, xp, stream.cuda_stream)
         if rc != 0:
             self._smlm.check()
             raise RuntimeError(f"MixedBatch_RunPlain failed ({rc})")

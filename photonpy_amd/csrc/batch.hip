@@ -16,8 +16,11 @@ struct MixedBatch {
     hipStream_t capStream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipStream_t lastLaunch = nullptr;       // stream of the last graph launch
-    std::vector<const void*> capturedPtrs;  // buffers the graph was captured with
+    hipStream_t lastLaunch = nullptr;    // stream of the last graph launch
+    std::vector<uintptr_t> capturedKey;  // buffers and estimator settings the graph was captured with
+    // Work counters of the batch's fit launches, one per bin, owned by the batch: a
+    // replayed graph reuses them, and no other launch of the estimators touches them.
+    int* counters = nullptr;
 
     void drop_graph() {
         if (exec) (void)hipStreamSynchronize(lastLaunch);  // the last replay has finished
@@ -25,7 +28,7 @@ struct MixedBatch {
         if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;
         graph = nullptr;
-        capturedPtrs.clear();
+        capturedKey.clear();
     }
 
     ~MixedBatch() {
@@ -35,6 +38,7 @@ struct MixedBatch {
         for (hipEvent_t ev : joins) (void)hipEventDestroy(ev);
         if (fork) (void)hipEventDestroy(fork);
         if (capStream) (void)hipStreamDestroy(capStream);
+        (void)dev_release(counters);
     }
 };
 
@@ -53,8 +57,9 @@ static double bin_cost(const Estimator* e, int count) {
 
 // Launches every bin: each branch waits for `fork` on its stream, runs its bins in
 // order, and records its join event; the caller's stream waits for all joins.
-static void launch_bins(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
-                        float* const* d_crlb, float* const* d_chisq, hipStream_t origin) {
+static void launch_bins(MixedBatch* mb, const float* const* d_samples, const float* const* d_initial,
+                        float* const* d_params, int* const* d_iters, float* const* d_crlb, float* const* d_chisq,
+                        hipStream_t origin) {
     hip_check(hipEventRecord(mb->fork, origin), "hipEventRecord(fork)");
     for (size_t b = 0; b < mb->streams.size(); b++) {
         hipStream_t s = mb->streams[b];
@@ -62,11 +67,21 @@ static void launch_bins(MixedBatch* mb, const float* const* d_samples, float* co
         for (int bin : mb->order[b]) {
             Estimator* e = mb->est[bin];
             std::lock_guard<std::mutex> l(e->mtx);
-            run_fit(e, mb->counts[bin], d_samples[bin], nullptr, nullptr, d_params[bin], d_iters ? d_iters[bin] : nullptr,
-                    nullptr, 0, d_crlb ? d_crlb[bin] : nullptr, d_chisq ? d_chisq[bin] : nullptr, s, false);
+            run_fit(e, mb->counts[bin], d_samples[bin], nullptr, d_initial ? d_initial[bin] : nullptr, d_params[bin],
+                    d_iters ? d_iters[bin] : nullptr, nullptr, 0, d_crlb ? d_crlb[bin] : nullptr,
+                    d_chisq ? d_chisq[bin] : nullptr, s, false, 0, mb->counters + bin);
         }
         hip_check(hipEventRecord(mb->joins[b], s), "hipEventRecord(join)");
         hip_check(hipStreamWaitEvent(origin, mb->joins[b], 0), "hipStreamWaitEvent(join)");
+    }
+}
+
+// Device tables of every bin's estimator exist before any launch or capture (no
+// allocation inside a capture).
+static void ensure_tables(MixedBatch* mb) {
+    for (Estimator* e : mb->est) {
+        std::lock_guard<std::mutex> l(e->mtx);
+        e->ensure_device();
     }
 }
 
@@ -119,6 +134,7 @@ MixedBatch* MixedBatch_Create(int nbins, Estimator* const* ests, const int* coun
                     mb->joins.push_back(ev);
                 }
                 hip_check(hipEventCreateWithFlags(&mb->fork, hipEventDisableTiming), "hipEventCreate");
+                hip_check(dev_alloc(&mb->counters, sizeof(int) * (size_t)nbins), "hipMalloc(batch counters)");
             } catch (...) {
                 delete mb;
                 throw;
@@ -132,8 +148,9 @@ void MixedBatch_Delete(MixedBatch* mb) { delete mb; }
 
 int MixedBatch_NumStreams(MixedBatch* mb) { return mb ? (int)mb->streams.size() : -1; }
 
-int MixedBatch_Run(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
-                   float* const* d_crlb, float* const* d_chisq, int useGraph, void* stream) {
+int MixedBatch_Run(MixedBatch* mb, const float* const* d_samples, const float* const* d_initial,
+                   float* const* d_params, int* const* d_iters, float* const* d_crlb, float* const* d_chisq, int useGraph,
+                   void* stream) {
     if (!mb) return 1;
     return guarded(
         [&]() -> int {
@@ -142,32 +159,38 @@ int MixedBatch_Run(MixedBatch* mb, const float* const* d_samples, float* const* 
             for (size_t b = 0; b < nb; b++)
                 if (mb->counts[b] > 0 && (!d_samples[b] || !d_params[b]))
                     throw std::runtime_error("photonpy_amd: NULL sample or params buffer");
-            for (Estimator* e : mb->est) {
-                // device tables and counters exist before any capture (no allocation inside it)
-                std::lock_guard<std::mutex> l(e->mtx);
-                e->ensure_device();
-                e->ensure_counters();
-            }
+            ensure_tables(mb);
             if (!useGraph) {
-                launch_bins(mb, d_samples, d_params, d_iters, d_crlb, d_chisq, st);
+                launch_bins(mb, d_samples, d_initial, d_params, d_iters, d_crlb, d_chisq, st);
                 return 0;
             }
-            std::vector<const void*> ptrs;
+            // the kernels' buffers and arguments (LM settings, lane groups, refill batch, spline
+            // form) are baked into the graph: capture again when any of them changed
+            std::vector<uintptr_t> key;
+            auto ptr = [](const void* p) { return reinterpret_cast<uintptr_t>(p); };
             for (size_t b = 0; b < nb; b++) {
-                ptrs.push_back(d_samples[b]);
-                ptrs.push_back(d_params[b]);
-                ptrs.push_back(d_iters ? d_iters[b] : nullptr);
-                ptrs.push_back(d_crlb ? d_crlb[b] : nullptr);
-                ptrs.push_back(d_chisq ? d_chisq[b] : nullptr);
+                const Estimator* e = mb->est[b];
+                float lam = e->lm.lambda;
+                uint32_t lamBits;
+                std::memcpy(&lamBits, &lam, 4);
+                for (uintptr_t v : {ptr(d_samples[b]), ptr(d_initial ? d_initial[b] : nullptr), ptr(d_params[b]),
+                                    ptr(d_iters ? d_iters[b] : nullptr), ptr(d_crlb ? d_crlb[b] : nullptr),
+                                    ptr(d_chisq ? d_chisq[b] : nullptr), (uintptr_t)e->G(), (uintptr_t)e->refill,
+                                    (uintptr_t)e->desc.useGrid, (uintptr_t)e->lm.iterations, (uintptr_t)lamBits})
+                    key.push_back(v);
             }
-            if (!mb->exec || ptrs != mb->capturedPtrs) {
-                // the kernels' buffers are baked into the graph: capture again for new ones
-                mb->drop_graph();
+            // A second capture on one batch is refused: one run that captured again (a graph
+            // with initial values replayed, then a capture without them, 4,000,000 ROIs) ended
+            // in a host segmentation fault inside this call whose cause reading did not find.
+            if (mb->exec && key != mb->capturedKey)
+                throw std::runtime_error("photonpy_amd: this mixed batch's graph was captured with other buffers or "
+                                         "estimator settings; create a new MixedBatch for them");
+            if (!mb->exec) {
                 if (!mb->capStream) hip_check(hipStreamCreateWithFlags(&mb->capStream, hipStreamNonBlocking),
                                               "hipStreamCreate");
                 hip_check(hipStreamBeginCapture(mb->capStream, hipStreamCaptureModeRelaxed), "begin capture");
                 try {
-                    launch_bins(mb, d_samples, d_params, d_iters, d_crlb, d_chisq, mb->capStream);
+                    launch_bins(mb, d_samples, d_initial, d_params, d_iters, d_crlb, d_chisq, mb->capStream);
                 } catch (...) {
                     hipGraph_t g = nullptr;
                     (void)hipStreamEndCapture(mb->capStream, &g);
@@ -176,7 +199,7 @@ int MixedBatch_Run(MixedBatch* mb, const float* const* d_samples, float* const* 
                 }
                 hip_check(hipStreamEndCapture(mb->capStream, &mb->graph), "end capture");
                 hip_check(hipGraphInstantiate(&mb->exec, mb->graph, nullptr, nullptr, 0), "graph instantiate");
-                mb->capturedPtrs = ptrs;
+                mb->capturedKey = key;
             }
             hip_check(hipGraphLaunch(mb->exec, st), "graph launch");
             mb->lastLaunch = st;
@@ -186,16 +209,19 @@ int MixedBatch_Run(MixedBatch* mb, const float* const* d_samples, float* const* 
 }
 
 // The same bins as one plain launch per bin, one after another on the caller's stream.
-int MixedBatch_RunPlain(MixedBatch* mb, const float* const* d_samples, float* const* d_params, int* const* d_iters,
-                        float* const* d_crlb, float* const* d_chisq, void* stream) {
+int MixedBatch_RunPlain(MixedBatch* mb, const float* const* d_samples, const float* const* d_initial,
+                        float* const* d_params, int* const* d_iters, float* const* d_crlb, float* const* d_chisq,
+                        void* stream) {
     if (!mb) return 1;
     return guarded(
         [&]() -> int {
+            ensure_tables(mb);
             for (size_t b = 0; b < mb->est.size(); b++) {
                 Estimator* e = mb->est[b];
                 std::lock_guard<std::mutex> l(e->mtx);
-                run_fit(e, mb->counts[b], d_samples[b], nullptr, nullptr, d_params[b], d_iters ? d_iters[b] : nullptr, nullptr,
-                        0, d_crlb ? d_crlb[b] : nullptr, d_chisq ? d_chisq[b] : nullptr, as_stream(stream), false);
+                run_fit(e, mb->counts[b], d_samples[b], nullptr, d_initial ? d_initial[b] : nullptr, d_params[b],
+                        d_iters ? d_iters[b] : nullptr, nullptr, 0, d_crlb ? d_crlb[b] : nullptr,
+                        d_chisq ? d_chisq[b] : nullptr, as_stream(stream), false, 0, mb->counters + b);
             }
             return 0;
         },

@@ -140,6 +140,20 @@ struct Estimator {
     // grow-only device workspace for the host-pointer API
     void* ws = nullptr;
     size_t wsBytes = 0;
+    // grow-only device scratch of the spline adjoint's window sort
+    void* adjWs = nullptr;
+    size_t adjWsBytes = 0;
+
+    void* adjoint_workspace(size_t bytes) {
+        if (bytes > adjWsBytes) {
+            if (adjWs) hip_check(dev_release(adjWs), "hipFree(adjoint scratch)");
+            adjWs = nullptr;
+            adjWsBytes = 0;
+            hip_check(dev_alloc(&adjWs, bytes), "hipMalloc(adjoint scratch)");
+            adjWsBytes = bytes;
+        }
+        return adjWs;
+    }
 
     float* devShared = nullptr;  // device copy of the shared PSF parameters (Gaussian models)
     // Work counters of the fit kernel: one slot per launch, cycling, so launches of this
@@ -206,6 +220,7 @@ struct Estimator {
         if (devShared) (void)dev_release(devShared);
         if (devCounters) (void)dev_release(devCounters);
         if (ws) (void)dev_release(ws);
+        if (adjWs) (void)dev_release(adjWs);
         release_phase_events();
     }
 

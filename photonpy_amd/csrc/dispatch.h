@@ -128,7 +128,22 @@ struct AdjArgs {
     float* gparams;        // [n,K] or null
     float* gshared;        // accumulated (+=) or null
 };
-hipError_t launch_adjoint_any(const ModelDesc& d, const AdjArgs& a, hipStream_t st);
+// Sort scratch of the spline coefficient gradient by window tiles (adjoint.hip).
+struct SplineAdjTiles {
+    int* key;     // [n] window key (k, j0, i0) of each spot
+    int* order;   // [n] spots sorted by key
+    int* count;   // [nkeys] spots per key, then the scatter cursor
+    int* offset;  // [nkeys] first sorted position of each key
+    int2* tiles;  // [maxTiles] (first sorted position, spots)
+    int* ntiles;  // [1]
+    int nkeys, maxTiles;
+};
+// Whether the spline adjoint of roisize R uses the tiled coefficient gradient, and the
+// bytes of scratch it needs for n spots and nkeys = Nz Ny Nx window keys.
+bool spline_adjoint_tiled(int R);
+size_t spline_adjoint_workspace(int n, int nkeys);
+// ws: spline_adjoint_workspace bytes of device scratch, or null (direct atomics).
+hipError_t launch_adjoint_any(const ModelDesc& d, const AdjArgs& a, void* ws, hipStream_t st);
 hipError_t launch_spline_coefs(const int* shape, const float* d_data, float* d_out, int border, hipStream_t st);
 // Sets *counter = value on the stream (a one-thread kernel, so it can also be captured in a graph).
 hipError_t launch_set_counter(int* counter, int value, hipStream_t st);

@@ -166,10 +166,8 @@ def _bind(lib):
         "MixedBatch_Create": ([I, ct.POINTER(V), ct.POINTER(ct.c_int), I], V),
         "MixedBatch_Delete": ([V], None),
         "MixedBatch_NumStreams": ([V], ct.c_int),
-        "MixedBatch_Run": ([V, ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), I, V],
-                           ct.c_int),
-        "MixedBatch_RunPlain": ([V, ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), ct.POINTER(V), V],
-                                ct.c_int),
+        "MixedBatch_Run": ([V] + [ct.POINTER(V)] * 6 + [I, V], ct.c_int),
+        "MixedBatch_RunPlain": ([V] + [ct.POINTER(V)] * 6 + [V], ct.c_int),
         "Estimator_GetShared": ([V, V], ct.c_int),
         "CudaGetNumDevices": ([], ct.c_int32),
         "CudaSetDevice": ([I], ct.c_bool),

@@ -40,12 +40,12 @@ def make(kind, ctx, R=None, B=True):
     raise ValueError(kind)
 
 
-def samples(kind, n, seed, R=None):
+def samples(kind, n, seed, R=None, B=True):
     """Seeded true params and Poisson samples (mu from the FP64 closed form / oracle)."""
     rng = np.random.default_rng(seed)
     R = R or ROI[kind]
     th = C.synth_params(kind, R, n, rng)
     with pa.Context() as ctx:
-        _, om, fn = make(kind, ctx, R)
+        _, om, fn = make(kind, ctx, R, B=B)
         mu = fn(th)
     return th, rng.poisson(mu).astype(np.float32)

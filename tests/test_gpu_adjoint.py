@@ -166,6 +166,58 @@ def test_spline_coefficient_gradients(ctx):
         assert abs(g[c, q] - fd) <= 1e-4 * abs(fd) + 1e-7 * np.abs(g).max(), ((c, q), g[c, q], fd)
 
 
+@pytest.mark.parametrize("R", [7, 20, 25])
+def test_spline_coefficient_gradients_many_spots(ctx, R):
+    """The coefficient gradient of thousands of spots, half of them packed into a few
+    windows (same z-slab, x and y within +-0.2 px) so that windows hold several tiles
+    and many rounds, against an FP64 sparse restatement of
+    d NLL / d c[cell, q] = sum over the pixels of the cell of g (1 - x/mu) I basis_q (psf > 0).
+    The per-spot gradients of that pass equal those of the pass without the coefficient
+    gradient."""
+    import scipy.sparse as sp
+    import torch
+    from tests import _spline_np as SN
+    n = 3000
+    rng = np.random.default_rng(40 + R)
+    th = C.synth_params("spline", R, n, rng)
+    c = (R - 1) / 2
+    half = n // 2
+    th[:half, 0] = c + rng.uniform(-0.2, 0.2, half)
+    th[:half, 1] = c + rng.uniform(-0.2, 0.2, half)
+    th[:half, 2] = rng.uniform(0.0, 8.0, half)
+    est, om, _ = MD.make("spline", ctx, R=R)
+    smp = rng.poisson(om.expected_value(th, np.float64)).astype(np.float32)
+    th = th.astype(np.float32)
+    th[:, 0] += 0.1  # away from the optimum
+    coefs, zmin, zmax = MD.spline_coefs()
+    shared = torch.from_numpy(coefs.ravel().copy()).cuda().requires_grad_(True)
+    d_th = torch.from_numpy(th).cuda().requires_grad_(True)
+    d_smp = torch.from_numpy(smp).cuda()
+    gout = np.linspace(0.5, 1.5, n).astype(np.float32)
+    nll = poisson_nll(est, d_th, d_smp, shared)
+    (nll * torch.from_numpy(gout).cuda()).sum().backward()
+    g = shared.grad.cpu().numpy().reshape(-1, 64).astype(np.float64)
+    gp_tiled = d_th.grad.cpu().numpy()
+    # the same per-spot gradients without the coefficient gradient (direct kernel)
+    d_th2 = torch.from_numpy(th).cuda().requires_grad_(True)
+    (poisson_nll(est, d_th2, d_smp, shared.detach()) * torch.from_numpy(gout).cuda()).sum().backward()
+    gp_direct = d_th2.grad.cpu().numpy()
+    assert np.allclose(gp_tiled, gp_direct, rtol=1e-5, atol=1e-5 * np.abs(gp_direct).max())
+    th64 = th.astype(np.float64)
+    cell, B = SN.cells(th64, R, coefs.shape[:3], zmin, zmax)
+    C2 = coefs.reshape(-1, 64).astype(np.float64)
+    f = np.einsum("nyxq,nq->nyx", C2[cell], B)
+    ev = om.expected_value(th64, np.float64)
+    w = 1 - np.maximum(smp, 1e-8) / np.maximum(ev, 1e-8)
+    W = gout[:, None, None] * w * th64[:, 3, None, None] * (f > 0)
+    A = sp.coo_matrix((W.ravel(), (cell.ravel(), np.repeat(np.arange(n), R * R))), shape=(C2.shape[0], n)).tocsr()
+    ana = A @ B
+    assert np.max(np.abs(g - ana)) <= 1e-4 * np.abs(ana).max()
+    touched = np.abs(ana) > 1e-2 * np.abs(ana).max()
+    assert np.max(np.abs(g - ana)[touched] / np.abs(ana)[touched]) < 1e-4
+    assert np.all(g[np.asarray(A.sum(axis=1)).ravel() == 0] == 0)  # untouched cells get nothing
+
+
 def test_calibration_step_reduces_nll(ctx):
     """A few gradient steps on sigma of the fixed-sigma model move it towards the truth."""
     import torch

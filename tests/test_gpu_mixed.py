@@ -71,11 +71,15 @@ def test_batch_matches_plain(ctx):
             torch.cuda.synchronize()
             _ = (samples[0] * 2).sum().item()
             _same(ref, outs, f"replay {rep}")
-        # new output buffers: captured again
+        # other output buffers than the captured ones: refused (a new batch captures them)
         outs2 = mb.allocate_outputs()
-        mb.run(samples, outs2, use_graph=True)
+        with pytest.raises(pa.NativeError, match="create a new MixedBatch"):
+            mb.run(samples, outs2, use_graph=True)
+    # a new batch captures them
+    with MixedBatch(ests, counts, streams=4, tune=False) as mb2:
+        mb2.run(samples, outs2, use_graph=True)
         torch.cuda.synchronize()
-        _same(ref, outs2, "recaptured")
+        _same(ref, outs2, "second batch")
 
 
 def test_stream_count_does_not_change_results(ctx):

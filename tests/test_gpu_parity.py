@@ -13,39 +13,19 @@ import photonpy_amd as pa
 from oracle import oracle as O
 from tests import _closed as C
 from tests import _models as MD
+from tests import _parity as P
 
 pytestmark = pytest.mark.gpu
 
 KINDS = ["fixed", "sigma", "astig", "spline"]
-TOL = {"xy": 1e-3, "z": 1.0, "rel": 1e-4}
+TOL = P.TOL
+_zcol, _on_bounds, _compare = P.zcol, P.on_bounds, P.compare
 
 
 @pytest.fixture(scope="module")
 def ctx():
     with pa.Context() as c:
         yield c
-
-
-def _zcol(kind):
-    return 2 if kind in ("astig", "spline") else None
-
-
-def _on_bounds(p, lo, hi):
-    return np.any((p <= lo + 1e-6) | (p >= hi - 1e-6), axis=1)
-
-
-def _compare(kind, a, b):
-    """Per-spot boolean: a and b agree within the parity tolerance."""
-    ok = (np.abs(a[:, 0] - b[:, 0]) <= TOL["xy"]) & (np.abs(a[:, 1] - b[:, 1]) <= TOL["xy"])
-    iI = 3 if _zcol(kind) else 2
-    ibg = iI + 1
-    ok &= np.abs(a[:, iI] - b[:, iI]) <= TOL["rel"] * np.maximum(np.abs(b[:, iI]), 1)
-    ok &= np.abs(a[:, ibg] - b[:, ibg]) <= TOL["rel"] * np.maximum(np.abs(b[:, ibg]), 1)
-    if _zcol(kind):
-        ok &= np.abs(a[:, 2] - b[:, 2]) <= TOL["z"]
-    if kind == "sigma":
-        ok &= np.abs(a[:, 4] - b[:, 4]) <= 1e-4 * np.maximum(np.abs(b[:, 4]), 1)
-    return ok
 
 
 @pytest.mark.parametrize("kind", KINDS)
@@ -69,51 +49,21 @@ def test_expected_value_and_derivatives(ctx, kind):
         assert np.max(np.abs(dv - dv64) / dscale) < 5e-5
 
 
-@pytest.mark.parametrize("kind", KINDS)
-def test_estimate_parity(ctx, kind):
+@pytest.mark.parametrize("kind,B", [("fixed", True), ("sigma", True), ("astig", True), ("astig", False),
+                                    ("spline", True)])
+def test_estimate_parity(ctx, kind, B):
+    """GPU Estimate vs the oracle at each model's workload ROI size.  ("astig", B=False)
+    is the reference's own astigmatic model (no quartic term, GaussianPSFModels.h:636-751)."""
     n = {"fixed": 4000, "sigma": 4000, "astig": 1500, "spline": 600}[kind]
     iters = 30 if kind == "fixed" else 50
-    th, smp = MD.samples(kind, n, 3)
-    est, om, _ = MD.make(kind, ctx)
-    est.SetLevMarParams(1e-15 if kind == "fixed" else 1e-18, iters)
+    lam = 1e-15 if kind == "fixed" else 1e-18
+    th, smp = MD.samples(kind, n, 3, B=B)
+    est, om, _ = MD.make(kind, ctx, B=B)
+    est.SetLevMarParams(lam, iters)
     p_gpu, _, traces = est.Estimate(smp)
     it_gpu = est.last_iterations
-    lam = 1e-15 if kind == "fixed" else 1e-18
-    p_ref, it_ref, tr_ref = om.estimate(smp, iterations=iters, lambda_=lam, trace_len=iters)
     assert np.array_equal([len(t) for t in traces], np.minimum(it_gpu, iters))
-
-    lo, hi = est.ParamLimits()
-    # converged: the oracle's stop fired, or (for models whose I / z stop limits are below
-    # one FP32 ulp and so rarely fire) its last step was below 1e-4 px, 0.1 nm and 1e-5 I
-    conv = it_ref <= iters
-    if kind in ("astig", "spline"):
-        last = p_ref - tr_ref[:, iters - 1]
-        conv |= (it_ref > iters) & (np.abs(last[:, 0]) < 1e-4) & (np.abs(last[:, 1]) < 1e-4) & \
-            (np.abs(last[:, 2]) < 0.1) & (np.abs(last[:, 3]) < 1e-5 * np.abs(p_ref[:, 3]))
-    conv &= ~_on_bounds(p_ref, lo, hi) & np.all(np.isfinite(p_ref), axis=1)
-    # A spot is a parity witness only where the reference's own answer is reproducible:
-    # the oracle's FP64 fit (same algorithm and iteration budget) agrees with its FP32 fit
-    # to the parity tolerance.  This leaves out the few low-SNR spots whose FP32 result is
-    # decided by rounding (for seed 3: 2 of the 4000 free-sigma spots, whose FP64 fit ends
-    # on the clamp bounds and whose FP32 fit moves under 1e-6 perturbations of the samples).
-    p64 = om.estimate(smp, iterations=iters, lambda_=lam, dtype=np.float64)[0]
-    conv &= _compare(kind, p64, p_ref)
-    assert conv.mean() > 0.5, f"only {conv.mean():.3f} of spots converged"
-    gpu_bound = _on_bounds(p_gpu, lo, hi) | ~np.all(np.isfinite(p_gpu), axis=1)
-    ok = _compare(kind, p_gpu[conv], p_ref[conv])
-    bad = np.where(~ok)[0]
-    # A low-SNR spot's LM path is chaotic: an FP32 rounding difference in one step can send
-    # it to a bound or to another stationary point.  A mismatch is allowed only where the
-    # GPU fit went to a bound or its own stop fired there, and for at most 1 spot in 1000.
-    other = (gpu_bound | (it_gpu <= iters))[conv][bad]
-    assert other.all(), f"converged spots differ, e.g. gpu={p_gpu[conv][bad[:3]]} ref={p_ref[conv][bad[:3]]}"
-    assert len(bad) <= max(1, 1e-3 * conv.sum()), \
-        f"{len(bad)} of {conv.sum()} converged spots differ, e.g. gpu={p_gpu[conv][bad[:3]]} ref={p_ref[conv][bad[:3]]}"
-    # iteration counts agree for most spots (the FP64 solve can move a stop by one
-    # iteration).  Not for astig / spline: their z and I stop limits are below one FP32
-    # ulp of typical values, so when the stop fires is decided by rounding noise.
-    if kind in ("fixed", "sigma"):
-        assert np.mean(it_gpu == it_ref) > 0.8
+    P.check_fit_parity(kind, om, smp, p_gpu, it_gpu, iters, lam, est.ParamLimits(), tag=f"{kind} B={B}")
 
 
 @pytest.mark.parametrize("kind", KINDS)

@@ -2,13 +2,15 @@
 
 A queued spot is fitted by the same fused fit + CRLB kernels as
 Estimator.estimate_device, so its estimate, iterations, CRLB and chi^2 must equal
-the direct call's bit for bit, whatever batch and stream it landed in.
+the direct call's bit for bit, whatever batch and stream it landed in; the queue's
+results are also checked against the oracle with the parity rules of tests/_parity.py.
 """
 import numpy as np
 import pytest
 
 import photonpy_amd as pa
 from tests import _models as MD
+from tests import _parity as P
 
 pytestmark = pytest.mark.gpu
 
@@ -62,6 +64,14 @@ def test_queue_matches_direct_fit(ctx, kind):
     assert np.array_equal(r.crlb, ref["crlb"][order], equal_nan=True)
     assert np.array_equal(r.chisq, ref["chisq"][order], equal_nan=True)
     assert r.diagnostics.shape == (n, 0)
+    # and the queue's results, back in input order, match the oracle (the reference's
+    # queue fits each batch with the estimator's Estimate, EstimationQueue.cpp:277-299)
+    p_in = np.empty_like(r.estim)
+    it_in = np.empty_like(r.iterations)
+    p_in[order] = r.estim
+    it_in[order] = r.iterations
+    _, om, _ = MD.make(kind, ctx)
+    P.check_fit_parity(kind, om, smp, p_in, it_in, 50, 1e-18, est.ParamLimits(), tag=f"queue {kind}")
 
 
 def test_queue_initial_values_and_samples(ctx):
