@@ -267,6 +267,18 @@ int Estimator_GetPhaseTimes(Estimator* e, float* fitMs, float* crlbMs);
 int Estimator_EstimateDevice(Estimator* e, int n, const float* d_sample, const float* d_constants,
                              const float* d_initial, float* d_params, int* d_iterations, float* d_trace, int traceLen,
                              float* d_crlb, float* d_chisq, void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc): Estimator_EstimateDevice with the
+ * same arguments and results, allocating nothing.  Its scratch -- the fit kernel's work
+ * counter -- comes from a fixed array of 64 counters (256 bytes) in the library's device
+ * memory, one per call in turn, reset by a kernel on `stream`; it does not grow with n (n is
+ * bounded as for Estimator_EstimateDevice).  Calls that overlap on one GPU with 64 or more
+ * others in flight share a counter and race.  A spline estimator's tables must already be on
+ * the device (any other fit or evaluation call on it uploads them); otherwise the call fails
+ * with an error. */
+int Estimator_EstimateDeviceNoAlloc(Estimator* e, int n, const float* d_sample, const float* d_constants,
+                                    const float* d_initial, float* d_params, int* d_iterations, float* d_trace,
+                                    int traceLen, float* d_crlb, float* d_chisq, void* stream);
 /* d_constants [n, NumConstants]: the per-spot constants of the models that have them
  * (XYIBgConstSigma, XYITiltedBg: sigma x, y; spline mode 2: the background image), else NULL. */
 int Estimator_ExpectedValueDevice(Estimator* e, int n, const float* d_params, const float* d_constants, float* d_ev,

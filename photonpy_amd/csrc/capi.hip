@@ -4,6 +4,8 @@
 // caller's stream.  Everything computes on the GPU: there is no CPU path.
 #include "capi_internal.h"
 
+#include <atomic>
+
 namespace ppa {
 hipError_t launch_spline_coefs(const int* shape, const float* d_data, float* d_out, int border, hipStream_t st);
 hipError_t launch_spline_evaluate(const int* shape, const float* d_coefs, const float* shift, int border,
@@ -215,6 +217,34 @@ void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constant
                                  std::to_string(G));
     hip_check(err, "fit kernel launch");
 }
+
+// ---------------------------------------------------------------- allocation-free fit
+// Work counters of Estimator_EstimateDeviceNoAlloc: a fixed array in device memory
+// (each device holds its own copy until the process ends), so that the call itself
+// allocates nothing.  Each call takes the next of the kNoAllocSlots counters and
+// resets it with a kernel on its stream before the fit reads it.
+constexpr int kNoAllocSlots = 64;
+static __device__ __align__(256) int scratch[kNoAllocSlots];
+
+// Device address of `scratch` on the current device, resolved once per device with
+// that device current.  Host code passes this address, never the array's name.
+static int* scratch_on_current_device() {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mtx;
+    static int* addr[kMaxDevices] = {};
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("photonpy_amd: device index out of range");
+    std::lock_guard<std::mutex> l(mtx);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        addr[dev] = static_cast<int*>(p);
+    }
+    return addr[dev];
+}
+
+static std::atomic<unsigned> g_noAllocSlot{0};
 
 // workspace layout helper: consecutive 256-byte aligned slices
 struct Slicer {
@@ -814,6 +844,26 @@ int Estimator_EstimateDevice(Estimator* e, int n, const float* d_sample, const f
             std::lock_guard<std::mutex> l(e->mtx);
             run_fit(e, n, d_sample, d_constants, d_initial, d_params, d_iterations, d_trace, traceLen, d_crlb, d_chisq,
                     as_stream(stream));
+            return 0;
+        },
+        2);
+}
+
+// Estimator_EstimateDevice without any allocation: the work counter comes from the
+// library's fixed device array (see scratch_on_current_device).
+int Estimator_EstimateDeviceNoAlloc(Estimator* e, int n, const float* d_sample, const float* d_constants,
+                                    const float* d_initial, float* d_params, int* d_iterations, float* d_trace,
+                                    int traceLen, float* d_crlb, float* d_chisq, void* stream) {
+    e = model_of(e);
+    if (!e) return 1;
+    if (n <= 0) return 0;
+    return guarded(
+        [&]() -> int {
+            std::lock_guard<std::mutex> l(e->mtx);
+            int* counters = scratch_on_current_device();
+            int* counter = counters + (g_noAllocSlot.fetch_add(1) % kNoAllocSlots);
+            run_fit(e, n, d_sample, d_constants, d_initial, d_params, d_iterations, d_trace, traceLen, d_crlb, d_chisq,
+                    as_stream(stream), false, 0, counter);
             return 0;
         },
         2);

@@ -169,6 +169,7 @@ def _bind(lib):
         "MixedBatch_Run": ([V] + [ct.POINTER(V)] * 6 + [I, V], ct.c_int),
         "MixedBatch_RunPlain": ([V] + [ct.POINTER(V)] * 6 + [V], ct.c_int),
         "Estimator_GetShared": ([V, V], ct.c_int),
+        "Estimator_EstimateDeviceNoAlloc": ([V, I, V, V, V, V, V, V, I, V, V, V], ct.c_int),
         "CudaGetNumDevices": ([], ct.c_int32),
         "CudaSetDevice": ([I], ct.c_bool),
         "CudaGetDeviceInfo": ([I, ct.POINTER(ct.c_int32), ct.c_char_p, I], ct.c_bool),
