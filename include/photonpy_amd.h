@@ -296,7 +296,8 @@ int Estimator_ChiSquareAndCRLBDevice(Estimator* e, int n, const float* d_params,
  *   spline the coefficient table [Nz,Ny,Nx,64]; free sigma: none.
  * d_gout [n]: upstream gradient of each spot's NLL (NULL = 1).  Outputs (each nullable):
  * d_nll [n], d_grad_params [n,K] (true derivatives), d_grad_shared (accumulated with +=).
- * roisize <= 32. */
+ * The spline coefficient gradient sorts the spots by spline window in a scratch buffer the
+ * estimator keeps (grown to the largest n seen). */
 int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const float* d_sample, const float* d_shared,
                             const float* d_gout, float* d_nll, float* d_grad_params, float* d_grad_shared,
                             void* stream);

@@ -18,16 +18,14 @@
 // The Gaussian adjoint evaluates the model in FP64 (erf/exp of double); the
 // spline adjoint in FP32 (its value has no erf-difference cancellation).
 //
-// Mapping: one wavefront per spot, lanes stride over pixels.  Scalar shared
-// gradients are reduced per block in LDS and added to HBM once per block; the
-// spline coefficient gradient is scattered pixel by pixel with the 64 lanes
-// covering the 64 coefficients of the pixel's cell, so every atomic
-// instruction updates one contiguous 256-byte cell.
+// Mapping: one wavefront per spot, lanes stride over pixels (any roisize).  Scalar
+// shared gradients are reduced per block in LDS and added to HBM once per block.
+// The spline coefficient gradient of roisize <= 28 is formed by window rounds
+// (spline_adjoint_tiled_kernel, below); larger ROIs scatter it pixel by pixel.
 #include "dispatch.h"
 
 namespace ppa {
 
-constexpr int kMaxAdjSlots = 16;  // pixels per lane: R <= 32
 
 PP_DEV float nll_weight(float mu, float x, float& nll) {
     const float m = fmaxf(mu, 1e-8f), xv = fmaxf(x, 1e-8f);
@@ -181,6 +179,41 @@ __global__ void __launch_bounds__(256) gauss_adjoint_kernel(GaussModel<V> M, Adj
     }
 }
 
+// Value and x/y/z derivatives of spline cell c at the frame's sub-voxel position, one
+// pair of z-powers at a time (32 coefficients in registers instead of 64): the tiled
+// kernel's 1024-thread blocks leave it 128 VGPRs.  The same polynomial as SplineModel::cell,
+// summed over z by Horner-free running powers.
+PP_DEV void spline_cell_lean(const float* c, const SplineModel::Frame& F, float& f, float& fx, float& fy,
+                             float& fz) {
+    const float4* c4 = reinterpret_cast<const float4*>(c);
+    const float z = F.zp[1];
+    float zm = 1.0f, dzm = 0.0f;  // z^m and m z^(m-1)
+    f = fx = fy = fz = 0.0f;
+#pragma unroll 2
+    for (int m = 0; m < 4; m++) {
+        float a = 0.0f, ax = 0.0f, ay = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const float4 q = c4[m * 4 + n];
+            const float px = q.x + q.y * F.x1 + q.z * F.x2 + q.w * F.x3;
+            const float dpx = q.y + q.z * F.dx2 + q.w * F.dx3;
+            a += px * F.yp[n];
+            ax += dpx * F.yp[n];
+            ay += px * F.dyp[n];
+        }
+        f += a * zm;
+        fx += ax * zm;
+        fy += ay * zm;
+        fz += a * dzm;
+        dzm = dzm * z + zm;
+        zm *= z;
+    }
+}
+
+// One wavefront per spot (any roisize): the NLL, the per-spot gradient and, unless the
+// tiled form below takes it, the coefficient gradient, scattered pixel by pixel with
+// the 64 lanes covering the 64 coefficients of the pixel's cell (each atomic
+// instruction updates one contiguous 256-byte cell).
 __global__ void __launch_bounds__(256) spline_adjoint_kernel(SplineModel M, AdjArgs a) {
     constexpr int K = 5;
     const int lane = threadIdx.x & (kWave - 1);
@@ -194,30 +227,42 @@ __global__ void __launch_bounds__(256) spline_adjoint_kernel(SplineModel M, AdjA
     SplineModel::Frame F;
     M.frame(t, R, F);
     const float go = a.gout ? a.gout[spot] : 1.0f;
+    // lane q owns coefficient q = 16m + 4n + o of every cell: basis z^m y^n x^o
+    const int m = lane >> 4, nn = (lane >> 2) & 3, o = lane & 3;
+    const float zb = m == 0 ? 1.0f : m == 1 ? F.zp[1] : m == 2 ? F.zp[2] : F.zp[3];
+    const float yb = nn == 0 ? 1.0f : nn == 1 ? F.yp[1] : nn == 2 ? F.yp[2] : F.yp[3];
+    const float xb = o == 0 ? 1.0f : o == 1 ? F.x1 : o == 2 ? F.x2 : F.x3;
+    const float basis = zb * yb * xb;
     float acc[K + 1];
 #pragma unroll
     for (int i = 0; i < K + 1; i++) acc[i] = 0.0f;
-    int cellOf[kMaxAdjSlots];
-    float wI[kMaxAdjSlots];
-#pragma unroll
-    for (int s = 0; s < kMaxAdjSlots; s++) {
-        const int p = lane + s * kWave;
-        cellOf[s] = 0;
-        wI[s] = 0.0f;
+    // every lane runs every trip (the scatter is a wavefront operation)
+    for (int p0 = 0; p0 < S; p0 += kWave) {
+        const int p = p0 + lane;
+        float W = 0.0f;
+        int cell = 0;
         if (p < S) {
-            const int y = p / R, x = p - (p / R) * R;
-            const float* rp = M.row_ptr(F, y);
-            const int i = M.col_index(F, x);
-            const float* c = rp + 64 * i;
+            const int y = p / R, x = p - y * R;
+            const float* c = M.row_ptr(F, y) + 64 * M.col_index(F, x);
             float f, fx, fy, fz, mu, J[K];
-            SplineModel::cell(c, F, f, fx, fy, fz);
+            SplineModel::cell(c, F, f, fx, fy, fz);  // all 64 coefficient loads in flight
             SplineModel::finish(F, t, f, fx, fy, fz, x, y, R, mu, J);
             if (f < 0.0f) J[0] = J[1] = J[2] = 0.0f;  // mu does not depend on position where clamped
             const float w = nll_weight(mu, a.sample[(size_t)spot * S + p], acc[K]);
 #pragma unroll
             for (int k = 0; k < K; k++) acc[k] += w * J[k];
-            cellOf[s] = (int)((c - M.coefs) >> 6);
-            wI[s] = (f > 0.0f) ? go * w * t[3] : 0.0f;
+            cell = (int)((c - M.coefs) >> 6);
+            W = (f > 0.0f) ? go * w * t[3] : 0.0f;
+        }
+        if (a.gshared) {
+            unsigned long long any = __ballot(W != 0.0f);
+            while (any) {
+                const int src = __ffsll((long long)any) - 1;
+                any &= any - 1;
+                const float g = __shfl(W, src, kWave);
+                const int cs = __shfl(cell, src, kWave);
+                atomicAdd(&a.gshared[(size_t)cs * 64 + lane], g * basis);
+            }
         }
     }
     group_sum_n<kWave>(acc);
@@ -226,23 +271,6 @@ __global__ void __launch_bounds__(256) spline_adjoint_kernel(SplineModel M, AdjA
         if (a.gparams) {
 #pragma unroll
             for (int k = 0; k < K; k++) a.gparams[(size_t)spot * K + k] = go * acc[k];
-        }
-    }
-    if (!a.gshared) return;
-    // lane q owns coefficient q = 16m + 4n + o of every cell: basis z^m y^n x^o
-    const int m = lane >> 4, nn = (lane >> 2) & 3, o = lane & 3;
-    const float xo[4] = {1.0f, F.x1, F.x2, F.x3};
-    const float basis = F.zp[m] * F.yp[nn] * xo[o];
-#pragma unroll
-    for (int s = 0; s < kMaxAdjSlots; s++) {
-        if (s * kWave >= S) break;
-        for (int src = 0; src < kWave; src++) {
-            if (s * kWave + src >= S) break;
-            const float g = __shfl(wI[s], src, kWave);
-            if (g != 0.0f) {
-                const int cell = __shfl(cellOf[s], src, kWave);
-                atomicAdd(&a.gshared[(size_t)cell * 64 + lane], g * basis);
-            }
         }
     }
 }
@@ -268,37 +296,6 @@ constexpr int kMaxTiledR = 28;
 constexpr int kWaves = 16;                   // wavefronts per block
 constexpr int kTileSpots = 256;              // spots per tile (a multiple of every round)
 constexpr int kTiledBlock = kWaves * kWave;
-
-// Value and x/y/z derivatives of spline cell c at the frame's sub-voxel position, one
-// z-power at a time (16 coefficients in registers instead of 64): the tiled kernel's
-// 1024-thread blocks leave it 128 VGPRs.  The same polynomial as SplineModel::cell,
-// summed over z by Horner-free running powers.
-PP_DEV void spline_cell_lean(const float* c, const SplineModel::Frame& F, float& f, float& fx, float& fy,
-                             float& fz) {
-    const float4* c4 = reinterpret_cast<const float4*>(c);
-    const float z = F.zp[1];
-    float zm = 1.0f, dzm = 0.0f;  // z^m and m z^(m-1)
-    f = fx = fy = fz = 0.0f;
-#pragma unroll 1
-    for (int m = 0; m < 4; m++) {
-        float a = 0.0f, ax = 0.0f, ay = 0.0f;
-#pragma unroll
-        for (int n = 0; n < 4; n++) {
-            const float4 q = c4[m * 4 + n];
-            const float px = q.x + q.y * F.x1 + q.z * F.x2 + q.w * F.x3;
-            const float dpx = q.y + q.z * F.dx2 + q.w * F.dx3;
-            a += px * F.yp[n];
-            ax += dpx * F.yp[n];
-            ay += px * F.dyp[n];
-        }
-        f += a * zm;
-        fx += ax * zm;
-        fy += ay * zm;
-        fz += a * dzm;
-        dzm = dzm * z + zm;
-        zm *= z;
-    }
-}
 
 PP_DEV int spline_window_key(const SplineModel& M, const float (&t)[5], int R, SplineModel::Frame& F, int& i0,
                              int& j0, int& k) {
@@ -545,7 +542,6 @@ static hipError_t launch_spline_tiled(const SplineModel& M, const AdjArgs& a, vo
 }
 
 hipError_t launch_adjoint_any(const ModelDesc& d, const AdjArgs& a, void* ws, hipStream_t st) {
-    if (a.R > 32) return hipErrorInvalidValue;
     if (d.kind == kKindSpline && a.gshared && ws && spline_adjoint_tiled(a.R)) {
         SplineModel M = d.sm;
         return launch_spline_tiled(M, a, ws, st);

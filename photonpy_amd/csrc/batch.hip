@@ -105,11 +105,6 @@ MixedBatch* MixedBatch_Create(int nbins, Estimator* const* ests, const int* coun
                     if (e->NC() > 0)
                         throw std::runtime_error("photonpy_amd: models with per-spot constants are not batched");
                     if (counts[b] < 0) throw std::runtime_error("photonpy_amd: negative spot count");
-                    const int R = e->roisize;
-                    if (R < kMinFitR || R > kMaxFitR)
-                        throw std::runtime_error("photonpy_amd: fit kernels are compiled for roisize " +
-                                                 std::to_string(kMinFitR) + ".." + std::to_string(kMaxFitR) +
-                                                 ", got " + std::to_string(R));
                     mb->est.push_back(e);
                     mb->counts.push_back(counts[b]);
                     cost.push_back({bin_cost(e, counts[b]), b});

@@ -180,9 +180,6 @@ void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constant
         throw std::runtime_error("photonpy_amd: at most " + std::to_string(kMaxSpots) + " spots per fit call, got " +
                                  std::to_string(n));
     const int R = e->roisize;
-    if (R < kMinFitR || R > kMaxFitR)
-        throw std::runtime_error("photonpy_amd: fit kernels are compiled for roisize " + std::to_string(kMinFitR) +
-                                 ".." + std::to_string(kMaxFitR) + ", got " + std::to_string(R));
     FitArgs a{};
     a.n = n;
     a.sample = d_sample;
@@ -209,6 +206,11 @@ void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constant
             e->phaseEvents.push_back(ev);
         }
         events = e->phaseEvents.data() + e->phaseEvents.size() - 3;
+    }
+    if (R < kMinFitR || R > kMaxFitR) {
+        // no per-size kernel: the any-size kernel (one wavefront per spot)
+        hip_check(launch_fit_large_any(e->desc, R, a, events, st), "fit kernel launch");
+        return;
     }
     const int G = fit_group(e, groupHint);
     hipError_t err = launch_fit_any(e->desc, R, G, a, events, st);
@@ -986,7 +988,6 @@ int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const fl
     return guarded(
         [&]() -> int {
             std::lock_guard<std::mutex> l(e->mtx);
-            if (e->roisize > 32) throw std::runtime_error("photonpy_amd: the adjoint supports roisize <= 32");
             if (e->desc.kind > kKindSpline)
                 throw std::runtime_error("photonpy_amd: the adjoint covers the fixed-sigma, free-sigma, astigmatic and "
                                          "spline models");

@@ -109,6 +109,8 @@ hipError_t launch_fit_splineimage(const ModelDesc& d, int R, int G, const FitArg
 hipError_t launch_fit_splineimagegrid(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events,
                                       hipStream_t st);
 
+// Fit (+ CRLB / chi^2) of a roisize with no compiled fit kernel (one wavefront per spot).
+hipError_t launch_fit_large_any(const ModelDesc& d, int R, const FitArgs& a, hipEvent_t* events, hipStream_t st);
 hipError_t launch_ev_any(const ModelDesc& d, int R, int n, const float* params, const float* constants, float* ev,
                          float* deriv, hipStream_t st);
 hipError_t launch_crlb_any(const ModelDesc& d, int R, int n, const float* params, const float* constants,

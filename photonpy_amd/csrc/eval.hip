@@ -1,5 +1,7 @@
 // Launchers of the any-size kernels: expected value / derivatives, stand-alone
 // CRLB + chi^2, and the spline coefficient solve.
+#include <algorithm>
+
 #include "dispatch.h"
 #include "fit_sizes.h"
 
@@ -61,6 +63,29 @@ hipError_t launch_ev_any(const ModelDesc& d, int R, int n, const float* params, 
 hipError_t launch_crlb_any(const ModelDesc& d, int R, int n, const float* params, const float* constants,
                            const float* sample, float* crlb, float* chisq, hipStream_t st) {
     return with_model(d, [&](const auto& m) { return crlb_t(m, R, n, params, constants, sample, crlb, chisq, st); });
+}
+
+template <class M>
+static hipError_t fit_large_t(const M& m, int R, const FitArgs& a, hipEvent_t* events, hipStream_t st) {
+    static int resident = 0;  // per model type; one device model (gfx950)
+    if (resident == 0) resident = resident_blocks(reinterpret_cast<const void*>(fit_large_kernel<M>), 256, 0);
+    if (resident <= 0) return hipErrorNoDevice;
+    const long need = ((long)a.n * kWave + 255) / 256;
+    const unsigned blocks = (unsigned)std::max(1L, std::min(need, (long)resident));
+    if (events) (void)hipEventRecord(events[0], st);
+    hipLaunchKernelGGL(fit_large_kernel<M>, dim3(blocks), dim3(256), 0, st, m, a, R);
+    if (events) (void)hipEventRecord(events[1], st);
+    if (a.crlb || a.chisq) {
+        const unsigned cb = (unsigned)(((size_t)a.n * kWave + 255) / 256);
+        hipLaunchKernelGGL(crlb_kernel<M>, dim3(cb), dim3(256), 0, st, m, a.n, R, a.params, a.constants, a.sample,
+                           a.crlb, a.chisq);
+    }
+    if (events) (void)hipEventRecord(events[2], st);
+    return hipGetLastError();
+}
+
+hipError_t launch_fit_large_any(const ModelDesc& d, int R, const FitArgs& a, hipEvent_t* events, hipStream_t st) {
+    return with_model(d, [&](const auto& m) { return fit_large_t(m, R, a, events, st); });
 }
 
 bool fit_compiled(const ModelDesc& d, int R, int G) {

@@ -1,4 +1,5 @@
-// Expected value / derivatives and stand-alone CRLB + chi^2 kernels (any ROI size).
+// Expected value / derivatives, stand-alone CRLB + chi^2, and the fit of ROI sizes
+// outside the compiled fit kernels (any ROI size).
 //
 // ev_kernel: one lane per (spot, pixel); writes ev[n,R,R] and deriv[n,K,R,R]
 //   (layout of ComputeDerivatives, Estimation.h:169-181).
@@ -7,7 +8,7 @@
 // Models with a background image (kPixelBg) read it from constants [n, R*R].
 #pragma once
 
-#include "models.h"
+#include "fit_kernel.h"
 
 namespace ppa {
 
@@ -38,6 +39,108 @@ __global__ void __launch_bounds__(256) ev_kernel(M model, int n, int R, const fl
     if (deriv) {
 #pragma unroll
         for (int k = 0; k < K; k++) deriv[((size_t)spot * K + k) * S + p] = J[k];
+    }
+}
+
+// Levenberg-Marquardt fit of ROIs whose size has no compiled fit kernel (roisize < 3 or
+// > 32; the reference's models take any roisize, GaussianPSFModels.h:76-97).  One
+// wavefront per spot; each iteration re-reads the spot's samples (they stay in L2)
+// and evaluates every pixel on its own (eval_pixel), so it trades the per-size
+// kernels' register-resident samples and shared edge terms for generality.  The
+// iteration, clamp, stop and trace semantics are those of fit_spots.
+template <class M>
+__global__ void __launch_bounds__(256) fit_large_kernel(M model, FitArgs a, int R) {
+    constexpr int K = M::K, KT = M::KT, NA = K * (K + 1) / 2;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int S = R * R;
+    const size_t waves = (size_t)gridDim.x * blockDim.x / kWave;
+    for (size_t sp = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; sp < (size_t)a.n; sp += waves) {
+        const float* smp = a.sample + sp * S;
+        const float* img = M::kPixelBg ? a.constants + sp * S : nullptr;
+        float th[KT];
+#pragma unroll
+        for (int c = 0; c < M::NC; c++) th[K + c] = a.constants[sp * M::NC + c];
+        if (a.initial) {
+#pragma unroll
+            for (int k = 0; k < K; k++) th[k] = a.initial[sp * K + k];
+        } else {
+            float sx = 0.0f, sy = 0.0f, sum = 0.0f;
+            for (int p = lane; p < S; p += kWave) {
+                const float v = smp[p];
+                const int y = p / R, x = p - y * R;
+                sx += (float)x * v;
+                sy += (float)y * v;
+                sum += v;
+            }
+            sx = group_sum<kWave>(sx);
+            sy = group_sum<kWave>(sy);
+            sum = group_sum<kWave>(sum);
+            model.initial(sx / sum, sy / sum, sum, th);
+        }
+        int iter = 0;
+        for (;;) {
+            if (iter >= a.maxIter) {
+                iter = a.maxIter + 1;  // loop exhausted without a stop: the reference reports max + 1
+                break;
+            }
+            if (a.trace && iter < a.traceLen && lane == 0) {
+#pragma unroll
+                for (int k = 0; k < K; k++) a.trace[(sp * a.traceLen + iter) * K + k] = th[k];
+            }
+            float acc[NA + K];
+#pragma unroll
+            for (int i = 0; i < NA + K; i++) acc[i] = 0.0f;
+            for (int p = lane; p < S; p += kWave) {
+                const int y = p / R, x = p - y * R;
+                float mu, J[K];
+                model.eval_pixel(th, x, y, R, mu, J);
+                if constexpr (M::kPixelBg) {
+                    mu += th[M::iBg] * img[p];
+                    J[M::iBg] = img[p];
+                }
+                const float v = smp[p];
+                const float xv = v < 1e-6f ? 1e-6f : v;  // keeps a NaN (Estimation.h:114)
+                const float inv = rcpf(fmaxf(mu, 1e-6f));
+                const float t = xv * inv;
+                const float xf2 = t * inv;
+                const float bf = t - 1.0f;
+                int q = 0;
+#pragma unroll
+                for (int i = 0; i < K; i++) {
+                    const float w = J[i] * xf2;
+#pragma unroll
+                    for (int j = i; j < K; j++) acc[q++] += w * J[j];
+                }
+#pragma unroll
+                for (int i = 0; i < K; i++) acc[NA + i] += bf * J[i];
+            }
+            group_sum_n<kWave>(acc);
+            double step[K];
+            bool stop;
+            if (!lm_step_f64<K>(acc, acc + NA, a.lambda, step)) {
+                stop = true;  // zero pivot: the reference breaks before updating
+            } else {
+                float last[K];
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    last[k] = th[k];
+                    th[k] += (float)step[k];
+                }
+                model.check_limits(th, R);
+                bool small = true;
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                    if (fabsf(last[k] - th[k]) > M::stop_limit(k)) small = false;
+                stop = (iter > 0 && small);
+            }
+            iter++;
+            if (stop) break;
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < K; k++) a.params[sp * K + k] = th[k];
+            if (a.iterations) a.iterations[sp] = iter;
+        }
     }
 }
 

@@ -94,9 +94,3 @@ def test_stream_count_does_not_change_results(ctx):
             results.append(outs)
     for r in results[1:]:
         _same(results[0], r, "streams")
-
-
-def test_unsupported_size_rejected(ctx):
-    est = pa.GaussianPSFMethods(ctx).CreatePSF_XYIBg(40, 1.5, True)
-    with pytest.raises(pa.NativeError, match="roisize"):
-        MixedBatch([est], [10], tune=False)

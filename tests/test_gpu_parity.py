@@ -145,11 +145,12 @@ def test_degenerate_rois(ctx, kind):
     assert np.all((p[fin, 1] >= 1.5) & (p[fin, 1] <= R - 2.5))
 
 
-def test_unsupported_roisize_raises(ctx):
-    est = pa.GaussianPSFMethods(ctx).CreatePSF_XYIBg(40, 1.5, True)
+def test_invalid_roisize_raises(ctx):
+    # any roisize >= 1 is fitted (sizes without a per-size kernel run on the any-size
+    # kernel, tests/test_gpu_large_roi.py); a non-positive one is refused at creation
     with pytest.raises(pa.NativeError, match="roisize"):
-        est.Estimate(np.ones((2, 40, 40), np.float32))
-    # expected values work at any size
+        pa.GaussianPSFMethods(ctx).CreatePSF_XYIBg(0, 1.5, True)
+    est = pa.GaussianPSFMethods(ctx).CreatePSF_XYIBg(40, 1.5, True)
     assert est.ExpectedValue(np.array([[20.0, 20.0, 100.0, 1.0]])).shape == (1, 40, 40)
 
 
