@@ -143,12 +143,24 @@ def synthetic_astig():
     return [1.3, 200.0, 400.0, 0.1 / 400], [1.3, -200.0, 400.0, 0.1 / 400], 0.05 / 400 ** 2
 
 
-def cpu_baseline(cfg, smp_host, est, run_s=1.0, repeats=5):
+# ROIs of each workload kept on the host for its CPU baseline
+CPU_MAX_ROIS = 131_072
+
+
+def cpu_job(cfg, smp, est):
+    """The CPU baseline of one workload, prepared now (oracle model, host copy of the
+    first ROIs) and run later by the caller, after every GPU measurement, so that the
+    host load of the oracle's threads cannot slow a GPU leg that follows it."""
+    om = oracle_model(cfg, est)
+    host_rois = smp[:CPU_MAX_ROIS].cpu().numpy()
+    return lambda: cpu_baseline(cfg, host_rois, om)
+
+
+def cpu_baseline(cfg, smp_host, om, run_s=0.3, repeats=5):
     """The oracle (FP32 restatement of the reference estimator) on the host: Estimate + CRLB
     + chi^2 over the first m ROIs of the same batch (the batch repeated when it is smaller),
-    m sized so that one run takes about run_s seconds; one warm-up run, then the median of
-    `repeats` timed runs."""
-    om = oracle_model(cfg, est)
+    m sized from a warm run so that one run takes about run_s seconds; one warm-up run,
+    then the median of `repeats` timed runs."""
     host = host_info()
     threads = max(1, min(CPU_SHARE, host["usable_cpus"]))
     total = len(smp_host)
@@ -164,23 +176,23 @@ def cpu_baseline(cfg, smp_host, est, run_s=1.0, repeats=5):
         om.crlb(p, s, nthreads=threads)
         return time.perf_counter() - t
 
-    m = min(total, 512)
+    m = min(total, 1024)
+    run(rois(m))  # first run: thread start-up, page faults
     dt = run(rois(m))
-    for _ in range(4):  # grow the sample until a run takes at least half of run_s
-        if dt >= 0.5 * run_s:
+    for _ in range(2):  # per-call overhead makes small runs slow per ROI: rescale once more
+        m = max(threads, int(m * run_s / max(dt, 1e-6)))
+        s = rois(m)
+        dt = run(s)  # also the warm-up of the timed runs
+        if dt >= 0.6 * run_s:
             break
-        m = int(m * min(64.0, 1.1 * run_s / max(dt, 1e-4)))
-        dt = run(rois(m))
-    s = rois(m)
-    run(s)  # warm-up
     times = sorted(run(s) for _ in range(repeats))
     med = times[len(times) // 2]
     print(f"bench: cpu baseline {cfg['kind']} R={cfg['R']}: {m} ROIs, median {med:.3f} s on {threads} threads",
           file=sys.stderr, flush=True)
-    what = f"first {m} ROIs of the same batch" if m <= total else f"the {total} ROIs of the batch, {m} in all"
+    what = f"first {m} ROIs of the same batch" if m <= total else f"the first {total} ROIs of the batch, {m} in all"
     return {"value": m / med, "unit": "localizations/s", "cores": threads, "kind": "port",
             "sample": f"{what}; oracle Estimate + CRLB + chi^2 on {threads} threads (one GPU's {CPU_SHARE}-core "
-                      f"share of the host); median of {repeats} runs after one warm-up",
+                      f"share of the host); median of {repeats} runs after one warm-up, run after all GPU legs",
             "runs_s": [round(t, 4) for t in times], "host": host}
 
 
@@ -207,7 +219,8 @@ def line(metric_value, unit, steps, warmup, ms, workload, extra_config, roofline
 
 
 def bench_fit(name, steps, warmup, n=0, group=0, refill=0, cpu=True, e2e=True):
-    """One single-model workload: `steps` timed fused launches after `warmup` untimed ones."""
+    """One single-model workload: `steps` timed fused launches after `warmup` untimed ones.
+    Returns (bench line, CPU-baseline job or None); finish_cpu runs the job into the line."""
     import torch
     import photonpy_amd as pa
     from photonpy_amd import autotune, synthetic
@@ -257,7 +270,7 @@ def bench_fit(name, steps, warmup, n=0, group=0, refill=0, cpu=True, e2e=True):
         achieved = fit_flops / (fit_ms * 1e-3) / 1e12
         traffic = hbm_traffic(name) if n == cfg["n"] else None
         e2e_rate = host_api_rate(est, smp[:min(n, 200_000)].cpu().numpy()) if e2e else None
-        cpu_line = cpu_baseline(cfg, smp.cpu().numpy(), est) if cpu else None
+        job = cpu_job(cfg, smp, est) if cpu else None
         group = est.GetGroupSize()
         form = est.spline_form()
         del smp, out
@@ -278,12 +291,19 @@ def bench_fit(name, steps, warmup, n=0, group=0, refill=0, cpu=True, e2e=True):
         "pcie_inclusive_loc_per_s": e2e_rate,
         "finite_fraction": float(np.mean(np.all(np.isfinite(p), axis=1))),
     }
-    if cpu_line:
-        detail["gpu_over_cpu"] = (n / (ms_per_step * 1e-3)) / cpu_line["value"]
     extra = {"roi": cfg["R"], "spots": n, "iterations": cfg["iters"], "lanes_per_spot": group,
              **({"spline_form": form} if form else {})}
     return line(n / (ms_per_step * 1e-3), "localizations/s", steps, warmup, ms_per_step, cfg["workload"], extra,
-                roof, cpu_line, detail)
+                roof, None, detail), job
+
+
+def finish_cpu(result, job):
+    """Runs a workload's CPU-baseline job into its bench line."""
+    if job is None:
+        return
+    cpu_line = job()
+    result["cpu_baseline"] = cpu_line
+    result["detail"]["gpu_over_cpu"] = result["value"] / cpu_line["value"]
 
 
 def bench_mixed(steps, warmup, n=0):
@@ -322,15 +342,25 @@ def bench_mixed(steps, warmup, n=0):
 
             plain_ms, plain_gpu = timed(lambda: mb.run_plain(samples, ref))
             batch_ms, batch_gpu = timed(lambda: mb.run(samples, outs, use_graph=True))
-            # per-bin kernel times: each bin's plain fit + CRLB launch on its own
-            bin_ms = []
-            for e, smp in zip(ests, samples):
+            # Per-bin kernel times: each bin's plain fit + CRLB launch on its own (t1), and
+            # the same launch over the bin's ROIs twice (t2).  t2 - t1 is the bin's marginal
+            # work at full occupancy; 2 t1 - t2 its fixed cost (launch, ramp-up, tail), the
+            # most any schedule of the bins can win back.
+            def kernel_ms(e, smp):
                 e.set_phase_timing(True)
-                e.estimate_device(smp, crlb=True)
+                for _ in range(2):
+                    e.estimate_device(smp, crlb=True)
                 torch.cuda.synchronize()
                 _, f_ms, c_ms = e.phase_times()
                 e.set_phase_timing(False)
-                bin_ms.append(f_ms + c_ms)
+                return f_ms + c_ms
+
+            bin_ms, bin_ms2 = [], []
+            for e, smp in zip(ests, samples):
+                bin_ms.append(kernel_ms(e, smp))
+                doubled = torch.cat([smp, smp])
+                bin_ms2.append(kernel_ms(e, doubled))
+                del doubled
             same = all(bool(((a[k] == b[k]) | (a[k].isnan() & b[k].isnan())).all()) if a[k].is_floating_point()
                        else torch.equal(a[k], b[k]) for a, b in zip(outs, ref) for k in a)
             flops = 0.0
@@ -347,14 +377,18 @@ def bench_mixed(steps, warmup, n=0):
         del samples, outs, ref, bins
     torch.cuda.empty_cache()
     total = max(sum(bin_ms), 1e-9)
-    shares = sorted(({"kind": k, "R": R, "n": c, "lanes": g, "ms": round(t, 4), "share": round(t / total, 4)}
-                     for (k, R), c, g, t in zip(meta, counts, groups, bin_ms)), key=lambda d: -d["ms"])
+    shares = sorted(({"kind": k, "R": R, "n": c, "lanes": g, "ms": round(t, 4), "share": round(t / total, 4),
+                      "fixed_ms": round(2 * t - t2, 4)}
+                     for (k, R), c, g, t, t2 in zip(meta, counts, groups, bin_ms, bin_ms2)), key=lambda d: -d["ms"])
+    marginal = sum(t2 - t for t, t2 in zip(bin_ms, bin_ms2))
     achieved = flops / (batch_gpu * 1e-3) / 1e12
     roof = {"kernel": "all fit and CRLB kernels of the batch", "bound": BOUND, "compute": COMPUTE_NOTE,
             "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None}
     detail = {"plain_ms_per_step": plain_ms, "plain_loc_per_s": n / (plain_ms * 1e-3),
               "batch_over_plain": plain_ms / batch_ms, "batch_gpu_ms": batch_gpu, "plain_gpu_ms": plain_gpu,
-              "results_identical": bool(same), "bin_time_shares": shares}
+              "results_identical": bool(same), "bin_time_shares": shares,
+              "sum_of_bin_ms": total, "sum_of_marginal_bin_ms": marginal,
+              "schedule_bound_over_plain": total / max(marginal, 1e-9)}
     return line(n / (batch_ms * 1e-3), "localizations/s", steps, warmup, batch_ms, synthetic.MIXED["workload"],
                 {"spots": n, "bins": len(counts), "streams": streams, "spline_form": forms}, roof, None, detail)
 
@@ -538,17 +572,19 @@ def main():
         print(json.dumps(bench_queue(args.steps, args.warmup, args.n)))
         return
     if args.config != "all":
-        print(json.dumps(bench_fit(args.config, args.steps, args.warmup, args.n, args.group, args.refill, cpu=cpu,
-                                   e2e=not args.no_e2e)))
+        res, job = bench_fit(args.config, args.steps, args.warmup, args.n, args.group, args.refill, cpu=cpu,
+                             e2e=not args.no_e2e)
+        finish_cpu(res, job)
+        print(json.dumps(res))
         return
     # headline: c2 with the driver's K and W; then every other workload of the metric
     t_all = time.perf_counter()
-    head = bench_fit("c2", args.steps, args.warmup, cpu=cpu, e2e=not args.no_e2e)
+    head, head_job = bench_fit("c2", args.steps, args.warmup, cpu=cpu, e2e=not args.no_e2e)
     print(f"bench: c2 {head['value']:.4g} loc/s", file=sys.stderr, flush=True)
-    configs = {}
+    configs, jobs = {}, {}
     for name in ("c1", "c3", "c4"):
         k, w = SUB_STEPS[name]
-        configs[name] = bench_fit(name, k, w, cpu=cpu, e2e=False)
+        configs[name], jobs[name] = bench_fit(name, k, w, cpu=cpu, e2e=False)
         print(f"bench: {name} {configs[name]['value']:.4g} loc/s", file=sys.stderr, flush=True)
     k, w = SUB_STEPS["c4adj"]
     configs["c4adj"] = bench_adjoint(k, w)
@@ -556,6 +592,10 @@ def main():
     k, w = SUB_STEPS["c5"]
     configs["c5"] = bench_mixed(k, w)
     print(f"bench: c5 {configs['c5']['value']:.4g} loc/s", file=sys.stderr, flush=True)
+    # CPU baselines last: no GPU leg runs after the host has been loaded
+    finish_cpu(head, head_job)
+    for name, job in jobs.items():
+        finish_cpu(configs[name], job)
     for c in configs.values():
         for key in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "dtype", "data"):
             c.pop(key, None)
