@@ -244,6 +244,12 @@ int Estimator_GetGroupSize(Estimator* e);
 /* Number of stopped lane groups of a wavefront that are refilled together
  * (0 = one eighth of the wavefront's groups, at least 1). */
 int Estimator_SetRefill(Estimator* e, int refill);
+/* Work counters of a fit launch larger than the resident grid (no reference
+ * counterpart; a tuning knob like the two above): 1 = one counter, 8 = the spots in 8
+ * contiguous segments with a counter each, one per XCD.  Returns nonzero for other
+ * values.  Results do not depend on it. */
+int Estimator_SetWorkSegments(Estimator* e, int segments);
+int Estimator_GetWorkSegments(Estimator* e);
 
 /* Kernel timing of the fit (no reference counterpart; for benchmarks).  While
  * enabled, every fit launch records HIP events on its stream around the fit
@@ -269,11 +275,11 @@ int Estimator_EstimateDevice(Estimator* e, int n, const float* d_sample, const f
                              float* d_crlb, float* d_chisq, void* stream);
 /* No counterpart in the reference.  For callers that cannot allocate (code under HIP graph
  * capture, serving loops, harnesses that forbid hipMalloc): Estimator_EstimateDevice with the
- * same arguments and results, allocating nothing.  Its scratch -- the fit kernel's work
- * counter -- comes from a fixed array of 64 counters (256 bytes) in the library's device
- * memory, one per call in turn, reset by a kernel on `stream`; it does not grow with n (n is
- * bounded as for Estimator_EstimateDevice).  Calls that overlap on one GPU with 64 or more
- * others in flight share a counter and race.  A spline estimator's tables must already be on
+ * same arguments and results, allocating nothing.  Its scratch -- the fit kernel's 8 work
+ * counters -- comes from a fixed array of 64 slots of 8 counters (2 KiB) in the library's
+ * device memory, one slot per call in turn, reset by a kernel on `stream`; it does not grow
+ * with n (n is bounded as for Estimator_EstimateDevice).  Calls that overlap on one GPU with
+ * 64 or more others in flight share a slot and race.  A spline estimator's tables must already be on
  * the device (any other fit or evaluation call on it uploads them); otherwise the call fails
  * with an error. */
 int Estimator_EstimateDeviceNoAlloc(Estimator* e, int n, const float* d_sample, const float* d_constants,
@@ -305,10 +311,10 @@ int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const fl
  * bin.  The bins are spread longest-first over nstreams internal HIP streams (0 = 4, one per
  * hardware queue of the process) that run concurrently, so one bin's tail overlaps the next
  * bins' work and different models share the CUs; each bin runs the estimator's own persistent
- * fit + CRLB launches with a work counter the batch owns.  With useGraph the fork, every launch
+ * fit + CRLB launches with work counters the batch owns.  With useGraph the fork, every launch
  * and the join are captured into one HIP graph on the first run and replayed with one
  * hipGraphLaunch on `stream`; a graph run with other buffers or changed estimator settings (LM
- * parameters, group size, refill batch, spline form) than the captured ones is refused with an
+ * parameters, group size, refill batch, work segments, spline form) than the captured ones is refused with an
  * error (make a new MixedBatch for them; eager runs take any buffers).  Run takes per-bin
  * device pointers (arrays of nbins pointers; d_initial [n,K] per bin, or the whole array NULL
  * for the centre-of-mass start; d_iters/d_crlb/d_chisq may be NULL).  Runs of one batch must

@@ -6,6 +6,8 @@ Knobs tuned per (model, ROI size) and stored in ``tuning.json`` next to this fil
                block size it sets the spots per block and, for the grid-form spline,
                the LDS window tables per block
 * ``refill``   stopped lane groups of a wavefront that are refilled together
+* ``segs``     work counters of a launch larger than the resident grid: 1, or 8
+               contiguous spot segments with a counter each
 
 and for a mixed batch ``streams``, the number of concurrent branches.
 ``tune_group`` times the compiled group sizes x refill batches of an estimator on
@@ -72,6 +74,8 @@ def apply(est):
         est.SetGroupSize(int(entry["group"]))
     if entry.get("refill") is not None:
         est.SetRefill(int(entry["refill"]))
+    if entry.get("segs"):
+        est.SetWorkSegments(int(entry["segs"]))
     return est.GetGroupSize()
 
 
@@ -88,9 +92,9 @@ def _time(fn, reps=3):
     return best
 
 
-def tune_group(est, samples, refills=(0, 1, 2, 4, 8), persist=True):
-    """Time every compiled group size x refill batch for this estimator's ROI size on
-    device samples [N,R,R]; keep the fastest pair."""
+def tune_group(est, samples, refills=(0, 1, 2, 4, 8), segments=(1, 8), persist=True):
+    """Time every compiled group size x refill batch x work-segment count for this
+    estimator's ROI size on device samples [N,R,R]; keep the fastest."""
     results = {}
     for g in compiled_groups(est):
         est.SetGroupSize(g)
@@ -98,15 +102,18 @@ def tune_group(est, samples, refills=(0, 1, 2, 4, 8), persist=True):
             if rf > 64 // g:
                 continue
             est.SetRefill(rf)
-            results[(g, rf)] = _time(lambda: est.estimate_device(samples, crlb=True))
+            for sg in segments:
+                est.SetWorkSegments(sg)
+                results[(g, rf, sg)] = _time(lambda: est.estimate_device(samples, crlb=True))
     best = min(results, key=results.get)
     est.SetGroupSize(best[0])
     est.SetRefill(best[1])
+    est.SetWorkSegments(best[2])
     if persist:
         t = load()
         e = t.setdefault(key(est), {})
-        e["group"], e["refill"] = int(best[0]), int(best[1])
-        e["loc_per_s"] = {f"G{g}_refill{rf}": samples.shape[0] / v for (g, rf), v in results.items()}
+        e["group"], e["refill"], e["segs"] = int(best[0]), int(best[1]), int(best[2])
+        e["loc_per_s"] = {f"G{g}_refill{rf}_segs{sg}": samples.shape[0] / v for (g, rf, sg), v in results.items()}
         save(t)
     return best, results
 
@@ -133,14 +140,14 @@ def batch_streams(default=4):
 
 
 def main():
-    """python -m photonpy_amd.autotune [--configs c1,c2,c3,c4] [--n 200000] [--bins] [--batch-n 400000]"""
+    """python -m photonpy_amd.autotune [--configs c1,c2,c3,c4] [--n 0] [--bins] [--batch-n 400000]"""
     import argparse
     import torch
     from . import Context
     from . import synthetic
     ap = argparse.ArgumentParser()
     ap.add_argument("--configs", default="c1,c2,c3,c4")
-    ap.add_argument("--n", type=int, default=200_000)
+    ap.add_argument("--n", type=int, default=0, help="spots per config run (0 = the workload's own count)")
     ap.add_argument("--batch-n", type=int, default=0, help="also tune the mixed batch's stream count on this many ROIs")
     ap.add_argument("--bins", action="store_true",
                     help="tune every (model, ROI size) bin of the mixed workload not yet in the table, on its own ROIs")
@@ -150,10 +157,10 @@ def main():
             w = synthetic.WORKLOADS[c]
             est = synthetic.make_estimator(ctx, w["kind"], w["R"])
             est.SetLevMarParams(w["lam"], w["iters"])
-            _, smp = synthetic.make_samples(est, w["kind"], min(args.n, w["n"]) if c != "c1" else w["n"])
+            _, smp = synthetic.make_samples(est, w["kind"], min(args.n, w["n"]) if args.n > 0 else w["n"])
             best, res = tune_group(est, smp)
-            print(c, key(est), "best", best, {f"{g}/{r}": round(smp.shape[0] / v / 1e6, 2) for (g, r), v in res.items()},
-                  flush=True)
+            print(c, key(est), "best", best,
+                  {f"{g}/{r}/{s}": round(smp.shape[0] / v / 1e6, 2) for (g, r, s), v in res.items()}, flush=True)
             del smp
             torch.cuda.empty_cache()
         if args.bins:
@@ -166,7 +173,7 @@ def main():
                 best, res = tune_group(b["est"], b["samples"])
                 tuned.add(k)
                 print("bin", k, b["n"], "best", best,
-                      {f"{g}/{r}": round(b["n"] / v / 1e6, 2) for (g, r), v in res.items()}, flush=True)
+                      {f"{g}/{r}/{s}": round(b["n"] / v / 1e6, 2) for (g, r, s), v in res.items()}, flush=True)
                 del b
                 torch.cuda.empty_cache()
         if args.batch_n:

@@ -18,7 +18,7 @@ struct MixedBatch {
     hipGraphExec_t exec = nullptr;
     hipStream_t lastLaunch = nullptr;    // stream of the last graph launch
     std::vector<uintptr_t> capturedKey;  // buffers and estimator settings the graph was captured with
-    // Work counters of the batch's fit launches, one per bin, owned by the batch: a
+    // Work counters of the batch's fit launches, kWorkSegs per bin, owned by the batch: a
     // replayed graph reuses them, and no other launch of the estimators touches them.
     int* counters = nullptr;
 
@@ -69,7 +69,7 @@ static void launch_bins(MixedBatch* mb, const float* const* d_samples, const flo
             std::lock_guard<std::mutex> l(e->mtx);
             run_fit(e, mb->counts[bin], d_samples[bin], nullptr, d_initial ? d_initial[bin] : nullptr, d_params[bin],
                     d_iters ? d_iters[bin] : nullptr, nullptr, 0, d_crlb ? d_crlb[bin] : nullptr,
-                    d_chisq ? d_chisq[bin] : nullptr, s, false, 0, mb->counters + bin);
+                    d_chisq ? d_chisq[bin] : nullptr, s, false, 0, mb->counters + (size_t)bin * kCounterInts);
         }
         hip_check(hipEventRecord(mb->joins[b], s), "hipEventRecord(join)");
         hip_check(hipStreamWaitEvent(origin, mb->joins[b], 0), "hipStreamWaitEvent(join)");
@@ -129,7 +129,7 @@ MixedBatch* MixedBatch_Create(int nbins, Estimator* const* ests, const int* coun
                     mb->joins.push_back(ev);
                 }
                 hip_check(hipEventCreateWithFlags(&mb->fork, hipEventDisableTiming), "hipEventCreate");
-                hip_check(dev_alloc(&mb->counters, sizeof(int) * (size_t)nbins), "hipMalloc(batch counters)");
+                hip_check(dev_alloc(&mb->counters, sizeof(int) * kCounterInts * (size_t)nbins), "hipMalloc(batch counters)");
             } catch (...) {
                 delete mb;
                 throw;
@@ -159,7 +159,7 @@ int MixedBatch_Run(MixedBatch* mb, const float* const* d_samples, const float* c
                 launch_bins(mb, d_samples, d_initial, d_params, d_iters, d_crlb, d_chisq, st);
                 return 0;
             }
-            // the kernels' buffers and arguments (LM settings, lane groups, refill batch, spline
+            // the kernels' buffers and arguments (LM settings, lane groups, refill batch, work segments, spline
             // form) are baked into the graph: capture again when any of them changed
             std::vector<uintptr_t> key;
             auto ptr = [](const void* p) { return reinterpret_cast<uintptr_t>(p); };
@@ -170,7 +170,7 @@ int MixedBatch_Run(MixedBatch* mb, const float* const* d_samples, const float* c
                 std::memcpy(&lamBits, &lam, 4);
                 for (uintptr_t v : {ptr(d_samples[b]), ptr(d_initial ? d_initial[b] : nullptr), ptr(d_params[b]),
                                     ptr(d_iters ? d_iters[b] : nullptr), ptr(d_crlb ? d_crlb[b] : nullptr),
-                                    ptr(d_chisq ? d_chisq[b] : nullptr), (uintptr_t)e->G(), (uintptr_t)e->refill,
+                                    ptr(d_chisq ? d_chisq[b] : nullptr), (uintptr_t)e->G(), (uintptr_t)e->refill, (uintptr_t)e->segShift,
                                     (uintptr_t)e->desc.useGrid, (uintptr_t)e->lm.iterations, (uintptr_t)lamBits})
                     key.push_back(v);
             }
@@ -216,7 +216,7 @@ int MixedBatch_RunPlain(MixedBatch* mb, const float* const* d_samples, const flo
                 std::lock_guard<std::mutex> l(e->mtx);
                 run_fit(e, mb->counts[b], d_samples[b], nullptr, d_initial ? d_initial[b] : nullptr, d_params[b],
                         d_iters ? d_iters[b] : nullptr, nullptr, 0, d_crlb ? d_crlb[b] : nullptr,
-                        d_chisq ? d_chisq[b] : nullptr, as_stream(stream), false, 0, mb->counters + b);
+                        d_chisq ? d_chisq[b] : nullptr, as_stream(stream), false, 0, mb->counters + (size_t)b * kCounterInts);
             }
             return 0;
         },

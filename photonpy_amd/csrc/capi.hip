@@ -194,6 +194,7 @@ void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constant
     a.lambda = e->lm.lambda;
     a.refill = e->refill;
     a.counter = counter ? counter : e->next_counter();
+    a.segShift = e->segShift;
     a.constants = d_constants;
     if (e->NC() > 0 && !d_constants)
         throw std::runtime_error("photonpy_amd: this estimator needs " + std::to_string(e->NC()) +
@@ -223,10 +224,10 @@ void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constant
 // ---------------------------------------------------------------- allocation-free fit
 // Work counters of Estimator_EstimateDeviceNoAlloc: a fixed array in device memory
 // (each device holds its own copy until the process ends), so that the call itself
-// allocates nothing.  Each call takes the next of the kNoAllocSlots counters and
-// resets it with a kernel on its stream before the fit reads it.
+// allocates nothing.  Each call takes the next of the kNoAllocSlots slots of
+// kWorkSegs counters and resets it with a kernel on its stream before the fit reads it.
 constexpr int kNoAllocSlots = 64;
-static __device__ __align__(256) int scratch[kNoAllocSlots];
+static __device__ __align__(256) int scratch[kNoAllocSlots * kCounterInts];
 
 // Device address of `scratch` on the current device, resolved once per device with
 // that device current.  Host code passes this address, never the array's name.
@@ -628,6 +629,18 @@ int Estimator_SetRefill(Estimator* e, int refill) {
     return 0;
 }
 
+int Estimator_SetWorkSegments(Estimator* e, int segments) {
+    e = model_of(e);
+    if (!e || (segments != 1 && segments != kWorkSegs)) return 1;
+    e->segShift = segments == 1 ? 0 : 3;
+    return 0;
+}
+
+int Estimator_GetWorkSegments(Estimator* e) {
+    e = model_of(e);
+    return e ? 1 << e->segShift : 0;
+}
+
 int Estimator_SetSplineMode(Estimator* e, int mode) {
     e = model_of(e);
     if (!e || !is_spline(e->desc.kind) || mode < 0 || mode > 1) return 1;
@@ -863,7 +876,7 @@ int Estimator_EstimateDeviceNoAlloc(Estimator* e, int n, const float* d_sample, 
         [&]() -> int {
             std::lock_guard<std::mutex> l(e->mtx);
             int* counters = scratch_on_current_device();
-            int* counter = counters + (g_noAllocSlot.fetch_add(1) % kNoAllocSlots);
+            int* counter = counters + (size_t)(g_noAllocSlot.fetch_add(1) % kNoAllocSlots) * kCounterInts;
             run_fit(e, n, d_sample, d_constants, d_initial, d_params, d_iterations, d_trace, traceLen, d_crlb, d_chisq,
                     as_stream(stream), false, 0, counter);
             return 0;

@@ -89,6 +89,7 @@ struct Estimator {
     int glrtMeanOfSpot0 = 1;  // the reference's GLRT mean (first spot of the call)
     int group = 0;                  // 0: default_group(roisize)
     int refill = 0;                 // stopped groups per batched refill (0: kernel default)
+    int segShift = 0;               // log2 of the fit's work segments (0: one counter, 3: eight)
     // spline coefficients: host copy (owned) and lazily uploaded device copy
     std::vector<float> hostCoefs;
     float* devCoefs = nullptr;
@@ -156,8 +157,9 @@ struct Estimator {
     }
 
     float* devShared = nullptr;  // device copy of the shared PSF parameters (Gaussian models)
-    // Work counters of the fit kernel: one slot per launch, cycling, so launches of this
-    // estimator on different streams do not share a counter unless 64 are in flight.
+    // Work counters of the fit kernel: one slot of kWorkSegs counters per launch,
+    // cycling, so launches of this estimator on different streams do not share
+    // counters unless 64 are in flight.
     static constexpr int kCounterSlots = 64;
     int* devCounters = nullptr;
     unsigned counterSlot = 0;
@@ -173,12 +175,13 @@ struct Estimator {
     }
 
     void ensure_counters() {
-        if (!devCounters) hip_check(dev_alloc(&devCounters, kCounterSlots * sizeof(int)), "hipMalloc(counters)");
+        if (!devCounters)
+            hip_check(dev_alloc(&devCounters, kCounterSlots * kCounterInts * sizeof(int)), "hipMalloc(counters)");
     }
 
     int* next_counter() {
         ensure_counters();
-        return devCounters + (counterSlot++ % kCounterSlots);
+        return devCounters + (size_t)(counterSlot++ % kCounterSlots) * kCounterInts;
     }
 
     int K() const { return num_params(desc.kind); }
@@ -308,8 +311,8 @@ int fit_group(Estimator* e, int groupHint);
 // groupHint > 0: try that many lanes per spot first (halving down to the estimator's
 // own group size while no kernel is compiled for it) -- small batches fill more of
 // the GPU with more lanes per spot.
-// counter != null: use that work counter and allocate nothing (the estimator's device
-// tables must already be resident; no timing events are recorded).
+// counter != null: use those kWorkSegs work counters and allocate nothing (the
+// estimator's device tables must already be resident; no timing events are recorded).
 void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constants, const float* d_initial,
              float* d_params, int* d_iter, float* d_trace, int traceLen, float* d_crlb, float* d_chisq, hipStream_t st,
              bool timing = true, int groupHint = 0, int* counter = nullptr);

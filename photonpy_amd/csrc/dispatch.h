@@ -147,8 +147,9 @@ size_t spline_adjoint_workspace(int n, int nkeys);
 // ws: spline_adjoint_workspace bytes of device scratch, or null (direct atomics).
 hipError_t launch_adjoint_any(const ModelDesc& d, const AdjArgs& a, void* ws, hipStream_t st);
 hipError_t launch_spline_coefs(const int* shape, const float* d_data, float* d_out, int border, hipStream_t st);
-// Sets *counter = value on the stream (a one-thread kernel, so it can also be captured in a graph).
-hipError_t launch_set_counter(int* counter, int value, hipStream_t st);
+// Sets the 2^shift work counters of a fit launch of `blocks` blocks on the stream
+// (a one-wave kernel, so it can also be captured in a graph).
+hipError_t launch_set_counters(int* counters, int blocks, int groupsPerBlock, int shift, hipStream_t st);
 
 inline hipError_t launch_fit_any(const ModelDesc& d, int R, int G, const FitArgs& a, hipEvent_t* events, hipStream_t st) {
     switch (d.kind) {
@@ -192,12 +193,14 @@ constexpr int fit_block(int G, int ldsFloatsPerGroup) {
 }
 
 // Persistent grids: enough blocks to cover every group once, capped at what the
-// device keeps resident.  The fit kernel starts every group on spot
-// blockIdx * groupsPerBlock + group and hands out the rest from the counter.
+// device keeps resident.  With work counters the capped fit kernel starts block b's
+// groups on segment b % kWorkSegs and hands out the rest from the segments' counters;
+// without them (or uncapped), on spot b * groupsPerBlock + group with the grid as
+// stride.
 template <class M, int R, int G>
 hipError_t launch_fit_t(const M& m, const FitArgs& a, hipEvent_t* events, hipStream_t st) {
-    constexpr int kBlock = fit_block(G, M::lds_floats(R));
-    constexpr size_t kLds = (size_t)(kBlock / G) * M::lds_floats(R) * sizeof(float);
+    constexpr int kBlock = fit_block(G, M::lds_floats(R, G));
+    constexpr size_t kLds = (size_t)(kBlock / G) * M::lds_floats(R, G) * sizeof(float);
     static_assert(kLds <= 65536, "per-group LDS table too large for one lane group per 64 lanes");
     static int residentFit = 0, residentCrlb = 0;  // per process; one device model (gfx950)
     if (residentFit == 0) {
@@ -209,12 +212,17 @@ hipError_t launch_fit_t(const M& m, const FitArgs& a, hipEvent_t* events, hipStr
     const long need = (a.n + groupsPerBlock - 1) / groupsPerBlock;
     const long fitBlocks = need < 1 ? 1 : (need > residentFit ? residentFit : need);
     const long crlbBlocks = need < 1 ? 1 : (need > residentCrlb ? residentCrlb : need);
-    if (a.counter) {
-        const hipError_t e = launch_set_counter(a.counter, (int)(fitBlocks * groupsPerBlock), st);
+    // A grid that covers every spot once hands out nothing: its groups take spots
+    // b * groupsPerBlock + group and stop (no counters, no atomics).  A capped
+    // (persistent) grid takes the rest from the work counters.
+    FitArgs fa = a;
+    if (need <= residentFit) fa.counter = nullptr;
+    if (fa.counter) {
+        const hipError_t e = launch_set_counters(fa.counter, (int)fitBlocks, (int)groupsPerBlock, fa.segShift, st);
         if (e != hipSuccess) return e;
     }
     if (events) (void)hipEventRecord(events[0], st);
-    hipLaunchKernelGGL((fit_kernel<M, R, G>), dim3((unsigned)fitBlocks), dim3(kBlock), kLds, st, m, a);
+    hipLaunchKernelGGL((fit_kernel<M, R, G>), dim3((unsigned)fitBlocks), dim3(kBlock), kLds, st, m, fa);
     if (events) (void)hipEventRecord(events[1], st);
     if (a.crlb || a.chisq)
         hipLaunchKernelGGL((crlb_phase_kernel<M, R, G>), dim3((unsigned)crlbBlocks), dim3(kBlock), kLds, st, m, a);

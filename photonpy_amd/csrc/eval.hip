@@ -238,10 +238,16 @@ hipError_t launch_spline_evaluate(const int* shape, const float* d_coefs, const 
     return hipGetLastError();
 }
 
-__global__ void set_counter_kernel(int* counter, int value) { *counter = value; }
+// counters[s] = the spots of segment s (of 2^shift) that the blocks starting there
+// take first: blocks b < blocks with b % 2^shift == s, groupsPerBlock spots each
+__global__ void set_counters_kernel(int* counters, int blocks, int groupsPerBlock, int shift) {
+    const int s = (int)threadIdx.x;
+    if (s < (1 << shift))
+        counters[s * kCounterStride] = (s < blocks ? ((blocks - 1 - s) >> shift) + 1 : 0) * groupsPerBlock;
+}
 
-hipError_t launch_set_counter(int* counter, int value, hipStream_t st) {
-    hipLaunchKernelGGL(set_counter_kernel, dim3(1), dim3(1), 0, st, counter, value);
+hipError_t launch_set_counters(int* counters, int blocks, int groupsPerBlock, int shift, hipStream_t st) {
+    hipLaunchKernelGGL(set_counters_kernel, dim3(1), dim3(kWave), 0, st, counters, blocks, groupsPerBlock, shift);
     return hipGetLastError();
 }
 

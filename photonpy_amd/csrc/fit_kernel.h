@@ -9,15 +9,22 @@
 // them with lane shuffles, and every lane solves the FP64 damped Cholesky system
 // redundantly (identical inputs give identical steps, so the group stays
 // convergent).  Models that stage per-spot tables in LDS get a region of
-// M::lds_floats(R) floats per group.
+// M::lds_floats(R, G) floats per group.
 //
 // Spots stop after different numbers of iterations.  A group whose spot has
 // stopped waits until `refill` groups of its wavefront (or every still-running
 // one) have stopped; those groups then write their results and load their next
 // spots together, so the wavefront pays the partly-masked refill once per batch
-// of groups instead of once per spot.  Next spots come from a work counter in
+// of groups instead of once per spot.  Next spots come from work counters in
 // HBM (one atomic per batch), so every group stays busy until the batch is
-// exhausted and the launch ends within about one spot's fit of each other.  The
+// exhausted and the launch ends within about one spot's fit of each other.
+// The spots can be split into 2^segShift (1 or kWorkSegs) contiguous segments
+// with a counter each, block b starting on segment b % 2^segShift: the hardware
+// deals a launch's blocks to the 8 XCDs in turn, so neighbouring spots are fitted
+// under one L2, and the claims of a fit that refills often are spread over eight
+// counters (one counter measured 6% slower on 16x16 astigmatic fits, eight 3%
+// slower on 11x11 free-sigma fits; the tuner picks per model and size).  A wave
+// whose segment runs out moves on to the next one with spots left.  The
 // CRLB and chi^2 are computed by a second kernel that walks all spots in
 // lockstep (every group does exactly one spot per pass), reading the samples and
 // the fitted parameters back from HBM.
@@ -44,9 +51,21 @@ struct FitArgs {
     int maxIter;
     float lambda;
     int refill;             // stopped groups per batched refill (<= 0: one eighth of the wave's groups)
-    int* counter;           // next unclaimed spot (starts at the grid's group count), or null: static stride
+    int* counter;           // next unclaimed spot of each segment (relative to the segment) at
+                            // counter[s * kCounterStride] (set by launch_set_counters), or null:
+                            // static stride
+    int segShift;           // log2 of the work segments (0 or 3)
     const float* constants; // [*, NC] per-spot model constants (models with NC > 0)
 };
+
+// Work segments of a fit launch (at most one per XCD of gfx950) and segment s's first
+// spot of 2^shift segments.
+constexpr int kWorkSegs = 8;
+// ints from one segment counter to the next (each on a 256-byte line of its own) and
+// ints of one launch's counter set
+constexpr int kCounterStride = 64;
+constexpr int kCounterInts = kWorkSegs * kCounterStride;
+PP_DEV int seg_begin(int n, int s, int shift) { return (int)(((long long)n * s) >> shift); }
 
 template <class M, int R, int G>
 struct FitTraits {
@@ -56,7 +75,7 @@ struct FitTraits {
     static constexpr int S = R * R;
     static constexpr int NS = L::NS;             // samples per lane
     static constexpr int NA = K * (K + 1) / 2;   // packed alpha
-    static constexpr int LDS = M::lds_floats(R); // LDS floats per group
+    static constexpr int LDS = M::lds_floats(R, G); // LDS floats per group
     static constexpr int NB = M::kPixelBg ? NS : 1;  // background-image values per lane
 };
 
@@ -145,7 +164,7 @@ PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTrai
         }
     };
     if constexpr (row_factored<M>())
-        model.template accumulate_rows<R, G, true>(th, g, s, fi);
+        model.template accumulate_rows<R, G, true>(th, g, s, fi, lds);
     else
         model.template for_pixels<R, G>(th, g, lds, pixel_bg<M>(th, pb, body));
     group_sum_n<G>(fi);
@@ -162,11 +181,13 @@ PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTrai
 }
 
 // Fits spots first, first+stride, ... < a.n with the lane groups of the calling
-// wavefront (or, with a work counter, the spots it hands out); g_first is this
-// lane group's first spot and lds its LDS region.  Every lane of the wavefront
-// must call it (the loops run until all groups of the wave are done).
+// wavefront, or, with work counters, its group's first spot g_first of segment seg
+// and then the spots the counters hand out; lds is the group's LDS region.  Every
+// lane of the wavefront must call it (the loops run until all groups of the wave
+// are done) with the same seg.
 template <class M, int R, int G>
-__device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int g_first, int stride, float* lds) {
+__device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int g_first, int stride, float* lds,
+                                          int seg = 0) {
     using T = FitTraits<M, R, G>;
     using L = typename T::L;
     constexpr int K = T::K, NS = T::NS, NA = T::NA;
@@ -184,6 +205,11 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
     float th[KT];
     int iter = 0;
     int spot = g_first;
+    const int nseg = 1 << a.segShift;
+    if (a.counter) {
+        const int lo = seg_begin(a.n, seg, a.segShift);
+        spot = g_first < seg_begin(a.n, seg + 1, a.segShift) - lo ? lo + g_first : a.n;
+    }
 
     auto start_spot = [&](int sp) {
         float sx, sy, sum;
@@ -203,34 +229,69 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
         iter = 0;
     };
 
-    // state 0 = iterating, 1 = stopped (results not yet written), 2 = no more spots
+    // state 0 = iterating, 1 = stopped (results not yet written; spot < 0: waiting for a
+    // first spot), 2 = no more spots.  With work counters a group whose segment has no
+    // first spot for it claims one from the counters like a stopped group.
     int state = 2;
     if (spot < a.n) {
         start_spot(spot);
         state = 0;
+    } else if (a.counter) {
+        spot = -1;
+        state = 1;
     }
     for (;;) {
         const unsigned long long stopped = __ballot(lead && state == 1);
         const unsigned long long running = __ballot(lead && state == 0);
         if (!stopped && !running) break;
         if (stopped && (!running || __popcll(stopped) >= (unsigned)refill)) {
-            // next spots: claimed together from the work counter (one atomic per batch,
-            // handed out in lane order), or the static stride
-            int base = 0;
+            // next spots: claimed together from the segment's work counter (one atomic
+            // per batch, handed out in lane order; the rest from the next segment when
+            // this one runs out), or the static stride
+            // (the claim bookkeeping is wave-uniform: kept in scalar registers)
+            int next = a.n;
             if (a.counter) {
-                if (lane == 0) base = atomicAdd(a.counter, (int)__popcll(stopped));
-                base = __shfl(base, 0, kWave);
+                const int need = (int)__popcll(stopped);
+                const int leadLane = lane & ~(G - 1);
+                const int rank = (int)__popcll(stopped & ((1ull << leadLane) - 1ull));
+                int done = 0;
+                // every try serves the batch or uses a segment up, so nseg tries visit
+                // every segment that still has spots
+                for (int tries = 0; done < need && tries < nseg; tries++) {
+                    int base = 0;
+                    if (lane == 0) base = atomicAdd(a.counter + seg * kCounterStride, need - done);
+                    base = __builtin_amdgcn_readfirstlane(base);
+                    const int lo = seg_begin(a.n, seg, a.segShift);
+                    const int avail = max(0, min(need - done, seg_begin(a.n, seg + 1, a.segShift) - lo - base));
+                    if (rank >= done && rank < done + avail) next = lo + base + (rank - done);
+                    done += avail;
+                    if (done < need) {
+                        // this segment is used up: read the other counters at once (lane k
+                        // the k-th next segment's) and go on with the first that has spots
+                        // left; counters only grow, so one seen used up stays so
+                        bool left = false;
+                        if (lane < nseg - 1) {
+                            const int s = (seg + 1 + lane) & (nseg - 1);
+                            const int c = __hip_atomic_load(a.counter + s * kCounterStride, __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT);
+                            left = c < seg_begin(a.n, s + 1, a.segShift) - seg_begin(a.n, s, a.segShift);
+                        }
+                        const unsigned long long has = __ballot(left);
+                        if (!has) break;
+                        seg = (seg + __ffsll((unsigned long long)has)) & (nseg - 1);
+                        seg = __builtin_amdgcn_readfirstlane(seg);
+                    }
+                }
             }
             if (state == 1) {
-                if (lead) {
+                if (lead && spot >= 0) {
                     float* pp = a.params + (size_t)spot * K;
 #pragma unroll
                     for (int k = 0; k < K; k++) pp[k] = th[k];
                     if (a.iterations) a.iterations[spot] = iter;
                 }
                 if (a.counter) {
-                    const int leadLane = lane & ~(G - 1);
-                    spot = base + (int)__popcll(stopped & ((1ull << leadLane) - 1ull));
+                    spot = next;
                 } else {
                     spot += stride;
                 }
@@ -275,7 +336,7 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
             for (int i = 0; i < K; i++) acc[NA + i] += bf * J[i];
         };
         if constexpr (row_factored<M>()) {
-            model.template accumulate_rows<R, G>(th, g, s, acc);
+            model.template accumulate_rows<R, G>(th, g, s, acc, lds);
         } else {
             model.template for_pixels<R, G>(th, g, lds, pixel_bg<M>(th, pb, body));
         }
@@ -332,7 +393,7 @@ __device__ __forceinline__ void crlb_spots(const M& model, const FitArgs& a, int
 template <class M, int R, int G>
 PP_DEV float* group_lds() {
     extern __shared__ float ppa_lds[];
-    constexpr int per = M::lds_floats(R);
+    constexpr int per = M::lds_floats(R, G);
     return per > 0 ? ppa_lds + (size_t)(threadIdx.x / G) * per : nullptr;
 }
 
@@ -355,8 +416,13 @@ template <class M, int R, int G>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves<M, R, G>::value)))
 fit_kernel(M model, FitArgs a) {
     const int groupsPerBlock = blockDim.x / G;
-    fit_spots<M, R, G>(model, a, blockIdx.x * groupsPerBlock + threadIdx.x / G, gridDim.x * groupsPerBlock,
-                       group_lds<M, R, G>());
+    // with work counters block b starts as the (b >> segShift)-th block of segment
+    // b & (2^segShift - 1) (one call site: the fit loop is inlined once)
+    const bool segs = a.counter != nullptr;
+    const int seg = segs ? (int)(blockIdx.x & ((1u << a.segShift) - 1)) : 0;
+    const int blk = segs ? (int)(blockIdx.x >> a.segShift) : (int)blockIdx.x;
+    fit_spots<M, R, G>(model, a, blk * groupsPerBlock + threadIdx.x / G, gridDim.x * groupsPerBlock,
+                       group_lds<M, R, G>(), seg);
 }
 
 template <class M, int R, int G>

@@ -360,7 +360,15 @@ struct GaussModel {
         if (V == kAstig && R == 16 && G == 4) return 3;
         return 1;
     }
-    static constexpr int lds_floats(int) { return 0; }
+    // The row-factored fits of groups of >= 2 lanes keep the R column terms of an
+    // iteration in the group's LDS region, (dE, D, S, pad) per column (sigma / z
+    // models) or (dE, D) (fixed sigma): each lane of the group evaluates its share of
+    // the columns once and every lane reads them per pixel, instead of every lane
+    // holding all of them in VGPRs (which spilled under the three-wave budgets).  A
+    // one-lane group shares nothing and keeps them in VGPRs.
+    static constexpr int kColFloats = (V == kSigma || V == kAstig) ? 4 : 2;
+    static constexpr bool cols_in_lds(int G) { return kRowFactored && G >= 2; }
+    static constexpr int lds_floats(int R, int G) { return cols_in_lds(G) ? kColFloats * R : 0; }
 
     // Fit-kernel evaluation of lane g's pixels: fn(slot, mu, J) for every slot of
     // RowLayout<R, G> that lies inside the ROI.
@@ -402,6 +410,54 @@ struct GaussModel {
                 col[E - 1] = scaled(axis_terms(prev, nx, SX.cp, SX.cs), t[iI]);
                 prev = nx;
             });
+        }
+    }
+
+    // The R column terms (I folded into dE and S) into the group's LDS table: lane g
+    // evaluates columns [g C, g C + C), C = ceil(R / G), from its C + 1 edges.  Called
+    // by every lane of an aligned group of G lanes; the group reads the table after it.
+    template <int R, int G>
+    PP_DEV void columns_lds(const float (&t)[KT], const AxisShape& SX, float* lds) const {
+        constexpr int C = (R + G - 1) / G;
+        const int x0 = (threadIdx.x & (G - 1)) * C;
+        // the previous iteration's reads of the table complete before it is overwritten
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        Edge prev = gauss_edge((float)x0, t[0], SX.c2);
+#pragma unroll
+        for (int j = 0; j < C; j++) {
+            const int x = x0 + j;
+            const Edge nx = gauss_edge((float)(x + 1), t[0], SX.c2);
+            const AxisTerms a = scaled(axis_terms(prev, nx, SX.cp, SX.cs), t[iI]);
+            prev = nx;
+            if (C * G == R || x < R) {
+                if constexpr (kColFloats == 4)
+                    reinterpret_cast<float4*>(lds)[x] = make_float4(a.dE, a.D, a.S, 0.0f);
+                else
+                    reinterpret_cast<float2*>(lds)[x] = make_float2(a.dE, a.D);
+            }
+        }
+        // the group's writes complete before any of its lanes reads the table (one
+        // wavefront: workgroup-scope fences order its LDS operations, no workgroup barrier)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+
+    // Column x's terms from the LDS table for the j-th pixel of a row: a = dE, b = D,
+    // c = S (0 for fixed sigma).  The reads are issued kColAhead pixels at a time (two
+    // (dE, D) pairs make one ds_read2_b64): the compiler barrier (it emits no
+    // instruction) keeps them from being hoisted over the whole row and held in VGPRs.
+    static constexpr int kColAhead = 2;
+    PP_DEV void column_from_lds(const float* lds, int x, int j, float& a, float& b, float& c) const {
+        if (j % kColAhead == 0) asm volatile("" ::: "memory");
+        if constexpr (kColFloats == 4) {
+            const float4 v = reinterpret_cast<const float4*>(lds)[x];
+            a = v.x; b = v.y; c = v.z;
+        } else {
+            const float2 v = reinterpret_cast<const float2*>(lds)[x];
+            a = v.x; b = v.y; c = 0.0f;
         }
     }
 
@@ -533,14 +589,27 @@ struct GaussModel {
     //   FISHER = true (CRLB): w = 1/mu with mu >= 1e-8 (the Fisher matrix), and
     //     chi^2 = sum (x - mu)^2 / mu; acc = packed Fisher matrix, then chi^2.
     template <int R, int G, bool FISHER = false, int NS, int NACC>
-    PP_DEV void accumulate_rows(const float (&t)[KT], int g, const float (&s)[NS], float (&acc)[NACC]) const {
+    PP_DEV void accumulate_rows(const float (&t)[KT], int g, const float (&s)[NS], float (&acc)[NACC],
+                                float* lds) const {
         constexpr int NA = K * (K + 1) / 2;
         static_assert(NACC == NA + (FISHER ? 1 : K), "accumulator layout");
         static_assert(NS == Layout<R, G>::NS, "sample slots of the layout");
         AxisShape SX, SY;
         shapes(t, SX, SY);
-        AxisTerms col[R];
-        columns<R, G>(t, SX, col);
+        constexpr bool kLds = cols_in_lds(G);
+        AxisTerms col[kLds ? 1 : R];
+        if constexpr (kLds)
+            columns_lds<R, G>(t, SX, lds);
+        else
+            columns<R, G>(t, SX, col);
+        // column x's terms for the j-th pixel of a row
+        auto colv = [&](int x, int j, float& a, float& b, float& c) {
+            if constexpr (kLds) {
+                column_from_lds(lds, x, j, a, b, c);
+            } else {
+                a = col[x].dE; b = col[x].D; c = col[x].S;
+            }
+        };
         const float I = t[iI], bg = t[iBg];
         if constexpr (kSplit<R, G>) {
             using SL = SplitRowLayout<R, G>;
@@ -557,7 +626,9 @@ struct GaussModel {
                 prev = nx;
                 RowSums z;
                 row_sums<FISHER, R>(I, bg, row.D, [&](int x, float& a, float& b, float& c, float& xs, bool& ok) {
-                    a = col[x].dE; b = col[x].D; c = col[x].S; xs = s[r * R + x]; ok = true;
+                    colv(x, x, a, b, c);
+                    xs = s[r * R + x];
+                    ok = true;
                 }, z);
                 fold_row<FISHER>(row, z, r == 0, acc);
             }
@@ -569,13 +640,10 @@ struct GaussModel {
             RowSums z;
             row_sums<FISHER, M>(I, bg, row.D, [&](int j, float& a, float& b, float& c, float& xs, bool& ok) {
                 if (M + j < R) {  // lane 0 column j, lane 1 column M + j
-                    const bool hiHalf = l != 0;
-                    a = hiHalf ? col[M + j].dE : col[j].dE;
-                    b = hiHalf ? col[M + j].D : col[j].D;
-                    c = hiHalf ? col[M + j].S : col[j].S;
+                    colv(j + (l != 0 ? M : 0), j, a, b, c);
                     ok = true;
                 } else {  // odd R: column M - 1 for lane 0, padding for lane 1
-                    a = col[j].dE; b = col[j].D; c = col[j].S;
+                    colv(j, j, a, b, c);
                     ok = l == 0;
                 }
                 xs = s[H * R + j];
@@ -594,7 +662,9 @@ struct GaussModel {
                 if (RL * G == R || y < R) {
                     RowSums z;
                     row_sums<FISHER, R>(I, bg, row.D, [&](int x, float& a, float& b, float& c, float& xs, bool& ok) {
-                        a = col[x].dE; b = col[x].D; c = col[x].S; xs = s[r * R + x]; ok = true;
+                        colv(x, x, a, b, c);
+                        xs = s[r * R + x];
+                        ok = true;
                     }, z);
                     fold_row<FISHER>(row, z, r == 0, acc);
                 } else if (r == 0) {
@@ -792,7 +862,7 @@ struct SplineModelT : SplineCommon<BG> {
 
     template <int R, int G>
     using Layout = RowLayout<R, G>;
-    static constexpr int lds_floats(int) { return 0; }
+    static constexpr int lds_floats(int, int) { return 0; }
 
     template <int R, int G, class Fn>
     PP_DEV void for_pixels(const float (&t)[K], int g, float*, Fn&& fn) const {
@@ -860,7 +930,7 @@ struct SplineGridModelT : SplineCommon<BG> {
     // window of R + 4 grid samples per axis (see for_pixels) as (value, z-derivative)
     // pairs, and the per-column and per-row cell offsets (2R ints); an even count
     // keeps every group's pairs 8-byte aligned
-    static constexpr int lds_floats(int R) { return 2 * (R + 4) * (R + 4) + 2 * R; }
+    static constexpr int lds_floats(int R, int) { return 2 * (R + 4) * (R + 4) + 2 * R; }
 
     // Lagrange weights on the nodes -1, 0, 1, 2 at t, and their derivatives.
     PP_DEV static void lagrange(float t, float (&w)[4], float (&dw)[4]) {
@@ -949,7 +1019,7 @@ struct SplineGridModelT : SplineCommon<BG> {
     }
 
     // Fit-kernel evaluation of lane g's pixels (FlatLayout<R, G>), called by every
-    // lane of the group with the group's LDS region (lds_floats(R) floats).
+    // lane of the group with the group's LDS region (lds_floats(R, G) floats).
     template <int R, int G, class Fn>
     PP_DEV void for_pixels(const float (&t)[K], int g, float* lds, Fn&& fn) const {
         // Window width in grid samples.  A pixel's cell is (int)(x - tx) in FP32, which
@@ -1083,7 +1153,7 @@ struct IBgModel {
 
     template <int R, int G>
     using Layout = RowLayout<R, G>;
-    static constexpr int lds_floats(int) { return 0; }
+    static constexpr int lds_floats(int, int) { return 0; }
 
     PP_DEV static float stop_limit(int k) { return k == 0 ? 1e-2f : 1e-4f; }
 

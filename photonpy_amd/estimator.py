@@ -357,6 +357,15 @@ class Estimator:
     def GetGroupSize(self):
         return self._lib.Estimator_GetGroupSize(self.inst)
 
+    def SetWorkSegments(self, segments):
+        """Work counters of a fit launch larger than the resident grid: 1, or 8 (the
+        spots in contiguous segments with a counter each); results do not depend on it."""
+        if self._lib.Estimator_SetWorkSegments(self.inst, int(segments)) != 0:
+            raise ValueError(f"invalid work segment count {segments}")
+
+    def GetWorkSegments(self):
+        return self._lib.Estimator_GetWorkSegments(self.inst)
+
     def spline_form(self):
         """'grid' or 'coefficients' for a spline estimator (None otherwise): the form its
         fit and expected-value kernels evaluate the PSF in."""

@@ -133,6 +133,8 @@ def _bind(lib):
         "Estimator_SetGroupSize": ([V, I], ct.c_int),
         "Estimator_GetGroupSize": ([V], ct.c_int),
         "Estimator_SetRefill": ([V, I], ct.c_int),
+        "Estimator_SetWorkSegments": ([V, I], ct.c_int),
+        "Estimator_GetWorkSegments": ([V], ct.c_int),
         "Estimator_SetPhaseTiming": ([V, I], ct.c_int),
         "Estimator_SetSplineMode": ([V, I], ct.c_int),
         "Estimator_SplineForm": ([V], ct.c_int),

@@ -217,6 +217,29 @@ def test_refill_batches_do_not_change_results(ctx):
         assert np.array_equal(p, outs[0][0], equal_nan=True) and np.array_equal(it, outs[0][1])
 
 
+@pytest.mark.parametrize("kind", ["sigma", "astig"])
+def test_work_segments_do_not_change_results(ctx, kind):
+    # a launch larger than the resident grid hands out spots from one counter or from
+    # eight segment counters; which group fits a spot does not change its fit
+    import torch
+    from photonpy_amd import synthetic
+    R = {"sigma": 11, "astig": 16}[kind]
+    est = synthetic.make_estimator(ctx, kind, R)
+    est.SetLevMarParams(1e-18, 50)
+    _, smp = synthetic.make_samples(est, kind, 400_000, seed=5)
+    outs = []
+    for segs in (1, 8):
+        est.SetWorkSegments(segs)
+        assert est.GetWorkSegments() == segs
+        o = est.estimate_device(smp, crlb=True)
+        torch.cuda.synchronize()
+        outs.append({k: v.cpu().numpy() for k, v in o.items()})
+    with pytest.raises(ValueError):
+        est.SetWorkSegments(3)
+    for k in ("params", "iterations", "crlb"):
+        assert np.array_equal(outs[0][k], outs[1][k], equal_nan=True), k
+
+
 def test_spline_grid_form_matches_coefficient_form(ctx):
     """The spline's grid form (default for CSpline_Compute tables) and its coefficient form
     agree on expected values, derivatives, CRLB and fits."""
