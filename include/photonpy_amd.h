@@ -276,10 +276,10 @@ int Estimator_EstimateDevice(Estimator* e, int n, const float* d_sample, const f
 /* No counterpart in the reference.  For callers that cannot allocate (code under HIP graph
  * capture, serving loops, harnesses that forbid hipMalloc): Estimator_EstimateDevice with the
  * same arguments and results, allocating nothing.  Its scratch -- the fit kernel's 8 work
- * counters -- comes from a fixed array of 64 slots of 8 counters (2 KiB) in the library's
- * device memory, one slot per call in turn, reset by a kernel on `stream`; it does not grow
- * with n (n is bounded as for Estimator_EstimateDevice).  Calls that overlap on one GPU with
- * 64 or more others in flight share a slot and race.  A spline estimator's tables must already be on
+ * counters -- comes from a fixed array of 16 slots of 8 counters 256 bytes apart (32 KiB) in
+ * the library's device memory, one slot per call in turn, reset by a kernel on `stream`; it
+ * does not grow with n (n is bounded as for Estimator_EstimateDevice).  Calls that overlap on
+ * one GPU with 16 or more others in flight share a slot and race.  A spline estimator's tables must already be on
  * the device (any other fit or evaluation call on it uploads them); otherwise the call fails
  * with an error. */
 int Estimator_EstimateDeviceNoAlloc(Estimator* e, int n, const float* d_sample, const float* d_constants,

@@ -226,7 +226,7 @@ void run_fit(Estimator* e, int n, const float* d_sample, const float* d_constant
 // (each device holds its own copy until the process ends), so that the call itself
 // allocates nothing.  Each call takes the next of the kNoAllocSlots slots of
 // kWorkSegs counters and resets it with a kernel on its stream before the fit reads it.
-constexpr int kNoAllocSlots = 64;
+constexpr int kNoAllocSlots = 16;  // 16 x 8 counters 256 bytes apart: 32 KiB
 static __device__ __align__(256) int scratch[kNoAllocSlots * kCounterInts];
 
 // Device address of `scratch` on the current device, resolved once per device with

@@ -157,10 +157,10 @@ struct Estimator {
     }
 
     float* devShared = nullptr;  // device copy of the shared PSF parameters (Gaussian models)
-    // Work counters of the fit kernel: one slot of kWorkSegs counters per launch,
-    // cycling, so launches of this estimator on different streams do not share
-    // counters unless 64 are in flight.
-    static constexpr int kCounterSlots = 64;
+    // Work counters of the fit kernel: one slot of kWorkSegs counters (256 bytes
+    // apart) per launch, cycling, so launches of this estimator on different streams
+    // do not share counters unless 16 are in flight.
+    static constexpr int kCounterSlots = 16;
     int* devCounters = nullptr;
     unsigned counterSlot = 0;
 
