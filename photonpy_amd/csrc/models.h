@@ -930,7 +930,26 @@ struct SplineGridModelT : SplineCommon<BG> {
     // window of R + 4 grid samples per axis (see for_pixels) as (value, z-derivative)
     // pairs, and the per-column and per-row cell offsets (2R ints); an even count
     // keeps every group's pairs 8-byte aligned
-    static constexpr int lds_floats(int R, int) { return 2 * (R + 4) * (R + 4) + 2 * R; }
+    //
+    // LDS banks: a ds_read_b64 is served 32 lanes at a time from 64 4-byte banks, so 32
+    // lanes reading pixels p .. p + 31 of a window stored with row stride S are
+    // conflict-free when the (value, z-derivative) entries y S + x are distinct modulo
+    // 32, i.e. when S = R (mod 32).  The plain stride W = R + 4 breaks this at every row
+    // end (measured: 2 conflict cycles per ds_read_b64, 4-8 per ds_read2_b64, for R = 20).
+    // 32-lane groups therefore pair up: the two groups of a wavefront share one table
+    // whose rows (stride S = R mod 32, S >= 2W) hold both windows side by side, when that
+    // costs no more LDS than four wavefronts per SIMD allow (1280 floats per group);
+    // their cell offsets are kept as 16-bit values.  Smaller groups keep the plain
+    // layout.
+    static constexpr int pair_stride(int R) {
+        int S = 2 * (R + 4);
+        while (S % 32 != R % 32) S++;
+        return S;
+    }
+    static constexpr bool paired(int R, int G) { return G == 32 && (R + 4) * pair_stride(R) + R <= 1280; }
+    static constexpr int lds_floats(int R, int G) {
+        return paired(R, G) ? (R + 4) * pair_stride(R) + R : 2 * (R + 4) * (R + 4) + 2 * R;
+    }
 
     // Lagrange weights on the nodes -1, 0, 1, 2 at t, and their derivatives.
     PP_DEV static void lagrange(float t, float (&w)[4], float (&dw)[4]) {
@@ -1041,23 +1060,32 @@ struct SplineGridModelT : SplineCommon<BG> {
         const float* slab = grid + (size_t)F.k * Dy * Dx;
         const size_t zs = (size_t)Dy * Dx;
         // (value, z-derivative) weight pairs: one 8-byte LDS read per stencil entry,
-        // two entries per ds_read2_b64 at immediate offsets from one address
-        float2* T = reinterpret_cast<float2*>(lds);
-        // cell offsets of the pixel columns and rows from the window origin, rows times W
-        int* OX = reinterpret_cast<int*>(lds + 2 * NE);
-        int* OY = OX + R;
+        // two entries per ds_read2_b64 at immediate offsets from one address; row
+        // stride S (the paired layout: this group's window starts W entries into the
+        // shared rows when it is the wavefront's second group)
+        constexpr bool kPaired = paired(R, G);
+        constexpr int S = kPaired ? pair_stride(R) : W;
+        constexpr int PER = lds_floats(R, G);
+        const int half = kPaired ? (int)((threadIdx.x / G) & 1) : 0;
+        float* base = lds - half * PER;  // the pair's region (the group's own when unpaired)
+        float2* T = reinterpret_cast<float2*>(base) + half * W;
+        // cell offsets of the pixel columns and rows from the window origin, rows times S
+        using Off = std::conditional_t<kPaired, unsigned short, int>;
+        Off* OX = reinterpret_cast<Off*>(base + (kPaired ? 2 * (R + 4) * S : 2 * NE)) + half * 2 * R;
+        Off* OY = OX + R;
 #pragma unroll
         for (int u0 = 0; u0 < 2 * R; u0 += G) {
             const int u = u0 + g;
             if (u < R)
-                OX[u] = cell_x(F, u) - i0;
+                OX[u] = (Off)(cell_x(F, u) - i0);
             else if (u < 2 * R)
-                OY[u - R] = (cell_y(F, u - R) - j0) * W;
+                OY[u - R] = (Off)((cell_y(F, u - R) - j0) * S);
         }
         // z-contraction of the window: rows j0.., columns i0.. (clamped to the grid;
         // entries past the last cell are never read by a pixel).  Every read offset
-        // (cell - origin) * W + stencil stays below W * W.  Two entries per
-        // trip keep 8 loads in flight per lane without holding the whole window.
+        // (cell - origin) * S + stencil stays inside the group's W x W window.  Two
+        // entries per trip keep 8 loads in flight per lane without holding the whole
+        // window.
 #pragma unroll 2
         for (int u = 0; u < PE; u++) {
             const int e = g + G * u;
@@ -1066,8 +1094,8 @@ struct SplineGridModelT : SplineCommon<BG> {
                 const int gy = min(j0 + r, Dy - 1), gx = min(i0 + c, Dx - 1);
                 const float* src = slab + (size_t)gy * Dx + gx;
                 const float v0 = src[0], v1 = src[zs], v2 = src[2 * zs], v3 = src[3 * zs];
-                T[e] = make_float2(v0 * F.wz[0] + v1 * F.wz[1] + v2 * F.wz[2] + v3 * F.wz[3],
-                                   v0 * F.wdz[0] + v1 * F.wdz[1] + v2 * F.wdz[2] + v3 * F.wdz[3]);
+                T[r * S + c] = make_float2(v0 * F.wz[0] + v1 * F.wz[1] + v2 * F.wz[2] + v3 * F.wz[3],
+                                           v0 * F.wdz[0] + v1 * F.wdz[1] + v2 * F.wdz[2] + v3 * F.wdz[3]);
             }
         }
         // the group is inside one wavefront: its writes must have completed before any
@@ -1083,14 +1111,14 @@ struct SplineGridModelT : SplineCommon<BG> {
                 // p / R as a 20-bit multiply-shift, exact for p < 2048 and R <= 32
                 constexpr int kDivR = (1 << 20) / R + 1;
                 const int y = (p * kDivR) >> 20, x = p - y * R;
-                const int o = OY[y] + OX[x];
+                const int o = (int)OY[y] + (int)OX[x];
                 float f = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
 #pragma unroll
                 for (int n = 0; n < 4; n++) {
                     float a = 0.0f, ad = 0.0f, az = 0.0f;
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
-                        const float2 v = T[o + n * W + q];
+                        const float2 v = T[o + n * S + q];
                         const float e = v.x, ez = v.y;
                         a += F.wx[q] * e;
                         ad += F.wdx[q] * e;
