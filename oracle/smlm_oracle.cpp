@@ -60,7 +60,8 @@ void pool_for(int n, int nthreads, F fn) {
         for (int i = 0; i < n; i++) fn(i);
         return;
     }
-    const int chunk = 256;
+    // chunks small enough that every thread gets several (per-spot cost varies a lot)
+    const int chunk = std::max(1, std::min(256, n / (nthreads * 8)));
     std::atomic<int> next{0};
     auto worker = [&]() {
         for (;;) {
