@@ -368,7 +368,22 @@ struct GaussModel {
     // one-lane group shares nothing and keeps them in VGPRs.
     static constexpr int kColFloats = (V == kSigma || V == kAstig) ? 4 : 2;
     static constexpr bool cols_in_lds(int G) { return kRowFactored && G >= 2; }
-    static constexpr int lds_floats(int R, int G) { return cols_in_lds(G) ? kColFloats * R : 0; }
+    // The groups of a half-wave read the same column at once from their own tables;
+    // the table stride keeps those reads on disjoint banks (64 banks of 4 bytes, each
+    // read spanning kColFloats banks): the smallest stride >= kColFloats R whose
+    // multiples by 1 .. groups - 1 stay kColFloats away from 0 mod 64.
+    static constexpr int col_stride(int R, int G) {
+        const int groups = G >= 32 ? 1 : 32 / G;
+        for (int per = kColFloats * R;; per += kColFloats) {
+            bool ok = true;
+            for (int k = 1; k < groups; k++) {
+                const int d = (k * per) % 64;
+                if (d < kColFloats || 64 - d < kColFloats) ok = false;
+            }
+            if (ok) return per;
+        }
+    }
+    static constexpr int lds_floats(int R, int G) { return cols_in_lds(G) ? col_stride(R, G) : 0; }
 
     // Fit-kernel evaluation of lane g's pixels: fn(slot, mu, J) for every slot of
     // RowLayout<R, G> that lies inside the ROI.
