@@ -218,7 +218,7 @@ def line(metric_value, unit, steps, warmup, ms, workload, extra_config, roofline
             "cpu_baseline": cpu, "detail": detail}
 
 
-def bench_fit(name, steps, warmup, n=0, group=0, refill=0, cpu=True, e2e=True):
+def bench_fit(name, steps, warmup, n=0, group=0, refill=0, cpu=True, e2e=True, segs=0):
     """One single-model workload: `steps` timed fused launches after `warmup` untimed ones.
     Returns (bench line, CPU-baseline job or None); finish_cpu runs the job into the line."""
     import torch
@@ -235,6 +235,8 @@ def bench_fit(name, steps, warmup, n=0, group=0, refill=0, cpu=True, e2e=True):
             est.SetGroupSize(group)
         if refill:
             est.SetRefill(refill)
+        if segs:
+            est.SetWorkSegments(segs)
         th, smp = synthetic.make_samples(est, cfg["kind"], n, seed=1)
         torch.cuda.synchronize()
         for _ in range(warmup):
@@ -539,6 +541,7 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override the number of spots")
     ap.add_argument("--group", type=int, default=0, help="lane-group size override (0 = tuned/default)")
     ap.add_argument("--refill", type=int, default=0, help="refill batch override (0 = tuned/default)")
+    ap.add_argument("--segs", type=int, default=0, help="work-segment override: 1 or 8 (0 = tuned/default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-e2e", action="store_true", help="skip the PCIe-inclusive host-API leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -573,7 +576,7 @@ def main():
         return
     if args.config != "all":
         res, job = bench_fit(args.config, args.steps, args.warmup, args.n, args.group, args.refill, cpu=cpu,
-                             e2e=not args.no_e2e)
+                             e2e=not args.no_e2e, segs=args.segs)
         finish_cpu(res, job)
         print(json.dumps(res))
         return

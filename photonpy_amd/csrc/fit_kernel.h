@@ -121,22 +121,50 @@ PP_DEV auto pixel_bg(const float (&th)[M::KT], const float (&b)[NB], Fn& fn) {
 // loads with the non-temporal hint (a few groups refill at a time and each ROI
 // is read once); the CRLB pass, where every group loads at once, uses ordinary
 // loads so that the lines its lanes share stay in L2 between loads.
-template <int R, class L, int NS, bool CLAMP, bool NT>
+template <int R, class L, int NS, bool CLAMP, bool NT, bool FACTORED = false>
 PP_DEV void load_samples(const float* src, int g, float (&s)[NS], float& sx, float& sy, float& sum) {
     // The per-slot offsets and coordinates depend only on g; for a strided layout
     // the opaque copy keeps the compiler from hoisting all of them out of the fit
     // loop into VGPRs (an empty asm: it emits no instruction).
     if constexpr (L::kStrided) asm volatile("" : "+v"(g));
     sx = sy = sum = 0.0f;
+    if constexpr (L::kStrided || !FACTORED) {
 #pragma unroll
-    for (int k = 0; k < NS; k++) {
-        const int idx = L::index(g, k);
-        float v = 0.0f;
-        if (idx < R * R) v = NT ? __builtin_nontemporal_load(src + idx) : src[idx];
-        s[k] = (CLAMP && v < 1e-6f) ? 1e-6f : v;
-        sx += (float)L::x(g, k) * v;
-        sy += (float)L::y(g, k) * v;
-        sum += v;
+        for (int k = 0; k < NS; k++) {
+            const int idx = L::index(g, k);
+            float v = 0.0f;
+            if (idx < R * R) v = NT ? __builtin_nontemporal_load(src + idx) : src[idx];
+            s[k] = (CLAMP && v < 1e-6f) ? 1e-6f : v;
+            sx += (float)L::x(g, k) * v;
+            sy += (float)L::y(g, k) * v;
+            sum += v;
+        }
+    } else {
+        // row layouts under a register budget: sum x v = sum over regions xbase * (sum v)
+        // + sum xoff v (and y likewise), with compile-time offsets, so that no per-slot
+        // coordinates are held (hoisted out of the fit loop they were spilled)
+        float part[L::NREG];
+#pragma unroll
+        for (int r = 0; r < L::NREG; r++) part[r] = 0.0f;
+        float sxo = 0.0f, syo = 0.0f;
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            const int idx = L::index(g, k);
+            float v = 0.0f;
+            if (idx < R * R) v = NT ? __builtin_nontemporal_load(src + idx) : src[idx];
+            s[k] = (CLAMP && v < 1e-6f) ? 1e-6f : v;
+            part[L::region(k)] += v;
+            if (L::xoff(k) != 0) sxo += (float)L::xoff(k) * v;
+            if (L::yoff(k) != 0) syo += (float)L::yoff(k) * v;
+        }
+        sx = sxo;
+        sy = syo;
+#pragma unroll
+        for (int r = 0; r < L::NREG; r++) {
+            sx += (float)L::xbase(g, r) * part[r];
+            sy += (float)L::ybase(g, r) * part[r];
+            sum += part[r];
+        }
     }
 }
 
@@ -180,6 +208,18 @@ PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTrai
     }
 }
 
+// Minimum wavefronts per SIMD the compiler must fit a model's fit kernel into
+// (its register budget): M::min_waves(R, G) where the model names one, else 1
+// (no bound).
+template <class M, int R, int G, class = void>
+struct MinWaves {
+    static constexpr int value = 1;
+};
+template <class M, int R, int G>
+struct MinWaves<M, R, G, std::void_t<decltype(M::min_waves(R, G))>> {
+    static constexpr int value = M::min_waves(R, G);
+};
+
 // Fits spots first, first+stride, ... < a.n with the lane groups of the calling
 // wavefront, or, with work counters, its group's first spot g_first of segment seg
 // and then the spots the counters hand out; lds is the group's LDS region.  Every
@@ -213,7 +253,8 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
 
     auto start_spot = [&](int sp) {
         float sx, sy, sum;
-        load_samples<R, L, NS, true, true>(a.sample + (size_t)sp * T::S, g, s, sx, sy, sum);
+        load_samples<R, L, NS, true, true, (MinWaves<M, R, G>::value > 1)>(a.sample + (size_t)sp * T::S, g, s, sx,
+                                                                          sy, sum);
         if constexpr (M::kPixelBg) load_pixels<R, L, NS>(a.constants + (size_t)sp * T::S, g, pb);
 #pragma unroll
         for (int c = 0; c < M::NC; c++) th[K + c] = a.constants[(size_t)sp * M::NC + c];
@@ -335,16 +376,29 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
 #pragma unroll
             for (int i = 0; i < K; i++) acc[NA + i] += bf * J[i];
         };
+        // The lane's pixel coordinates, edge positions and table addresses depend only
+        // on g, and lambda is fixed.  Under a register budget, opaque copies made per
+        // iteration (empty asm, no instruction) keep the compiler from hoisting them out
+        // of the loop, where they were spilled to scratch and reloaded every iteration;
+        // recomputing them takes a few VALU instructions (11x11 free sigma: fit kernel
+        // 3.69 -> 3.56 ms, 120 -> 0 bytes of scratch).  Without a budget nothing spills
+        // and the hoisted values are cheaper (10x10 fixed sigma: 48.2 vs 49.8 us).
+        int gi = g;
+        float lam = a.lambda;
+        if constexpr (MinWaves<M, R, G>::value > 1) {
+            asm volatile("" : "+v"(gi));
+            asm volatile("" : "+s"(lam));
+        }
         if constexpr (row_factored<M>()) {
-            model.template accumulate_rows<R, G>(th, g, s, acc, lds);
+            model.template accumulate_rows<R, G>(th, gi, s, acc, lds);
         } else {
-            model.template for_pixels<R, G>(th, g, lds, pixel_bg<M>(th, pb, body));
+            model.template for_pixels<R, G>(th, gi, lds, pixel_bg<M>(th, pb, body));
         }
         group_sum_n<G>(acc);
 
         double step[K];
         bool stop;
-        if (!lm_step_f64<K>(acc, acc + NA, a.lambda, step)) {
+        if (!lm_step_f64<K>(acc, acc + NA, lam, step)) {
             stop = true;  // zero pivot: the reference breaks before updating
         } else {
             float last[K];
@@ -400,18 +454,6 @@ PP_DEV float* group_lds() {
 // Phase 1: fit.  Phase 2 (CRLB and chi^2 of the fitted spots) is a kernel of its
 // own, launched after it on the same stream, so that it sees every wavefront's
 // results and runs at the occupancy of its own, smaller register set.
-// Minimum wavefronts per SIMD the compiler must fit a model's fit kernel into
-// (its register budget): M::min_waves(R, G) where the model names one, else 1
-// (no bound).
-template <class M, int R, int G, class = void>
-struct MinWaves {
-    static constexpr int value = 1;
-};
-template <class M, int R, int G>
-struct MinWaves<M, R, G, std::void_t<decltype(M::min_waves(R, G))>> {
-    static constexpr int value = M::min_waves(R, G);
-};
-
 template <class M, int R, int G>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MinWaves<M, R, G>::value)))
 fit_kernel(M model, FitArgs a) {

@@ -43,14 +43,24 @@ namespace ppa {
 // spline models, whose per-row and per-column terms are shared along rows).
 // FlatLayout: lane g owns pixels g, g+G, g+2G, ... (models that read their
 // per-pixel terms from LDS, where any pixel order costs the same).
+//
+// The row layouts also split each slot's coordinates as x = xbase(g, region(s)) +
+// xoff(s) (and y likewise), region and offsets known at compile time, so a sum over
+// the slots such as the centre of mass needs only a few per-lane values.
 template <int R, int G>
 struct RowLayout {
     static constexpr int RL = (R + G - 1) / G;
     static constexpr int NS = RL * R;
     static constexpr bool kStrided = false;
+    static constexpr int NREG = 1;
     PP_DEV static int index(int g, int s) { return g * RL * R + s; }
     PP_DEV static int x(int, int s) { return s % R; }
     PP_DEV static int y(int g, int s) { return g * RL + s / R; }
+    static constexpr int region(int) { return 0; }
+    static constexpr int xoff(int s) { return s % R; }
+    static constexpr int yoff(int s) { return s / R; }
+    PP_DEV static int xbase(int, int) { return 0; }
+    PP_DEV static int ybase(int g, int) { return g * RL; }
 };
 
 // SplitRowLayout (row-factored models where the rows do not divide evenly): the
@@ -73,6 +83,13 @@ struct SplitRowLayout {
     }
     PP_DEV static int x(int g, int s) { return s < H * R ? s % R : (g & 1) * M + (s - H * R); }
     PP_DEV static int index(int g, int s) { return x(g, s) < R ? y(g, s) * R + x(g, s) : R * R; }
+    // region 0: the whole rows; region 1: the half of the middle row
+    static constexpr int NREG = 2;
+    static constexpr int region(int s) { return s < H * R ? 0 : 1; }
+    static constexpr int xoff(int s) { return s < H * R ? s % R : s - H * R; }
+    static constexpr int yoff(int s) { return s < H * R ? s / R : 0; }
+    PP_DEV static int xbase(int g, int r) { return r == 0 ? 0 : (g & 1) * M; }
+    PP_DEV static int ybase(int g, int r) { return (g >> 1) * B + (r == 0 ? (g & 1) * (H + 1) : H); }
 };
 
 template <int R, int G>
@@ -432,9 +449,9 @@ struct GaussModel {
     // evaluates columns [g C, g C + C), C = ceil(R / G), from its C + 1 edges.  Called
     // by every lane of an aligned group of G lanes; the group reads the table after it.
     template <int R, int G>
-    PP_DEV void columns_lds(const float (&t)[KT], const AxisShape& SX, float* lds) const {
+    PP_DEV void columns_lds(const float (&t)[KT], const AxisShape& SX, int g, float* lds) const {
         constexpr int C = (R + G - 1) / G;
-        const int x0 = (threadIdx.x & (G - 1)) * C;
+        const int x0 = g * C;
         // the previous iteration's reads of the table complete before it is overwritten
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
@@ -614,7 +631,7 @@ struct GaussModel {
         constexpr bool kLds = cols_in_lds(G);
         AxisTerms col[kLds ? 1 : R];
         if constexpr (kLds)
-            columns_lds<R, G>(t, SX, lds);
+            columns_lds<R, G>(t, SX, g, lds);
         else
             columns<R, G>(t, SX, col);
         // column x's terms for the j-th pixel of a row

@@ -41,8 +41,12 @@ def check_fit_parity(kind, om, smp, p_gpu, it_gpu, iters, lam, limits, tag="", m
     one FP32 ulp and so rarely fire, its last step was below 1e-4 px, 0.1 nm and 1e-5 I),
     off the parameter bounds, finite, and the oracle's FP64 fit agrees with its FP32 fit to
     the tolerance.  A mismatch is allowed only where the GPU fit went to a bound or its own
-    stop fired there (a low-SNR LM path that FP32 rounding sends elsewhere), for at most 1
-    witness in 1000.  The witness share is asserted as a fraction (min_converged) or, for
+    stop fired there (a low-SNR LM path that FP32 rounding sends elsewhere), or where the
+    GPU fit ran out of iterations on a spot whose oracle path was clamped at a bound of x,
+    y, I, sigma or z after its first step (a clamped step drops the Newton direction, and
+    the path after it is decided by rounding: on one such 7x7 free-sigma spot, starting the
+    oracle's own FP32 fit 3e-7 further in x sends it to the bounds), for at most 1 witness
+    in 1000.  The witness share is asserted as a fraction (min_converged) or, for
     workloads where the reference's iteration budget leaves most spots unconverged (large
     ROIs with 30 iterations), as a count (min_witnesses)."""
     lo, hi = limits
@@ -62,7 +66,14 @@ def check_fit_parity(kind, om, smp, p_gpu, it_gpu, iters, lam, limits, tag="", m
     gpu_bound = on_bounds(p_gpu, lo, hi) | ~np.all(np.isfinite(p_gpu), axis=1)
     ok = compare(kind, p_gpu[conv], p_ref[conv])
     bad = np.where(~ok)[0]
-    other = (gpu_bound | (it_gpu <= iters))[conv][bad]
+    # oracle paths clamped at a bound of a non-background parameter after the first step
+    ib = 4 if zcol(kind) else 3
+    keep = [k for k in range(p_ref.shape[1]) if k != ib]
+    steps = np.arange(tr_ref.shape[1])
+    live = (steps[None, :] >= 1) & (steps[None, :] < np.minimum(it_ref, iters)[:, None])
+    hit = (tr_ref <= lo[None, None, :] + 1e-6) | (tr_ref >= hi[None, None, :] - 1e-6)
+    clamped = np.any(hit[:, :, keep] & live[:, :, None], axis=(1, 2))
+    other = (gpu_bound | (it_gpu <= iters) | ((it_gpu > iters) & clamped))[conv][bad]
     assert other.all(), f"{tag}: converged spots differ, e.g. gpu={p_gpu[conv][bad[:3]]} ref={p_ref[conv][bad[:3]]}"
     assert len(bad) <= max(1, 1e-3 * conv.sum()), \
         f"{tag}: {len(bad)} of {conv.sum()} witnesses differ, e.g. gpu={p_gpu[conv][bad[:3]]} ref={p_ref[conv][bad[:3]]}"
