@@ -114,6 +114,12 @@ PP_DEV auto pixel_bg(const float (&th)[M::KT], const float (&b)[NB], Fn& fn) {
     };
 }
 
+// The Levenberg-Marquardt terms' sample clamp `if (x < 1e-6) x = 1e-6` (a NaN stays
+// NaN, Estimation.h:114) as one NaN-propagating maximum (v_maximum3_f32).  Written
+// as a compare and select, the compiler evaluated all of a lane's compares before
+// the selects and parked the masks in VGPR lanes (122 SGPRs for 61 samples).
+PP_DEV float clamp_sample(float v) { return __builtin_elementwise_maximum(v, 1e-6f); }
+
 // Loads lane g's samples of a spot (layout L) and returns the lane's
 // centre-of-mass partial sums.  With CLAMP the stored samples are max(x, 1e-6)
 // as the Levenberg-Marquardt terms use them (`if (x < 1e-6) x = 1e-6`, which
@@ -134,7 +140,7 @@ PP_DEV void load_samples(const float* src, int g, float (&s)[NS], float& sx, flo
             const int idx = L::index(g, k);
             float v = 0.0f;
             if (idx < R * R) v = NT ? __builtin_nontemporal_load(src + idx) : src[idx];
-            s[k] = (CLAMP && v < 1e-6f) ? 1e-6f : v;
+            s[k] = CLAMP ? clamp_sample(v) : v;
             sx += (float)L::x(g, k) * v;
             sy += (float)L::y(g, k) * v;
             sum += v;
@@ -152,7 +158,7 @@ PP_DEV void load_samples(const float* src, int g, float (&s)[NS], float& sx, flo
             const int idx = L::index(g, k);
             float v = 0.0f;
             if (idx < R * R) v = NT ? __builtin_nontemporal_load(src + idx) : src[idx];
-            s[k] = (CLAMP && v < 1e-6f) ? 1e-6f : v;
+            s[k] = CLAMP ? clamp_sample(v) : v;
             part[L::region(k)] += v;
             if (L::xoff(k) != 0) sxo += (float)L::xoff(k) * v;
             if (L::yoff(k) != 0) syo += (float)L::yoff(k) * v;
