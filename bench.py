@@ -54,27 +54,42 @@ ADJ_DIRECT_ATOMICS_PER_ROI = 400 * 64
 #   CRLB kernel           S (model + K^2 + 2K + 3) + (W+H) rowcol + 8K^3/3
 MODEL_TERMS = {"fixed": 5, "sigma": 8, "astig": 8, "spline": 422, "splinegrid": 136}
 ROWCOL_TERMS = {"fixed": 11, "sigma": 15, "astig": 16, "spline": 0, "splinegrid": 0}
-# The grid-form spline (tables made by CSpline_Compute) evaluates a pixel from 4x4 entries of
-# two z-contracted window tables: 64 FMA + 8 (finish); per spot-iteration the window costs
-# (R+3)^2 x 8 FMA of z-contraction, counted per row below as (R+3)^2 * 16 / (2R).
+# The grid-form spline (tables made by CSpline_Compute) contracts, per spot-iteration, the
+# window's four z-slices into a (value, z-derivative) table: (R+3)^2 x 8 FMA (the window
+# term below).  Each pixel then needs its 4 x 4 block of that table.  Flat layout: 64 FMA
+# per pixel + 8 (finish) = 136 FLOP.  Strip layout of length L (each lane walks vertical
+# strips; a window row's x-contraction, 12 FMA, serves the four pixels below it): per
+# pixel 16 FMA of y-contraction + 8 (finish), plus, per column piece of l pixels, l + 3
+# row contractions, i.e. 24 (R^2 + 3 R P) FLOP per spot with P = ceil(R / L) pieces per
+# column.
 
 
-def fit_flops_per_loc(kind, R, executed_iters):
+def spline_pixel_terms(R, strip):
+    """Model FLOP per pixel of the grid-form spline at strip length `strip` (0: flat)."""
+    if not strip:
+        return MODEL_TERMS["splinegrid"]
+    P = -(-R // strip)
+    return 40 + 24 * (1 + 3 * P / R)
+
+
+def fit_flops_per_loc(kind, R, executed_iters, strip=0):
     """Fit kernel: centre of mass + executed iterations."""
     K = 4 if kind == "fixed" else 5
     S = R * R
     window = (R + 3) ** 2 * 16 if kind == "splinegrid" else 0
-    per_iter = S * (K * K + 4 * K + 4 + MODEL_TERMS[kind]) + 2 * R * ROWCOL_TERMS[kind] + window + \
+    model = spline_pixel_terms(R, strip) if kind == "splinegrid" else MODEL_TERMS[kind]
+    per_iter = S * (K * K + 4 * K + 4 + model) + 2 * R * ROWCOL_TERMS[kind] + window + \
         (4 * K * K + 4 * K + 2 * K ** 3 / 3)
     return per_iter * executed_iters + 3 * S
 
 
-def crlb_flops_per_loc(kind, R):
+def crlb_flops_per_loc(kind, R, strip=0):
     """CRLB kernel: Fisher matrix + chi^2 + FP64 LU inversion."""
     K = 4 if kind == "fixed" else 5
     S = R * R
     window = (R + 3) ** 2 * 16 if kind == "splinegrid" else 0
-    return S * (MODEL_TERMS[kind] + K * K + 2 * K + 3) + 2 * R * ROWCOL_TERMS[kind] + window + 8 * K ** 3 / 3
+    model = spline_pixel_terms(R, strip) if kind == "splinegrid" else MODEL_TERMS[kind]
+    return S * (model + K * K + 2 * K + 3) + 2 * R * ROWCOL_TERMS[kind] + window + 8 * K ** 3 / 3
 
 
 def hbm_traffic(workload):
@@ -265,10 +280,11 @@ def bench_fit(name, steps, warmup, n=0, group=0, refill=0, cpu=True, e2e=True, s
         p = out["params"].cpu().numpy()
         executed = np.minimum(it, cfg["iters"]).astype(np.float64)
         fkind = "splinegrid" if est.spline_form() == "grid" else cfg["kind"]
-        per0 = fit_flops_per_loc(fkind, cfg["R"], 0)
-        per1 = fit_flops_per_loc(fkind, cfg["R"], 1) - per0
+        strip = est.spline_strip_length() or 0
+        per0 = fit_flops_per_loc(fkind, cfg["R"], 0, strip)
+        per1 = fit_flops_per_loc(fkind, cfg["R"], 1, strip) - per0
         fit_flops = n * per0 + per1 * executed.sum()
-        crlb_flops = n * crlb_flops_per_loc(fkind, cfg["R"])
+        crlb_flops = n * crlb_flops_per_loc(fkind, cfg["R"], strip)
         achieved = fit_flops / (fit_ms * 1e-3) / 1e12
         traffic = hbm_traffic(name) if n == cfg["n"] else None
         e2e_rate = host_api_rate(est, smp[:min(n, 200_000)].cpu().numpy()) if e2e else None
@@ -294,7 +310,8 @@ def bench_fit(name, steps, warmup, n=0, group=0, refill=0, cpu=True, e2e=True, s
         "finite_fraction": float(np.mean(np.all(np.isfinite(p), axis=1))),
     }
     extra = {"roi": cfg["R"], "spots": n, "iterations": cfg["iters"], "lanes_per_spot": group,
-             **({"spline_form": form} if form else {})}
+             **({"spline_form": form} if form else {}),
+             **({"spline_strip_length": strip} if form == "grid" else {})}
     return line(n / (ms_per_step * 1e-3), "localizations/s", steps, warmup, ms_per_step, cfg["workload"], extra,
                 roof, None, detail), job
 
@@ -369,9 +386,10 @@ def bench_mixed(steps, warmup, n=0):
             for b, o in zip(bins, outs):
                 it = np.minimum(o["iterations"].cpu().numpy(), b["iters"]).astype(np.float64)
                 fk = "splinegrid" if b["est"].spline_form() == "grid" else b["kind"]
-                per0 = fit_flops_per_loc(fk, b["R"], 0)
-                per1 = fit_flops_per_loc(fk, b["R"], 1) - per0
-                flops += b["n"] * (per0 + crlb_flops_per_loc(fk, b["R"])) + per1 * it.sum()
+                strip = b["est"].spline_strip_length() or 0
+                per0 = fit_flops_per_loc(fk, b["R"], 0, strip)
+                per1 = fit_flops_per_loc(fk, b["R"], 1, strip) - per0
+                flops += b["n"] * (per0 + crlb_flops_per_loc(fk, b["R"], strip)) + per1 * it.sum()
             streams = mb.num_streams()
             groups = [e.GetGroupSize() for e in ests]
             forms = sorted({b["est"].spline_form() for b in bins if b["kind"] == "spline"})

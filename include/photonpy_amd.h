@@ -263,6 +263,12 @@ int Estimator_GetWorkSegments(Estimator* e);
  * a spline estimator.  Results of the two forms agree to FP32 rounding of the model. */
 int Estimator_SetSplineMode(Estimator* e, int mode);
 int Estimator_SplineForm(Estimator* e);
+/* Pixel layout of a grid-form spline fit (no reference counterpart; for FLOP accounting):
+ * the strip length L of the strip layout at the estimator's ROI size and lane-group size
+ * (each lane walks vertical strips of up to L pixels, contracting each window row once per
+ * strip), 0 for the flat layout (every pixel contracts its own 4x4 block), -1 when the
+ * estimator is not a grid-form spline. */
+int Estimator_SplineStripLength(Estimator* e);
 
 int Estimator_SetPhaseTiming(Estimator* e, int enable);
 int Estimator_GetPhaseTimes(Estimator* e, float* fitMs, float* crlbMs);

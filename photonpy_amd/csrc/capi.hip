@@ -656,6 +656,13 @@ int Estimator_SplineForm(Estimator* e) {
     return e->desc.useGrid ? 1 : 0;
 }
 
+int Estimator_SplineStripLength(Estimator* e) {
+    e = model_of(e);
+    if (!e || !is_spline(e->desc.kind) || !e->desc.useGrid) return -1;
+    if (e->roisize < kMinFitR || e->roisize > kMaxFitR) return 0;
+    return SplineGridModel::strip_len(e->roisize, e->G());
+}
+
 int Estimator_SetPhaseTiming(Estimator* e, int enable) {
     e = model_of(e);
     if (!e) return 1;

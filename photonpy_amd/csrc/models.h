@@ -101,6 +101,29 @@ struct FlatLayout {
     PP_DEV static int y(int g, int s) { return index(g, s) / R; }
 };
 
+// StripLayout (grid-form spline): lane g owns M vertical strips of up to LS pixels.
+// Each of the R columns is cut into P = ceil(R / LS) pieces of LS rows (the last one
+// shorter); piece p = g + G j (j < M) of the R P pieces is column p % R, rows
+// [(p / R) LS, (p / R) LS + LS).  Slot s = j LS + i is row i of lane g's strip j;
+// slots past the ROI's last row or past the last piece are padding (index() puts
+// them past the ROI).
+template <int R, int G, int LS>
+struct StripLayout {
+    static constexpr int P = (R + LS - 1) / LS;  // pieces per column
+    static constexpr int NP = R * P;             // pieces
+    static constexpr int M = (NP + G - 1) / G;   // strips per lane
+    static constexpr int NS = M * LS;
+    static constexpr bool kStrided = true;
+    PP_DEV static int piece(int g, int s) { return g + G * (s / LS); }
+    PP_DEV static int x(int g, int s) { return piece(g, s) % R; }
+    PP_DEV static int y(int g, int s) { return (piece(g, s) / R) * LS + s % LS; }
+    PP_DEV static int index(int g, int s) {
+        const int p = piece(g, s);
+        const int yy = (p / R) * LS + s % LS;
+        return (p < NP && yy < R) ? yy * R + p % R : R * R;
+    }
+};
+
 enum GaussVariant { kFixed = 0, kSigma = 1, kAstig = 2, kConstSigma = 3, kTiltedBg = 4, kSigmaXY = 5, kTiltedBgSigmaXY = 6 };
 
 constexpr float kPiF = 3.141592653589793f;
@@ -950,14 +973,46 @@ struct SplineGridModelT : SplineCommon<BG> {
     using Base::iI;
     const float* grid;
 
+    // Strip length of the strip layout for R x R spots at G lanes, or 0 for the flat
+    // layout.  A pixel's value and derivatives contract a 4 x 4 block of the window
+    // table: 64 FMAs.  Its x-contraction of one window row (12 FMAs for value,
+    // x-derivative and z-derivative) is the same for the four pixels of a column that
+    // use that row, so a lane walking down a strip of L pixels contracts L + 3 rows and
+    // then 16 FMAs per pixel.  Model cost per lane (VALU instructions): strips
+    // M (12 (L + 3) + 8) + NS (16 + 35), flat NS (64 + 35), 35 for a pixel's
+    // normal-equation terms; strips are taken where they cost under 0.95 of flat and
+    // the last piece's rows stay inside the R + 4 window (P L <= R + 1).
+    static constexpr int strip_len(int R, int G) {
+        const int flat = (R * R + G - 1) / G * (64 + 35);
+        int best = 0, cost = flat * 95 / 100;
+        for (int L = 3; L <= R; L++) {
+            const int P = (R + L - 1) / L;
+            if (P * L > R + 1) continue;
+            const int M = (R * P + G - 1) / G;
+            const int c = M * (12 * (L + 3) + 8) + M * L * (16 + 35);
+            if (c < cost) { cost = c; best = L; }
+        }
+        return best;
+    }
     template <int R, int G>
-    using Layout = FlatLayout<R, G>;
-    // Five wavefronts per SIMD (<= 96 VGPRs) hide the L2 latency of the window
-    // loads; that register budget holds the flat-background form with groups of
-    // >= 8 lanes and <= 13 pixels per lane without spilling.  Other sizes and the
-    // tilted / image backgrounds keep the compiler's own choice.
+    using Layout = std::conditional_t<strip_len(R, G) != 0, StripLayout<R, G, strip_len(R, G) ? strip_len(R, G) : 1>,
+                                      FlatLayout<R, G>>;
+    // Four or five wavefronts per SIMD hide the L2 latency of the window loads: the
+    // flat-background form with groups of >= 8 lanes and <= 14 pixels per lane gets the
+    // register budget of as many wavefronts as its LDS tables allow (at most five, 96
+    // VGPRs, for the flat layout; four, 128 VGPRs, for the strip layout, whose four rows
+    // in flight spill under five, and where the tables of five would not fit in the
+    // CU's 160 KiB).  Other sizes and the tilted / image backgrounds keep the
+    // compiler's own choice.
+    static constexpr int slots(int R, int G) {
+        const int L = strip_len(R, G);
+        return L ? (R * ((R + L - 1) / L) + G - 1) / G * L : (R * R + G - 1) / G;
+    }
+    static constexpr int lds_waves(int R, int G) { return 40960 / (4 * lds_floats(R, G) * (64 / G)); }
     static constexpr int min_waves(int R, int G) {
-        return (BG == kSplineFlat && G >= 8 && (R * R + G - 1) / G <= 13) ? 5 : 1;
+        const int cap = strip_len(R, G) ? 4 : 5;
+        const int w = lds_waves(R, G) < cap ? lds_waves(R, G) : cap;
+        return (BG == kSplineFlat && G >= 8 && slots(R, G) <= 14 && w >= 4) ? w : 1;
     }
     // window of R + 4 grid samples per axis (see for_pixels) as (value, z-derivative)
     // pairs, and the per-column and per-row cell offsets (2R ints); an even count
@@ -1069,7 +1124,60 @@ struct SplineGridModelT : SplineCommon<BG> {
         finish(t, f, fx, fy, fz, x, y, R, mu, J);
     }
 
-    // Fit-kernel evaluation of lane g's pixels (FlatLayout<R, G>), called by every
+    // Lane g's pixels of the strip layout (StripLayout<R, G, LS>) from the group's
+    // window table T (row stride S) on a spot whose pixel rows take consecutive window
+    // rows: per strip, the x-contractions (value, x-derivative, z-derivative weights)
+    // of its first three rows, then per pixel one more row and the 16-FMA y-contraction.
+    // The four rows in flight rotate through fixed registers (the loops unroll).
+    template <int R, int G, int LS, int S, class Off, class Fn>
+    PP_DEV void strip_pixels(const float (&t)[K], const Frame& F, const float2* T, const Off* OX, Fn& fn,
+                             int g) const {
+        using SL = StripLayout<R, G, LS>;
+#pragma unroll
+        for (int j = 0; j < SL::M; j++) {
+            int p = g + G * j;
+            const bool live = SL::NP % G == 0 || SL::M * G == SL::NP || p < SL::NP;
+            if (!live) p = SL::NP - 1;  // padding lanes walk a real strip and use nothing
+            const int c = p % R, y0 = (p / R) * LS;
+            const float2* B = T + y0 * S + (int)OX[c];
+            float a[4], ad[4], az[4];
+            auto row = [&](int r) {
+                const float2* e = B + r * S;
+                const float2 e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3];
+                const int k = r & 3;
+                a[k] = F.wx[0] * e0.x + F.wx[1] * e1.x + F.wx[2] * e2.x + F.wx[3] * e3.x;
+                ad[k] = F.wdx[0] * e0.x + F.wdx[1] * e1.x + F.wdx[2] * e2.x + F.wdx[3] * e3.x;
+                az[k] = F.wx[0] * e0.y + F.wx[1] * e1.y + F.wx[2] * e2.y + F.wx[3] * e3.y;
+            };
+            row(0);
+            row(1);
+            row(2);
+#pragma unroll
+            for (int i = 0; i < LS; i++) {
+                row(i + 3);
+                const int y = y0 + i;
+                if (live && (SL::P * LS == R || y < R)) {
+                    float f = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll
+                    for (int n = 0; n < 4; n++) {
+                        const int k = (i + n) & 3;
+                        f += F.wy[n] * a[k];
+                        fx += F.wy[n] * ad[k];
+                        fy += F.wdy[n] * a[k];
+                        fz += F.wy[n] * az[k];
+                    }
+                    float mu, J[K];
+                    finish(t, f, fx, fy, fz, c, y, R, mu, J);
+                    fn(j * LS + i, mu, J);
+                }
+                // one pixel at a time (the scheduler would otherwise hoist later rows'
+                // LDS reads and run out of VGPRs)
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+
+    // Fit-kernel evaluation of lane g's pixels (Layout<R, G>), called by every
     // lane of the group with the group's LDS region (lds_floats(R, G) floats).
     template <int R, int G, class Fn>
     PP_DEV void for_pixels(const float (&t)[K], int g, float* lds, Fn&& fn) const {
@@ -1080,7 +1188,8 @@ struct SplineGridModelT : SplineCommon<BG> {
         constexpr int W = R + 4;
         constexpr int NE = W * W;         // window entries
         constexpr int PE = (NE + G - 1) / G;
-        using L = FlatLayout<R, G>;
+        using L = Layout<R, G>;
+        constexpr int LS = strip_len(R, G);
         // Pixel coordinates depend only on g and the slot; without this opaque copy
         // the compiler hoists all of them out of the iteration loop and keeps them
         // in VGPRs for the whole fit.  (An empty asm: it emits no instruction.)
@@ -1130,12 +1239,38 @@ struct SplineGridModelT : SplineCommon<BG> {
                                            v0 * F.wdz[0] + v1 * F.wdz[1] + v2 * F.wdz[2] + v3 * F.wdz[3]);
             }
         }
+        // Strip layout: the strips walk the window one row per pixel, which holds when
+        // the pixel rows' cells are consecutive (cell_y(y) = j0 + y for every row y; not
+        // where the grid edge clamps them or FP32 rounding skips a cell).  A group with
+        // any other row pattern takes the per-pixel path below.
+        bool strips = false;
+        if constexpr (LS != 0) {
+            bool ok = true;
+#pragma unroll
+            for (int u0 = 0; u0 < R; u0 += G) {
+                const int u = u0 + g;
+                if (u < R && cell_y(F, u) - j0 != u) ok = false;
+            }
+            const unsigned long long bad = __ballot(!ok);
+            const int first = (int)(threadIdx.x & (kWave - 1)) & ~(G - 1);
+            const unsigned long long mine = G == 64 ? ~0ull : (((1ull << G) - 1ull) << first);
+            strips = (bad & mine) == 0;
+        }
         // the group is inside one wavefront: its writes must have completed before any
         // lane reads the tables (workgroup-scope fences wait for the wave's LDS
         // operations; no workgroup barrier is needed)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if constexpr (LS != 0) {
+            if (strips) {
+                strip_pixels<R, G, LS, S>(t, F, T, OX, fn, g);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                return;
+            }
+        }
 #pragma unroll
         for (int s = 0; s < L::NS; s++) {
             const int p = L::index(g, s);

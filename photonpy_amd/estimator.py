@@ -372,6 +372,12 @@ class Estimator:
         f = self._lib.Estimator_SplineForm(self.inst)
         return None if f < 0 else ("grid" if f == 1 else "coefficients")
 
+    def spline_strip_length(self):
+        """Grid-form spline fits: the strip length of the pixel layout at the current
+        group size (0: flat layout), None for other estimators."""
+        L = self._lib.Estimator_SplineStripLength(self.inst)
+        return None if L < 0 else L
+
     def set_spline_mode(self, mode):
         """'auto' (the grid form when the table is a CSpline_Compute interpolant) or
         'coefficients'."""

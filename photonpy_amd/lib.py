@@ -138,6 +138,7 @@ def _bind(lib):
         "Estimator_SetPhaseTiming": ([V, I], ct.c_int),
         "Estimator_SetSplineMode": ([V, I], ct.c_int),
         "Estimator_SplineForm": ([V], ct.c_int),
+        "Estimator_SplineStripLength": ([V], ct.c_int),
         "Estimator_GetPhaseTimes": ([V, ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)], ct.c_int),
         "Estimator_ComputeExpectedValue": ([V, I, FloatArray, FloatArray, IntArray, FloatArray], None),
         "Estimator_Estimate": ([V, I, FloatArray, FloatArray, IntArray, NullableFloatArrayType, FloatArray,
