@@ -171,7 +171,7 @@ def cpu_job(cfg, smp, est):
     return lambda: cpu_baseline(cfg, host_rois, om)
 
 
-def cpu_baseline(cfg, smp_host, om, run_s=0.3, repeats=5):
+def cpu_baseline(cfg, smp_host, om, run_s=2.0, repeats=5):
     """The oracle (FP32 restatement of the reference estimator) on the host: Estimate + CRLB
     + chi^2 over the first m ROIs of the same batch (the batch repeated when it is smaller),
     m sized from a warm run so that one run takes about run_s seconds; one warm-up run,
