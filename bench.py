@@ -38,9 +38,12 @@ PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 vector peak
 # bound is the FP32 VALU roofline, labelled "valu".
 BOUND = "valu"
 COMPUTE_NOTE = "FP32 VALU roofline (no MFMA is issued); packed FP32 disabled"
-# Algorithmic FLOP per ROI of the spline adjoint's backward pass (per pixel: the spline
-# value and derivatives, the NLL gradient terms and the 64 coefficient-gradient terms).
-ADJ_FLOP_PER_ROI = 276_400
+# Algorithmic FLOP per ROI of the spline adjoint's backward pass, per pixel of each spot:
+# the spline value and x/y/z derivatives from the cell's 64 coefficients (139 FMA), mu, the
+# NLL weight and the 5 parameter-gradient terms (~22), and the spot's share of its cell's
+# coefficient gradient, sum_s W_s B_s[q] over the 64 coefficients (64 FMA): about 435 FLOP
+# per pixel, 174,000 per 20x20 ROI.
+ADJ_FLOP_PER_ROI = 174_000
 # ROI gradient evaluations per 65,536-ROI step: 20x20 px x 64 coefficients each.
 ADJ_DIRECT_ATOMICS_PER_ROI = 400 * 64
 
@@ -467,8 +470,8 @@ def bench_queue(steps, warmup, n=0):
 
 
 def spline_adjoint_rounds(th, R, shape, zmin, zmax):
-    """Rounds of the tiled coefficient gradient for spot parameters th: spots share a round
-    when they share the window (z-slab, first pixel's cell row and column), FP32 as on the GPU."""
+    """Rounds of the spline adjoint for spot parameters th: up to 64 spots of one window
+    (z-slab, first pixel's cell row and column, FP32 as on the GPU) per round."""
     Nz, Ny, Nx = shape
     f32 = np.float32
     tx = (th[:, 0].astype(f32) - f32(Nx * 0.5)).astype(f32)
@@ -479,8 +482,7 @@ def spline_adjoint_rounds(th, R, shape, zmin, zmax):
     i0 = np.clip(np.trunc((f32(0) - tx).astype(f32)).astype(np.int64), 0, Nx - 1)
     j0 = np.clip(np.trunc((f32(0) - ty).astype(f32)).astype(np.int64), 0, Ny - 1)
     _, cnt = np.unique((k * Ny + j0) * Nx + i0, return_counts=True)
-    per_round = 32 if (R + 1) ** 2 * 32 * 4 + 32 * 64 * 4 <= 65536 else 16
-    return int(np.sum((cnt + per_round - 1) // per_round)), len(cnt)
+    return int(np.sum((cnt + 63) // 64)), len(cnt)
 
 
 def bench_adjoint(steps, warmup, n=0):
@@ -531,8 +533,8 @@ def bench_adjoint(steps, warmup, n=0):
     torch.cuda.empty_cache()
     ms = wall * 1e3 / steps
     achieved = n * ADJ_FLOP_PER_ROI / (bwd_ms * 1e-3) / 1e12
-    atomics_tiled = rounds * (R + 1) ** 2 * 64
-    roof = {"kernel": "backward pass (window sort + spline_adjoint_tiled_kernel)", "bound": BOUND,
+    atomics_tiled = rounds * R * R * 64  # at most one 64-lane atomic per pixel cell and round
+    roof = {"kernel": "backward pass (window sort + spline_adjoint_round_kernel<true>)", "bound": BOUND,
             "compute": COMPUTE_NOTE, "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32, "traffic": None}
     detail = {"forward_ms": fwd_ms, "backward_ms": bwd_ms, "backward_algorithmic_flop": n * ADJ_FLOP_PER_ROI,

@@ -20,8 +20,9 @@
 //
 // Mapping: one wavefront per spot, lanes stride over pixels (any roisize).  Scalar
 // shared gradients are reduced per block in LDS and added to HBM once per block.
-// The spline coefficient gradient of roisize <= 28 is formed by window rounds
-// (spline_adjoint_tiled_kernel, below); larger ROIs scatter it pixel by pixel.
+// The spline NLL and adjoint of roisize <= 28 run by window rounds, one spot per lane
+// (spline_adjoint_round_kernel, below); larger ROIs take the one-wavefront-per-spot
+// form and scatter the coefficient gradient pixel by pixel.
 #include "dispatch.h"
 
 namespace ppa {
@@ -276,26 +277,31 @@ __global__ void __launch_bounds__(256) spline_adjoint_kernel(SplineModel M, AdjA
 }
 
 // ---------------------------------------------------------------------------
-// Coefficient gradient by window rounds (roisize <= kMaxTiledR).
+// Spline NLL and adjoint by window rounds (roisize <= kMaxTiledR).
 //
-// Spot s adds  W_p z_s^m y_s^n x_s^o  to coefficient (m, n, o) of the cell of each of
-// its pixels p, with W_p = g (1 - x/mu) I (zero where the spline value is clamped) and
-// (x_s, y_s, z_s) the spot's sub-voxel position.  All pixels of a spot share that
-// position, and their cells lie in the (R+1) x (R+1) window of z-slab k starting at the
-// cell of pixel (0, 0).  So the gradient of a window cell c is a sum over the spots
-// with that window of W_s[c] B_s[q] (B_s = the spot's 64 basis values): an outer
-// product.  Spots are sorted by window (k, j0, i0) and cut into tiles of up to
-// kTileSpots spots of one window.  A workgroup of 16 wavefronts takes a tile in rounds
-// of 16 x SPW spots (each wavefront evaluates SPW spots, its lanes over the pixels),
-// scatters their W_s[c] into LDS, forms the round's (R+1)^2 x 64 gradient entries
-// from LDS (thread = one coefficient of cells v, v + 16, ...) and adds each to HBM
-// with one atomic: (R+1)^2 x 64 atomics per round of up to 16 SPW spots instead of
-// R^2 x 64 per spot.  Pixels whose cell falls outside the window (none, by the bound
-// above; kept as a guard) add straight to HBM as the direct kernel does.
+// A spot's pixels all share its sub-voxel position (x_s, y_s, z_s), and pixel (y, x)
+// takes cell (k, j0 + y, i0 + x) of its z-slab k, (j0, i0) the cell of pixel (0, 0):
+// the spots of one window (k, j0, i0) read the same cell at the same pixel.  Spots
+// are sorted by window and cut into tiles of up to kTileSpots spots of one window; a
+// workgroup of kRoundWaves wavefronts takes up to 64 spots of a tile, one spot per
+// lane, each wavefront a band of pixel rows.  At each pixel the cell's address is the
+// same for all lanes, so its 64 coefficients are loaded once for 64 spots (the
+// per-spot form gathered them once per spot), and each lane evaluates its own spot.
+// A lane whose own cell differs (FP32 rounding of the cell index, or the grid edge
+// clamping it) evaluates from its own cell instead.  The spot's NLL and parameter
+// gradient stay in the lane.
+//
+// Coefficient gradient: spot s adds W_s(p) z_s^m y_s^n x_s^o to coefficient (m, n, o)
+// of the cell of its pixel p, W_s(p) = g (1 - x/mu) I (zero where the spline value is
+// clamped).  Per pixel row the lanes store their W_s in LDS; then lane q forms, for
+// each cell of the row, sum_s W_s B_s[q] over the 64 spots (B_s[q] = the spot's basis
+// value, held in the lane's registers for the round) and adds it with one atomic per
+// cell (a wavefront atomic over the cell's 256 contiguous bytes).  A lane whose pixel
+// fell in another cell adds its 64 terms to that cell directly.
 constexpr int kMaxTiledR = 28;
-constexpr int kWaves = 16;                   // wavefronts per block
-constexpr int kTileSpots = 256;              // spots per tile (a multiple of every round)
-constexpr int kTiledBlock = kWaves * kWave;
+constexpr int kTileSpots = 256;              // spots per tile
+constexpr int kRoundWaves = 4;               // wavefronts per round (pixel-row bands)
+constexpr int kRoundBlock = kRoundWaves * kWave;
 
 PP_DEV int spline_window_key(const SplineModel& M, const float (&t)[5], int R, SplineModel::Frame& F, int& i0,
                              int& j0, int& k) {
@@ -325,40 +331,21 @@ __global__ void __launch_bounds__(256) adj_key_kernel(SplineModel M, AdjArgs a, 
     atomicAdd(&w.count[key], 1);
 }
 
-// One block of 1024 threads: exclusive scans of the spot counts (-> first sorted
-// position of each key) and of the tile counts, the tile table, and the count array
-// reset to zero for use as the scatter cursor.  Thread i owns keys [i*per, (i+1)*per).
-__global__ void __launch_bounds__(1024) adj_scan_kernel(SplineAdjTiles w) {
-    __shared__ int sc[1024], st[1024];
-    const int tid = threadIdx.x;
-    const int per = (w.nkeys + 1023) / 1024;
-    const int k0 = min(w.nkeys, tid * per), k1 = min(w.nkeys, k0 + per);
-    int c = 0, tl = 0;
-    for (int k = k0; k < k1; k++) {
+// Each window with spots gets a contiguous range of the sorted order and its tiles,
+// both taken from counters (cursor[0]: tiles, cursor[1]: sorted positions), so that
+// only the windows with spots do any work; the count array is reset to zero for use
+// as the scatter cursor.  The order of the windows is arbitrary.
+__global__ void __launch_bounds__(256) adj_alloc_kernel(SplineAdjTiles w) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < w.nkeys; k += gridDim.x * blockDim.x) {
         const int n = w.count[k];
-        c += n;
-        tl += (n + kTileSpots - 1) / kTileSpots;
-    }
-    sc[tid] = c;
-    st[tid] = tl;
-    __syncthreads();
-    // inclusive Hillis-Steele scan of both sums
-    for (int d = 1; d < 1024; d <<= 1) {
-        const int vc = tid >= d ? sc[tid - d] : 0, vt = tid >= d ? st[tid - d] : 0;
-        __syncthreads();
-        sc[tid] += vc;
-        st[tid] += vt;
-        __syncthreads();
-    }
-    int pos = sc[tid] - c, tile = st[tid] - tl;
-    for (int k = k0; k < k1; k++) {
-        const int n = w.count[k];
+        if (n == 0) continue;
+        const int pos = atomicAdd(&w.ntiles[1], n);
+        const int nt = (n + kTileSpots - 1) / kTileSpots;
+        int tile = atomicAdd(&w.ntiles[0], nt);
         w.offset[k] = pos;
         for (int f = 0; f < n; f += kTileSpots) w.tiles[tile++] = make_int2(pos + f, min(kTileSpots, n - f));
-        pos += n;
         w.count[k] = 0;
     }
-    if (tid == 1023) w.ntiles[0] = st[1023];
 }
 
 // spots in key order (the order inside a key is arbitrary)
@@ -369,136 +356,238 @@ __global__ void __launch_bounds__(256) adj_scatter_kernel(AdjArgs a, SplineAdjTi
     w.order[w.offset[key] + atomicAdd(&w.count[key], 1)] = s;
 }
 
-// Per round of RS = 16 SPW spots, LDS holds per window cell c the round's RS weights
-// W_s[c] (at position s ^ (c & 15) inside the cell's RS floats, so that a wavefront's
-// scatter over consecutive cells spreads over the banks) and the round's basis vectors
-// B[s][q].  Thread (wave v, lane q) then forms the entries (cell v + 16 i, coefficient q).
-template <int SPW>
-__global__ void __launch_bounds__(kTiledBlock) spline_adjoint_tiled_kernel(SplineModel M, AdjArgs a,
-                                                                           SplineAdjTiles w) {
-    extern __shared__ float lds[];
-    constexpr int K = 5, RS = kWaves * SPW;
-    M.coefs = a.shared;
-    const int R = a.R, S = R * R, WR = R + 1, ncells = WR * WR;
-    float* Wt = lds;                         // [ncells][RS]
-    float* Bs = lds + (size_t)ncells * RS;   // [RS][64]
-    const int lane = threadIdx.x & (kWave - 1), v = threadIdx.x / kWave;
-    const int m = lane >> 4, nn = (lane >> 2) & 3, o = lane & 3;
-    const int ntiles = w.ntiles[0];
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int2 td = w.tiles[tile];
-        const int first = td.x, cnt = td.y;
-        const int key0 = w.key[w.order[first]];
-        const int i0 = key0 % M.Nx, j0 = (key0 / M.Nx) % M.Ny, k0 = key0 / (M.Nx * M.Ny);
-        for (int c0 = 0; c0 < cnt; c0 += RS) {
-            for (int i = threadIdx.x; i < ncells * RS; i += kTiledBlock) Wt[i] = 0.0f;
-            __syncthreads();
-#pragma unroll 1
-            for (int u = 0; u < SPW; u++) {
-                const int sl = v + kWaves * u;  // the spot's column in this round
-                const int slot = c0 + sl;
-                float basis = 0.0f;
-                if (slot < cnt) {
-                    const int spot = w.order[first + slot];
-                    float t[K];
+// Value of spline cell c at the frame's sub-voxel position (the NLL needs no derivatives).
+PP_DEV float spline_cell_value(const float* c, const SplineModel::Frame& F) {
+    const float4* c4 = reinterpret_cast<const float4*>(c);
+    float f = 0.0f, zm = 1.0f;
 #pragma unroll
-                    for (int k = 0; k < K; k++) t[k] = a.params[(size_t)spot * K + k];
-                    SplineModel::Frame F;
-                    M.frame(t, R, F);
-                    const float go = a.gout ? a.gout[spot] : 1.0f;
-                    {
-                        const float zb = m == 0 ? 1.0f : m == 1 ? F.zp[1] : m == 2 ? F.zp[2] : F.zp[3];
-                        const float yb = nn == 0 ? 1.0f : nn == 1 ? F.yp[1] : nn == 2 ? F.yp[2] : F.yp[3];
-                        const float xb = o == 0 ? 1.0f : o == 1 ? F.x1 : o == 2 ? F.x2 : F.x3;
-                        basis = zb * yb * xb;
-                    }
-                    float sums[K + 1];
+    for (int m = 0; m < 4; m++) {
+        float a = 0.0f;
 #pragma unroll
-                    for (int i = 0; i < K + 1; i++) sums[i] = 0.0f;
-                    const int kSpot = (int)((F.slab - M.coefs) / ((size_t)64 * M.Nx * M.Ny));
-                    // every lane runs every trip (the guard path below is a wavefront operation)
-                    for (int p0 = 0; p0 < S; p0 += kWave) {
-                        const int p = p0 + lane;
-                        const bool valid = p < S;
-                        const int y = valid ? p / R : 0, x = valid ? p - y * R : 0;
-                        const int ci = M.col_index(F, x);
-                        float W = 0.0f;
-                        bool outside = false;
-                        if (valid) {
-                            float f, fx, fy, fz, mu, J[K];
-                            spline_cell_lean(M.row_ptr(F, y) + 64 * ci, F, f, fx, fy, fz);
-                            SplineModel::finish(F, t, f, fx, fy, fz, x, y, R, mu, J);
-                            if (f < 0.0f) J[0] = J[1] = J[2] = 0.0f;  // mu does not depend on position where clamped
-                            const float wv = nll_weight(mu, a.sample[(size_t)spot * S + p], sums[K]);
-#pragma unroll
-                            for (int k = 0; k < K; k++) sums[k] += wv * J[k];
-                            W = (f > 0.0f) ? go * wv * t[3] : 0.0f;
-                        }
-                        if (W != 0.0f) {
-                            const int cj = clampi((int)((float)y - F.ty), 0, M.Ny - 1) - j0;
-                            const int cx = ci - i0;
-                            if (kSpot == k0 && cx >= 0 && cx < WR && cj >= 0 && cj < WR) {
-                                const int c = cj * WR + cx;
-                                atomicAdd(&Wt[c * RS + (sl ^ (c & 15))], W);
-                            } else {
-                                outside = true;
-                            }
-                        }
-                        // guard path: the wavefront adds a flagged pixel's 64 terms straight to HBM
-                        unsigned long long any = __ballot(outside);
-                        while (any) {
-                            const int src = __ffsll((long long)any) - 1;
-                            any &= any - 1;
-                            const float g = __shfl(W, src, kWave);
-                            const int ys = __shfl(y, src, kWave), xs = __shfl(x, src, kWave);
-                            const int cell = (int)((M.row_ptr(F, ys) + 64 * M.col_index(F, xs) - M.coefs) >> 6);
-                            atomicAdd(&a.gshared[(size_t)cell * 64 + lane], g * basis);
-                        }
-                    }
-                    group_sum_n<kWave>(sums);
-                    if (lane == 0) {
-                        if (a.nll) a.nll[spot] = sums[K];
-                        if (a.gparams) {
-#pragma unroll
-                            for (int k = 0; k < K; k++) a.gparams[(size_t)spot * K + k] = go * sums[k];
-                        }
-                    }
-                }
-                Bs[sl * kWave + lane] = basis;
-            }
-            __syncthreads();
-            // this thread's cells are v + 16 i, whose swizzle is v: spot s sits at s ^ v
-            float Bq[RS];
-#pragma unroll
-            for (int s2 = 0; s2 < RS; s2++) Bq[s2] = Bs[(s2 ^ v) * kWave + lane];
-            for (int c = v; c < ncells; c += kWaves) {
-                const float4* wc = reinterpret_cast<const float4*>(Wt + (size_t)c * RS);
-                float sum = 0.0f;
-#pragma unroll
-                for (int q4 = 0; q4 < RS / 4; q4++) {
-                    const float4 wv = wc[q4];
-                    sum += wv.x * Bq[4 * q4] + wv.y * Bq[4 * q4 + 1] + wv.z * Bq[4 * q4 + 2] + wv.w * Bq[4 * q4 + 3];
-                }
-                const int gi = i0 + c % WR, gj = j0 + c / WR;
-                if (sum != 0.0f && gi < M.Nx && gj < M.Ny)
-                    atomicAdd(&a.gshared[(((size_t)k0 * M.Ny + gj) * M.Nx + gi) * 64 + lane], sum);
-            }
-            __syncthreads();
+        for (int n = 0; n < 4; n++) {
+            const float4 q = c4[m * 4 + n];
+            a += (q.x + q.y * F.x1 + q.z * F.x2 + q.w * F.x3) * F.yp[n];
         }
+        f += a * zm;
+        zm *= F.zp[1];
     }
+    return f;
 }
 
-// Spots per wavefront and round: 2 where the round's weights fit in 64 KiB of LDS.
-static int tiled_spw(int R) {
-    const size_t cells = (size_t)(R + 1) * (R + 1);
-    return (cells * 2 * kWaves + 2 * kWaves * kWave) * 4 <= 65536 ? 2 : 1;
+// LDS floats of a round's workgroup: the 64 spots' powers x^o, y^n, z^m (12 each), the
+// per-wave partial sums (K + 1 per spot), the round's spot indices, each wave's samples
+// of one pixel row ([R][kSmStride]) and, with the coefficient gradient, each wave's
+// weights of one pixel row ([R][64]).
+// Only the NLL-only form stages the samples: the gradient form's arithmetic per pixel
+// hides its per-lane sample loads (staged, it measured 15% slower: 0.72 vs 0.63 ms on
+// 65,536 20x20 ROIs), the NLL form's does not (0.40 vs 0.56 ms).
+constexpr int kSmStride = kWave + 1;  // the staging writes of consecutive pixels spread over the banks
+constexpr size_t round_lds_floats(int R, bool grad) {
+    return 12 * kWave + (size_t)kRoundWaves * 6 * kWave + kWave +
+           (grad ? (size_t)kRoundWaves * R * kWave : (size_t)kRoundWaves * R * kSmStride);
+}
+
+// cf: the coefficient table (a.shared) as a restrict argument, through which the
+// shared cell's wave-uniform loads become scalar loads: 256 bytes into SGPRs per
+// pixel and wavefront, where a vector load delivers them to all 64 lanes' VGPRs.
+template <bool GRAD>
+__global__ void __launch_bounds__(kRoundBlock) spline_adjoint_round_kernel(SplineModel M, AdjArgs a,
+                                                                           SplineAdjTiles w,
+                                                                           const float* __restrict__ cf) {
+    extern __shared__ float lds[];
+    constexpr int K = 5;
+    M.coefs = a.shared;
+    const int R = a.R, S = R * R;
+    // v is wave-uniform; readfirstlane tells the compiler, which keeps the row loop and
+    // the shared cell's address in scalar registers
+    const int lane = threadIdx.x & (kWave - 1), v = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    float* Pw = lds;                          // [12][64]: x^o (4), y^n (4), z^m (4) of each spot
+    float* Ps = lds + 12 * kWave;             // [waves][K + 1][64] partial sums
+    int* Sp = reinterpret_cast<int*>(Ps + kRoundWaves * 6 * kWave);  // [64] the round's spots
+    // NLL form: this wave's samples of one pixel row, [R][65]; gradient form: its row
+    // weights, [R][64]
+    float* Sm = reinterpret_cast<float*>(Sp + kWave) + (size_t)v * R * kSmStride;
+    float* Wt = reinterpret_cast<float*>(Sp + kWave) + (size_t)v * R * kWave;
+    // row staging: element i = s R + x of a row's 64 R samples (spot s, pixel x), i = lane + 64 k
+    const int sq = kWave / R, sr = kWave - sq * R;
+    const int s0 = lane / R, x0 = lane - s0 * R;
+    const int rows = (R + kRoundWaves - 1) / kRoundWaves;
+    const int y0 = v * rows, y1 = min(R, y0 + rows);
+    // lane q's basis index q = 16m + 4n + o (coefficient gradient)
+    const int m = lane >> 4, nn = (lane >> 2) & 3, o = lane & 3;
+    const int items = w.ntiles[0] * (kTileSpots / kWave);
+    for (int item = blockIdx.x; item < items; item += gridDim.x) {
+        const int2 td = w.tiles[item / (kTileSpots / kWave)];
+        const int base = (item % (kTileSpots / kWave)) * kWave;
+        if (base >= td.y) continue;  // the whole workgroup skips together
+        const int cnt = min(kWave, td.y - base);
+        const bool live = lane < cnt;
+        const int spot = w.order[td.x + base + (live ? lane : 0)];
+        const int key0 = __builtin_amdgcn_readfirstlane(w.key[w.order[td.x]]);
+        const int i0 = key0 % M.Nx, j0 = (key0 / M.Nx) % M.Ny, k0 = key0 / (M.Nx * M.Ny);
+        float t[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) t[k] = a.params[(size_t)spot * K + k];
+        SplineModel::Frame F;
+        M.frame(t, R, F);
+        const float go = a.gout ? a.gout[spot] : 1.0f;
+        if (v == 0) Sp[lane] = spot;
+        if constexpr (GRAD) {
+            if (v == 0) {
+                Pw[0 * kWave + lane] = 1.0f;
+                Pw[1 * kWave + lane] = F.x1;
+                Pw[2 * kWave + lane] = F.x2;
+                Pw[3 * kWave + lane] = F.x3;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    Pw[(4 + i) * kWave + lane] = F.yp[i];
+                    Pw[(8 + i) * kWave + lane] = F.zp[i];
+                }
+            }
+        }
+        __syncthreads();
+        float sums[K + 1];
+#pragma unroll
+        for (int i = 0; i < K + 1; i++) sums[i] = 0.0f;
+        for (int y = y0; y < y1; y++) {
+            // the 64 spots' samples of row y into LDS, transposed to [x][spot]: each load
+            // instruction reads runs of R consecutive floats of about 64 / R spots (reading
+            // its own spot's sample per pixel, a lane would touch a line of its own per load)
+            // (in chunks of 8 elements per lane, so that a chunk's index reads, loads and
+            // stores are each in flight together)
+            if constexpr (!GRAD) {
+                int st = s0, xt = x0;
+                for (int k0 = 0; k0 < R; k0 += 8) {
+                    int si[8], xi[8], sp[8];
+                    float sv[8];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        si[j] = st;
+                        xi[j] = xt;
+                        xt += sr;
+                        st += sq;
+                        if (xt >= R) {
+                            xt -= R;
+                            st++;
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; j++) sp[j] = k0 + j < R ? Sp[si[j]] : 0;
+#pragma unroll
+                    for (int j = 0; j < 8; j++)
+                        sv[j] = k0 + j < R ? a.sample[(size_t)sp[j] * S + y * R + xi[j]] : 0.0f;
+#pragma unroll
+                    for (int j = 0; j < 8; j++)
+                        if (k0 + j < R) Sm[xi[j] * kSmStride + si[j]] = sv[j];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            }
+            const int cyu = min(j0 + y, M.Ny - 1);
+            const float* rowu = cf + 64 * ((size_t)(k0 * M.Ny + cyu) * M.Nx);
+            const int cy = clampi((int)((float)y - F.ty), 0, M.Ny - 1);
+            const float* rowo = cf + 64 * ((size_t)(k0 * M.Ny + cy) * M.Nx);
+            for (int x = 0; x < R; x++) {
+                const int cxu = min(i0 + x, M.Nx - 1);
+                const int cx = M.col_index(F, x);
+                const bool own = cy == cyu && cx == cxu;  // the lane's cell is the shared one
+                const float xs = GRAD ? a.sample[(size_t)spot * S + y * R + x] : Sm[x * kSmStride + lane];
+                float W = 0.0f;
+                if constexpr (GRAD) {
+                    float f, fx, fy, fz, mu, J[K];
+                    SplineModel::cell(rowu + 64 * cxu, F, f, fx, fy, fz);  // scalar loads
+                    if (!own) SplineModel::cell(rowo + 64 * cx, F, f, fx, fy, fz);
+                    SplineModel::finish(F, t, f, fx, fy, fz, x, y, R, mu, J);
+                    if (f < 0.0f) J[0] = J[1] = J[2] = 0.0f;  // mu does not depend on position where clamped
+                    const float wv = nll_weight(mu, xs, sums[K]);
+#pragma unroll
+                    for (int k = 0; k < K; k++) sums[k] += wv * J[k];
+                    W = (live && f > 0.0f) ? go * wv * t[3] : 0.0f;
+                    Wt[x * kWave + lane] = own ? W : 0.0f;
+                    // a pixel in another cell: its 64 terms straight to that cell
+                    unsigned long long any = a.gshared ? __ballot(!own && W != 0.0f) : 0ull;
+                    while (any) {
+                        const int src = __ffsll((long long)any) - 1;
+                        any &= any - 1;
+                        const float g = __shfl(W, src, kWave);
+                        const int cs = __shfl((k0 * M.Ny + cy) * M.Nx + cx, src, kWave);
+                        const float b = Pw[(8 + m) * kWave + src] * Pw[(4 + nn) * kWave + src] * Pw[o * kWave + src];
+                        atomicAdd(&a.gshared[(size_t)cs * 64 + lane], g * b);
+                    }
+                } else {
+                    float f = spline_cell_value(rowu + 64 * cxu, F);  // scalar loads
+                    if (!own) f = spline_cell_value(rowo + 64 * cx, F);
+                    const float psf = f < 0.0f ? 0.0f : f;
+                    nll_weight(t[3] * psf + t[4], xs, sums[K]);
+                }
+            }
+            if constexpr (GRAD) {
+                if (a.gshared) {
+                    // the row's weights are in LDS before any lane reads them (one wavefront:
+                    // workgroup-scope fences order its LDS operations)
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    // lane q: B_s[q] = z_s^m y_s^n x_s^o of every spot s of the round, formed
+                    // here per row (the fences keep it from being held over the pixel loop)
+                    float B[kWave];
+#pragma unroll
+                    for (int s4 = 0; s4 < kWave; s4 += 4) {
+                        const float4 zq = *reinterpret_cast<const float4*>(Pw + (8 + m) * kWave + s4);
+                        const float4 yq = *reinterpret_cast<const float4*>(Pw + (4 + nn) * kWave + s4);
+                        const float4 xq = *reinterpret_cast<const float4*>(Pw + o * kWave + s4);
+                        B[s4] = zq.x * yq.x * xq.x;
+                        B[s4 + 1] = zq.y * yq.y * xq.y;
+                        B[s4 + 2] = zq.z * yq.z * xq.z;
+                        B[s4 + 3] = zq.w * yq.w * xq.w;
+                    }
+                    for (int x = 0; x < R; x++) {
+                        const float4* wc = reinterpret_cast<const float4*>(Wt + x * kWave);
+                        float g = 0.0f;
+#pragma unroll
+                        for (int s4 = 0; s4 < kWave / 4; s4++) {
+                            const float4 wv = wc[s4];
+                            g += wv.x * B[4 * s4] + wv.y * B[4 * s4 + 1] + wv.z * B[4 * s4 + 2] + wv.w * B[4 * s4 + 3];
+                        }
+                        if (g != 0.0f) {
+                            const int cxu = min(i0 + x, M.Nx - 1);
+                            atomicAdd(&a.gshared[((size_t)(k0 * M.Ny + cyu) * M.Nx + cxu) * 64 + lane], g);
+                        }
+                    }
+                }
+            }
+            // every lane's reads complete before the next row overwrites the samples and weights
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        }
+        // the spot's sums over the waves' row bands, in a fixed order
+#pragma unroll
+        for (int i = 0; i < K + 1; i++) Ps[(v * (K + 1) + i) * kWave + lane] = sums[i];
+        __syncthreads();
+        if (v == 0 && live) {
+            float tot[K + 1];
+#pragma unroll
+            for (int i = 0; i < K + 1; i++) {
+                tot[i] = Ps[i * kWave + lane];
+                for (int u = 1; u < kRoundWaves; u++) tot[i] += Ps[(u * (K + 1) + i) * kWave + lane];
+            }
+            if (a.nll) a.nll[spot] = tot[K];
+            if (GRAD && a.gparams) {
+#pragma unroll
+                for (int k = 0; k < K; k++) a.gparams[(size_t)spot * K + k] = go * tot[k];
+            }
+        }
+        __syncthreads();  // Pw and Ps are reused by the next round
+    }
 }
 
 
 size_t spline_adjoint_workspace(int n, int nkeys) {
     const size_t maxTiles = (size_t)(n + kTileSpots - 1) / kTileSpots + (size_t)std::min(n, nkeys);
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    return al((size_t)n * 4) * 2 + al((size_t)nkeys * 4) * 2 + al(maxTiles * 8) + al(4);
+    return al((size_t)n * 4) * 2 + al((size_t)nkeys * 4) * 2 + al(maxTiles * 8) + al(8);
 }
 
 static SplineAdjTiles carve_tiles(void* ws, int n, int nkeys) {
@@ -523,26 +612,32 @@ static hipError_t launch_spline_tiled(const SplineModel& M, const AdjArgs& a, vo
     const int nkeys = M.Nz * M.Ny * M.Nx;
     SplineAdjTiles w = carve_tiles(ws, a.n, nkeys);
     const unsigned spotBlocks = (unsigned)((a.n + 255) / 256);
-    // count and the tile counter start at zero (a fill kernel, no memset: capturable)
-    hipLaunchKernelGGL(adj_fill_kernel, dim3(std::min(1024, (nkeys + 255) / 256)), dim3(256), 0, st, w.count, nkeys, 0);
+    const unsigned keyBlocks = (unsigned)std::min(1024, (nkeys + 255) / 256);
+    // counts and the two allocation counters start at zero (fill kernels, no memset:
+    // capturable)
+    hipLaunchKernelGGL(adj_fill_kernel, dim3(keyBlocks), dim3(256), 0, st, w.count, nkeys, 0);
+    hipLaunchKernelGGL(adj_fill_kernel, dim3(1), dim3(64), 0, st, w.ntiles, 2, 0);
     hipLaunchKernelGGL(adj_key_kernel, dim3(spotBlocks), dim3(256), 0, st, M, a, w);
-    hipLaunchKernelGGL(adj_scan_kernel, dim3(1), dim3(1024), 0, st, w);
+    hipLaunchKernelGGL(adj_alloc_kernel, dim3(keyBlocks), dim3(256), 0, st, w);
     hipLaunchKernelGGL(adj_scatter_kernel, dim3(spotBlocks), dim3(256), 0, st, a, w);
-    const int WR = a.R + 1, spw = tiled_spw(a.R);
-    const size_t lds = ((size_t)WR * WR * kWaves * spw + (size_t)kWaves * spw * kWave) * sizeof(float);
-    auto go = [&](auto kfn) -> hipError_t {
-        static int resident[kMaxTiledR + 1] = {};  // persistent grid per roisize (one device model)
-        if (resident[a.R] == 0) resident[a.R] = resident_blocks(reinterpret_cast<const void*>(kfn), kTiledBlock, lds);
-        if (resident[a.R] <= 0) return hipErrorNoDevice;
-        const int blocks = std::max(1, std::min(resident[a.R], w.maxTiles));
-        hipLaunchKernelGGL(kfn, dim3(blocks), dim3(kTiledBlock), lds, st, M, a, w);
+    const bool grad = a.gshared || a.gparams;
+    const size_t lds = round_lds_floats(a.R, grad) * sizeof(float);
+    auto go = [&](auto kfn, int* cache) -> hipError_t {
+        if (*cache == 0) *cache = resident_blocks(reinterpret_cast<const void*>(kfn), kRoundBlock, lds);
+        if (*cache <= 0) return hipErrorNoDevice;
+        // one work item per 64 spots of a tile: at most maxTiles x 4
+        const int blocks = std::max(1, std::min(*cache, w.maxTiles * (kTileSpots / kWave)));
+        hipLaunchKernelGGL(kfn, dim3(blocks), dim3(kRoundBlock), lds, st, M, a, w, a.shared);
         return hipGetLastError();
     };
-    return spw == 2 ? go(spline_adjoint_tiled_kernel<2>) : go(spline_adjoint_tiled_kernel<1>);
+    // resident workgroups per roisize and form (one device model)
+    static int residentGrad[kMaxTiledR + 1] = {}, residentNll[kMaxTiledR + 1] = {};
+    return grad ? go(spline_adjoint_round_kernel<true>, &residentGrad[a.R])
+                : go(spline_adjoint_round_kernel<false>, &residentNll[a.R]);
 }
 
 hipError_t launch_adjoint_any(const ModelDesc& d, const AdjArgs& a, void* ws, hipStream_t st) {
-    if (d.kind == kKindSpline && a.gshared && ws && spline_adjoint_tiled(a.R)) {
+    if (d.kind == kKindSpline && ws && spline_adjoint_tiled(a.R)) {
         SplineModel M = d.sm;
         return launch_spline_tiled(M, a, ws, st);
     }

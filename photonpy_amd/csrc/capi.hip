@@ -1022,7 +1022,7 @@ int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const fl
             a.gparams = d_grad_params;
             a.gshared = d_grad_shared;
             void* ws = nullptr;
-            if (e->desc.kind == kKindSpline && d_grad_shared && spline_adjoint_tiled(e->roisize)) {
+            if (e->desc.kind == kKindSpline && spline_adjoint_tiled(e->roisize)) {
                 const long long nkeys = (long long)e->desc.sm.Nz * e->desc.sm.Ny * e->desc.sm.Nx;
                 if (nkeys > (1LL << 30)) throw std::runtime_error("photonpy_amd: spline table too large");
                 ws = e->adjoint_workspace(spline_adjoint_workspace(n, (int)nkeys));

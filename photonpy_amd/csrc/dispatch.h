@@ -137,11 +137,11 @@ struct SplineAdjTiles {
     int* count;   // [nkeys] spots per key, then the scatter cursor
     int* offset;  // [nkeys] first sorted position of each key
     int2* tiles;  // [maxTiles] (first sorted position, spots)
-    int* ntiles;  // [1]
+    int* ntiles;  // [2]: tiles, then the sorted positions handed out (allocation counters)
     int nkeys, maxTiles;
 };
-// Whether the spline adjoint of roisize R uses the tiled coefficient gradient, and the
-// bytes of scratch it needs for n spots and nkeys = Nz Ny Nx window keys.
+// Whether the spline NLL and adjoint of roisize R run by window rounds (spots sorted by
+// window), and the bytes of scratch that needs for n spots and nkeys = Nz Ny Nx window keys.
 bool spline_adjoint_tiled(int R);
 size_t spline_adjoint_workspace(int n, int nkeys);
 // ws: spline_adjoint_workspace bytes of device scratch, or null (direct atomics).
