@@ -180,17 +180,18 @@ __global__ void __launch_bounds__(256) gauss_adjoint_kernel(GaussModel<V> M, Adj
     }
 }
 
-// Value and x/y/z derivatives of spline cell c at the frame's sub-voxel position, one
-// pair of z-powers at a time (32 coefficients in registers instead of 64): the tiled
-// kernel's 1024-thread blocks leave it 128 VGPRs.  The same polynomial as SplineModel::cell,
-// summed over z by Horner-free running powers.
+// Value and x/y/z derivatives of spline cell c at the frame's sub-voxel position,
+// UNROLL_M z-powers of 16 coefficients at a time (fewer coefficients in registers than
+// SplineModel::cell's 64), for the rarely taken per-lane cell of the round kernel.  The
+// same polynomial as SplineModel::cell, summed over z by Horner-free running powers.
+template <int UNROLL_M = 2>
 PP_DEV void spline_cell_lean(const float* c, const SplineModel::Frame& F, float& f, float& fx, float& fy,
                              float& fz) {
     const float4* c4 = reinterpret_cast<const float4*>(c);
     const float z = F.zp[1];
     float zm = 1.0f, dzm = 0.0f;  // z^m and m z^(m-1)
     f = fx = fy = fz = 0.0f;
-#pragma unroll 2
+#pragma unroll UNROLL_M
     for (int m = 0; m < 4; m++) {
         float a = 0.0f, ax = 0.0f, ay = 0.0f;
 #pragma unroll
@@ -356,11 +357,15 @@ __global__ void __launch_bounds__(256) adj_scatter_kernel(AdjArgs a, SplineAdjTi
     w.order[w.offset[key] + atomicAdd(&w.count[key], 1)] = s;
 }
 
-// Value of spline cell c at the frame's sub-voxel position (the NLL needs no derivatives).
+// Value of spline cell c at the frame's sub-voxel position (the NLL needs no derivatives),
+// UNROLL_M z-powers of 16 coefficients at a time (4: all 64 loads in flight; 1: 16
+// registers of coefficients, for the rarely taken per-lane cell of the round kernels,
+// which would otherwise set their register peak).
+template <int UNROLL_M = 4>
 PP_DEV float spline_cell_value(const float* c, const SplineModel::Frame& F) {
     const float4* c4 = reinterpret_cast<const float4*>(c);
     float f = 0.0f, zm = 1.0f;
-#pragma unroll
+#pragma unroll UNROLL_M
     for (int m = 0; m < 4; m++) {
         float a = 0.0f;
 #pragma unroll
@@ -450,6 +455,14 @@ __global__ void __launch_bounds__(kRoundBlock) spline_adjoint_round_kernel(Splin
 #pragma unroll
         for (int i = 0; i < K + 1; i++) sums[i] = 0.0f;
         for (int y = y0; y < y1; y++) {
+            if constexpr (GRAD) {
+                // the frame again per row from an opaque copy of the parameters (an empty asm,
+                // no instruction), so that its powers are not held through the row's outer
+                // product, whose 64 basis values set the kernel's register peak
+#pragma unroll
+                for (int k = 0; k < K; k++) asm volatile("" : "+v"(t[k]));
+                M.frame(t, R, F);
+            }
             // the 64 spots' samples of row y into LDS, transposed to [x][spot]: each load
             // instruction reads runs of R consecutive floats of about 64 / R spots (reading
             // its own spot's sample per pixel, a lane would touch a line of its own per load)
@@ -497,7 +510,7 @@ __global__ void __launch_bounds__(kRoundBlock) spline_adjoint_round_kernel(Splin
                 if constexpr (GRAD) {
                     float f, fx, fy, fz, mu, J[K];
                     SplineModel::cell(rowu + 64 * cxu, F, f, fx, fy, fz);  // scalar loads
-                    if (!own) SplineModel::cell(rowo + 64 * cx, F, f, fx, fy, fz);
+                    if (!own) spline_cell_lean<1>(rowo + 64 * cx, F, f, fx, fy, fz);
                     SplineModel::finish(F, t, f, fx, fy, fz, x, y, R, mu, J);
                     if (f < 0.0f) J[0] = J[1] = J[2] = 0.0f;  // mu does not depend on position where clamped
                     const float wv = nll_weight(mu, xs, sums[K]);
@@ -517,7 +530,7 @@ __global__ void __launch_bounds__(kRoundBlock) spline_adjoint_round_kernel(Splin
                     }
                 } else {
                     float f = spline_cell_value(rowu + 64 * cxu, F);  // scalar loads
-                    if (!own) f = spline_cell_value(rowo + 64 * cx, F);
+                    if (!own) f = spline_cell_value<1>(rowo + 64 * cx, F);
                     const float psf = f < 0.0f ? 0.0f : f;
                     nll_weight(t[3] * psf + t[4], xs, sums[K]);
                 }
