@@ -170,11 +170,12 @@ def test_spline_coefficient_gradients(ctx):
 def test_spline_coefficient_gradients_many_spots(ctx, R):
     """The coefficient gradient of thousands of spots, half of them packed into a few
     windows (same z-slab, x and y within +-0.2 px) so that windows hold several tiles
-    and many rounds (R <= 28; R = 30 takes the direct per-pixel scatter), against an FP64
-    sparse restatement of
+    and many rounds (R <= 28 run by window rounds, one spot per lane; R = 30 takes the
+    one-wavefront-per-spot form and its per-pixel scatter), against an FP64 sparse
+    restatement of
     d NLL / d c[cell, q] = sum over the pixels of the cell of g (1 - x/mu) I basis_q (psf > 0).
-    The per-spot gradients of that pass equal those of the pass without the coefficient
-    gradient."""
+    The forward NLL of every spot matches the FP64 oracle's, and the per-spot gradients of
+    that pass equal those of a pass that forms no coefficient gradient."""
     import scipy.sparse as sp
     import torch
     from tests import _spline_np as SN
@@ -196,10 +197,11 @@ def test_spline_coefficient_gradients_many_spots(ctx, R):
     d_smp = torch.from_numpy(smp).cuda()
     gout = np.linspace(0.5, 1.5, n).astype(np.float32)
     nll = poisson_nll(est, d_th, d_smp, shared)
+    assert _rel(nll.detach().cpu().numpy(), _nll64(om, th.astype(np.float64), smp)) < 1e-5
     (nll * torch.from_numpy(gout).cuda()).sum().backward()
     g = shared.grad.cpu().numpy().reshape(-1, 64).astype(np.float64)
     gp_tiled = d_th.grad.cpu().numpy()
-    # the same per-spot gradients without the coefficient gradient (direct kernel)
+    # the same per-spot gradients from a pass that forms no coefficient gradient
     d_th2 = torch.from_numpy(th).cuda().requires_grad_(True)
     (poisson_nll(est, d_th2, d_smp, shared.detach()) * torch.from_numpy(gout).cuda()).sum().backward()
     gp_direct = d_th2.grad.cpu().numpy()
