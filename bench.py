@@ -544,7 +544,7 @@ def bench_adjoint(steps, warmup, n=0):
     ms = wall * 1e3 / steps
     achieved = n * ADJ_FLOP_PER_ROI / (bwd_ms * 1e-3) / 1e12
     atomics_tiled = rounds * R * R * 64  # at most one 64-lane atomic per pixel cell and round
-    roof = {"kernel": "backward pass (window sort + spline_adjoint_round_kernel<true>)", "bound": BOUND,
+    roof = {"kernel": "backward pass (spline_adjoint_round_kernel<true>, reusing the forward's window sort)", "bound": BOUND,
             "compute": COMPUTE_NOTE, "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32, "traffic": None}
     detail = {"forward_ms": fwd_ms, "backward_ms": bwd_ms, "backward_algorithmic_flop": n * ADJ_FLOP_PER_ROI,

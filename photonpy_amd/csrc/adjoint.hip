@@ -180,16 +180,20 @@ __global__ void __launch_bounds__(256) gauss_adjoint_kernel(GaussModel<V> M, Adj
     }
 }
 
-// Value and x/y/z derivatives of spline cell c at the frame's sub-voxel position,
-// UNROLL_M z-powers of 16 coefficients at a time (fewer coefficients in registers than
-// SplineModel::cell's 64), for the rarely taken per-lane cell of the round kernel.  The
-// same polynomial as SplineModel::cell, summed over z by Horner-free running powers.
-template <int UNROLL_M = 2>
-PP_DEV void spline_cell_lean(const float* c, const SplineModel::Frame& F, float& f, float& fx, float& fy,
+// Value and x/y/z derivatives of spline cell c at the frame's sub-voxel position, one
+// z-power (16 coefficients) after another, for a lane's own cell in the round kernel
+// (rarely taken; with all 64 coefficients in registers it set the kernel's register
+// peak).  The arithmetic and its order are SplineModel::cell's
+//   f = fm0 + fm1 z + fm2 z^2 + fm3 z^3,  fz = fm1 + fm2 2z + fm3 3z^2
+// (z-powers formed as the frame forms them), so it gives SplineModel::cell's bits.
+template <int UNROLL_M>
+PP_DEV void spline_cell_by_z(const float* c, const SplineModel::Frame& F, float& f, float& fx, float& fy,
                              float& fz) {
     const float4* c4 = reinterpret_cast<const float4*>(c);
-    const float z = F.zp[1];
-    float zm = 1.0f, dzm = 0.0f;  // z^m and m z^(m-1)
+    // the frame's z-powers, formed from z by the frame's own operations (z z, (z z) z,
+    // 2 z, 3 (z z)) and selected per m in registers: indexing the frame's arrays with m
+    // put the frame in scratch memory
+    const float z = F.zp[1], z2 = z * z, z3 = z2 * z, dz2 = 2.0f * z, dz3 = 3.0f * z2;
     f = fx = fy = fz = 0.0f;
 #pragma unroll UNROLL_M
     for (int m = 0; m < 4; m++) {
@@ -203,12 +207,18 @@ PP_DEV void spline_cell_lean(const float* c, const SplineModel::Frame& F, float&
             ax += dpx * F.yp[n];
             ay += px * F.dyp[n];
         }
-        f += a * zm;
-        fx += ax * zm;
-        fy += ay * zm;
-        fz += a * dzm;
-        dzm = dzm * z + zm;
-        zm *= z;
+        const float zm = m == 1 ? z : m == 2 ? z2 : z3;
+        const float dzm = m == 2 ? dz2 : dz3;
+        if (m == 0) {
+            f = a;
+            fx = ax;
+            fy = ay;
+        } else {
+            f += a * zm;
+            fx += ax * zm;
+            fy += ay * zm;
+            fz = m == 1 ? a : fz + a * dzm;
+        }
     }
 }
 
@@ -436,6 +446,9 @@ __global__ void __launch_bounds__(kRoundBlock) spline_adjoint_round_kernel(Splin
         SplineModel::Frame F;
         M.frame(t, R, F);
         const float go = a.gout ? a.gout[spot] : 1.0f;
+        // the lane's own z-slab: the shared cells are the window's only for spots of the
+        // window's slab (always, after the sort of this call; a reused order may be stale)
+        const int ks = (int)((F.slab - M.coefs) / ((size_t)64 * M.Nx * M.Ny));
         if (v == 0) Sp[lane] = spot;
         if constexpr (GRAD) {
             if (v == 0) {
@@ -500,17 +513,20 @@ __global__ void __launch_bounds__(kRoundBlock) spline_adjoint_round_kernel(Splin
             const int cyu = min(j0 + y, M.Ny - 1);
             const float* rowu = cf + 64 * ((size_t)(k0 * M.Ny + cyu) * M.Nx);
             const int cy = clampi((int)((float)y - F.ty), 0, M.Ny - 1);
-            const float* rowo = cf + 64 * ((size_t)(k0 * M.Ny + cy) * M.Nx);
+            const float* rowo = cf + 64 * ((size_t)(ks * M.Ny + cy) * M.Nx);
             for (int x = 0; x < R; x++) {
                 const int cxu = min(i0 + x, M.Nx - 1);
                 const int cx = M.col_index(F, x);
-                const bool own = cy == cyu && cx == cxu;  // the lane's cell is the shared one
+                const bool own = ks == k0 && cy == cyu && cx == cxu;  // the lane's cell is the shared one
                 const float xs = GRAD ? a.sample[(size_t)spot * S + y * R + x] : Sm[x * kSmStride + lane];
                 float W = 0.0f;
                 if constexpr (GRAD) {
                     float f, fx, fy, fz, mu, J[K];
-                    SplineModel::cell(rowu + 64 * cxu, F, f, fx, fy, fz);  // scalar loads
-                    if (!own) spline_cell_lean<1>(rowo + 64 * cx, F, f, fx, fy, fz);
+                    // the shared cell by scalar loads, a lane's own one (rare) 16 coefficients
+                    // at a time: the same arithmetic in the same order, so a spot's results do
+                    // not depend on which round it falls in
+                    SplineModel::cell(rowu + 64 * cxu, F, f, fx, fy, fz);
+                    if (!own) spline_cell_by_z<1>(rowo + 64 * cx, F, f, fx, fy, fz);
                     SplineModel::finish(F, t, f, fx, fy, fz, x, y, R, mu, J);
                     if (f < 0.0f) J[0] = J[1] = J[2] = 0.0f;  // mu does not depend on position where clamped
                     const float wv = nll_weight(mu, xs, sums[K]);
@@ -524,7 +540,7 @@ __global__ void __launch_bounds__(kRoundBlock) spline_adjoint_round_kernel(Splin
                         const int src = __ffsll((long long)any) - 1;
                         any &= any - 1;
                         const float g = __shfl(W, src, kWave);
-                        const int cs = __shfl((k0 * M.Ny + cy) * M.Nx + cx, src, kWave);
+                        const int cs = __shfl((ks * M.Ny + cy) * M.Nx + cx, src, kWave);
                         const float b = Pw[(8 + m) * kWave + src] * Pw[(4 + nn) * kWave + src] * Pw[o * kWave + src];
                         atomicAdd(&a.gshared[(size_t)cs * 64 + lane], g * b);
                     }
@@ -621,18 +637,21 @@ static SplineAdjTiles carve_tiles(void* ws, int n, int nkeys) {
 
 bool spline_adjoint_tiled(int R) { return R >= 1 && R <= kMaxTiledR; }
 
-static hipError_t launch_spline_tiled(const SplineModel& M, const AdjArgs& a, void* ws, hipStream_t st) {
+static hipError_t launch_spline_tiled(const SplineModel& M, const AdjArgs& a, void* ws, bool sorted,
+                                      hipStream_t st) {
     const int nkeys = M.Nz * M.Ny * M.Nx;
     SplineAdjTiles w = carve_tiles(ws, a.n, nkeys);
     const unsigned spotBlocks = (unsigned)((a.n + 255) / 256);
     const unsigned keyBlocks = (unsigned)std::min(1024, (nkeys + 255) / 256);
-    // counts and the two allocation counters start at zero (fill kernels, no memset:
-    // capturable)
-    hipLaunchKernelGGL(adj_fill_kernel, dim3(keyBlocks), dim3(256), 0, st, w.count, nkeys, 0);
-    hipLaunchKernelGGL(adj_fill_kernel, dim3(1), dim3(64), 0, st, w.ntiles, 2, 0);
-    hipLaunchKernelGGL(adj_key_kernel, dim3(spotBlocks), dim3(256), 0, st, M, a, w);
-    hipLaunchKernelGGL(adj_alloc_kernel, dim3(keyBlocks), dim3(256), 0, st, w);
-    hipLaunchKernelGGL(adj_scatter_kernel, dim3(spotBlocks), dim3(256), 0, st, a, w);
+    if (!sorted) {
+        // counts and the two allocation counters start at zero (fill kernels, no memset:
+        // capturable)
+        hipLaunchKernelGGL(adj_fill_kernel, dim3(keyBlocks), dim3(256), 0, st, w.count, nkeys, 0);
+        hipLaunchKernelGGL(adj_fill_kernel, dim3(1), dim3(64), 0, st, w.ntiles, 2, 0);
+        hipLaunchKernelGGL(adj_key_kernel, dim3(spotBlocks), dim3(256), 0, st, M, a, w);
+        hipLaunchKernelGGL(adj_alloc_kernel, dim3(keyBlocks), dim3(256), 0, st, w);
+        hipLaunchKernelGGL(adj_scatter_kernel, dim3(spotBlocks), dim3(256), 0, st, a, w);
+    }
     const bool grad = a.gshared || a.gparams;
     const size_t lds = round_lds_floats(a.R, grad) * sizeof(float);
     auto go = [&](auto kfn, int* cache) -> hipError_t {
@@ -649,10 +668,10 @@ static hipError_t launch_spline_tiled(const SplineModel& M, const AdjArgs& a, vo
                 : go(spline_adjoint_round_kernel<false>, &residentNll[a.R]);
 }
 
-hipError_t launch_adjoint_any(const ModelDesc& d, const AdjArgs& a, void* ws, hipStream_t st) {
+hipError_t launch_adjoint_any(const ModelDesc& d, const AdjArgs& a, void* ws, bool sorted, hipStream_t st) {
     if (d.kind == kKindSpline && ws && spline_adjoint_tiled(a.R)) {
         SplineModel M = d.sm;
-        return launch_spline_tiled(M, a, ws, st);
+        return launch_spline_tiled(M, a, ws, sorted, st);
     }
     const unsigned blocks = (unsigned)(((size_t)a.n * kWave + 255) / 256);
     switch (d.kind) {

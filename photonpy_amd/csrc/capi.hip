@@ -1022,12 +1022,21 @@ int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const fl
             a.gparams = d_grad_params;
             a.gshared = d_grad_shared;
             void* ws = nullptr;
+            bool sorted = false;
             if (e->desc.kind == kKindSpline && spline_adjoint_tiled(e->roisize)) {
                 const long long nkeys = (long long)e->desc.sm.Nz * e->desc.sm.Ny * e->desc.sm.Nx;
                 if (nkeys > (1LL << 30)) throw std::runtime_error("photonpy_amd: spline table too large");
                 ws = e->adjoint_workspace(spline_adjoint_workspace(n, (int)nkeys));
+                // a gradient call reuses the window sort of the previous call for as many
+                // spots on the same stream (per-spot results do not depend on the order)
+                sorted = (d_grad_params || d_grad_shared) && e->adjSortN == n && e->adjSortStream == stream;
+                e->adjSortN = -1;  // recorded again only once this call's launches succeeded
             }
-            hip_check(launch_adjoint_any(e->desc, a, ws, as_stream(stream)), "adjoint kernel");
+            hip_check(launch_adjoint_any(e->desc, a, ws, sorted, as_stream(stream)), "adjoint kernel");
+            if (ws) {
+                e->adjSortN = n;
+                e->adjSortStream = stream;
+            }
             return 0;
         },
         2);

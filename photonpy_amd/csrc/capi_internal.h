@@ -144,9 +144,15 @@ struct Estimator {
     // grow-only device scratch of the spline adjoint's window sort
     void* adjWs = nullptr;
     size_t adjWsBytes = 0;
+    // the window sort held in adjWs: its spot count (-1: none) and the stream it was made
+    // on; a gradient call for as many spots on that stream reuses it (the forward NLL
+    // call of an autograd step sorts, the backward of the same parameters reuses)
+    int adjSortN = -1;
+    void* adjSortStream = nullptr;
 
     void* adjoint_workspace(size_t bytes) {
         if (bytes > adjWsBytes) {
+            adjSortN = -1;
             if (adjWs) hip_check(dev_release(adjWs), "hipFree(adjoint scratch)");
             adjWs = nullptr;
             adjWsBytes = 0;

@@ -145,7 +145,11 @@ struct SplineAdjTiles {
 bool spline_adjoint_tiled(int R);
 size_t spline_adjoint_workspace(int n, int nkeys);
 // ws: spline_adjoint_workspace bytes of device scratch, or null (direct atomics).
-hipError_t launch_adjoint_any(const ModelDesc& d, const AdjArgs& a, void* ws, hipStream_t st);
+// sorted: ws already holds a window sort of a.n spots (made by an earlier call on the same
+// stream): it is used as it is.  Any order of the spots gives the same per-spot results (a
+// spot outside its round's window evaluates its own cells with the same arithmetic); a
+// sort of other parameters only costs speed.
+hipError_t launch_adjoint_any(const ModelDesc& d, const AdjArgs& a, void* ws, bool sorted, hipStream_t st);
 hipError_t launch_spline_coefs(const int* shape, const float* d_data, float* d_out, int border, hipStream_t st);
 // Sets the 2^shift work counters of a fit launch of `blocks` blocks on the stream
 // (a one-wave kernel, so it can also be captured in a graph).

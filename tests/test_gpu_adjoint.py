@@ -221,6 +221,41 @@ def test_spline_coefficient_gradients_many_spots(ctx, R):
     assert np.all(g[np.asarray(A.sum(axis=1)).ravel() == 0] == 0)  # untouched cells get nothing
 
 
+def test_spline_gradients_do_not_depend_on_the_window_sort(ctx):
+    """A gradient call reuses the window sort of the previous call for as many spots on
+    the same stream (the forward NLL of an autograd step sorts, its backward reuses).  With
+    the sort of other parameters (most spots then outside their round's window, evaluated
+    from their own cells) the per-spot gradients are bit-identical to those after a fresh
+    sort, and the coefficient gradient agrees to FP32 summation order."""
+    import torch
+    R, n = 20, 2000
+    rng = np.random.default_rng(77)
+    th_a = C.synth_params("spline", R, n, rng).astype(np.float32)
+    th_b = th_a.copy()
+    th_b[:, :3] = th_a[rng.permutation(n), :3]  # other windows for most spots
+    est, om, _ = MD.make("spline", ctx, R=R)
+    smp = rng.poisson(om.expected_value(th_b.astype(np.float64), np.float64)).astype(np.float32)
+    coefs, _, _ = MD.spline_coefs()
+    shared = torch.from_numpy(coefs.ravel().copy()).cuda()
+    d_a, d_b = torch.from_numpy(th_a).cuda(), torch.from_numpy(th_b).cuda()
+    d_smp = torch.from_numpy(smp).cuda()
+    gout = torch.from_numpy(np.linspace(0.5, 1.5, n).astype(np.float32)).cuda()
+    nll = torch.empty(n, device="cuda")
+
+    def grads():
+        gp, gs = torch.empty_like(d_b), torch.zeros_like(shared)
+        est._nll_grad(d_b, d_smp, shared, gout, None, gp, gs)
+        torch.cuda.synchronize()
+        return gp.cpu().numpy(), gs.cpu().numpy()
+
+    est._nll_grad(d_a, d_smp, shared, None, nll, None, None)  # sorts the windows of th_a
+    gp_stale, gs_stale = grads()                             # reuses that sort for th_b
+    est._nll_grad(d_b, d_smp, shared, None, nll, None, None)  # sorts the windows of th_b
+    gp_fresh, gs_fresh = grads()
+    assert np.array_equal(gp_stale, gp_fresh)
+    assert np.max(np.abs(gs_stale - gs_fresh)) <= 1e-5 * np.abs(gs_fresh).max()
+
+
 def test_calibration_step_reduces_nll(ctx):
     """A few gradient steps on sigma of the fixed-sigma model move it towards the truth."""
     import torch
