@@ -54,6 +54,9 @@ using namespace ppa;
 // way (smoothing splines) fail the check and keep the coefficient form.
 static bool spline_grid_from_coefs(int Nz, int Ny, int Nx, const float* c, std::vector<float>& grid) {
     const int D[3] = {Nz + 3, Ny + 3, Nx + 3}, N[3] = {Nz, Ny, Nx};
+    // the fit kernels address the table and its padding by 32-bit byte offsets
+    if (((size_t)D[0] * D[1] * D[2] + SplineGridModel::pad_floats(Nx)) * sizeof(float) >= ((size_t)1 << 31))
+        return false;
     grid.assign((size_t)D[0] * D[1] * D[2], 0.0f);
     auto cellOf = [&](int axis, int d, int& cell, double& t) {
         cell = std::min(std::max(d - 1, 0), N[axis] - 1);

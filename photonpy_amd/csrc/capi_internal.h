@@ -241,9 +241,12 @@ struct Estimator {
             desc.sm.coefs = devCoefs;
         }
         if (is_spline(desc.kind) && desc.useGrid && !devGrid) {
+            // the table, then the zeros the fit kernels' window loads may reach past it
             const size_t bytes = hostGrid.size() * sizeof(float);
-            hip_check(dev_alloc(&devGrid, bytes), "hipMalloc(spline grid)");
+            const size_t pad = SplineGridModel::pad_floats(desc.sg.Nx) * sizeof(float);
+            hip_check(dev_alloc(&devGrid, bytes + pad), "hipMalloc(spline grid)");
             hip_check(hipMemcpy(devGrid, hostGrid.data(), bytes, hipMemcpyHostToDevice), "upload spline grid");
+            hip_check(hipMemset(reinterpret_cast<char*>(devGrid) + bytes, 0, pad), "clear spline grid padding");
             desc.sg.grid = devGrid;
         }
     }

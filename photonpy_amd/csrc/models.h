@@ -1009,6 +1009,12 @@ struct SplineGridModelT : SplineCommon<BG> {
         return L ? (R * ((R + L - 1) / L) + G - 1) / G * L : (R * R + G - 1) / G;
     }
     static constexpr int lds_waves(int R, int G) { return 40960 / (4 * lds_floats(R, G) * (64 / G)); }
+    // Zero floats behind the device sample table that for_pixels' window loads may
+    // touch: a window of R + 4 samples at the last cell reaches R (Nx + 4) floats past
+    // the table's end, and a lane that shares its column with NPL - 1 others loads up
+    // to NPL - 1 <= 8 rows more (R <= kMaxWindowR).
+    static constexpr int kMaxWindowR = 32;
+    __host__ __device__ static size_t pad_floats(int Nx) { return (size_t)(kMaxWindowR + 9) * (size_t)(Nx + 4); }
     static constexpr int min_waves(int R, int G) {
         const int cap = strip_len(R, G) ? 4 : 5;
         const int w = lds_waves(R, G) < cap ? lds_waves(R, G) : cap;
@@ -1186,8 +1192,8 @@ struct SplineGridModelT : SplineCommon<BG> {
         // 19 + 2.9999990 = 22.0), so cells run from i0 to i0 + R and the 4x4 stencil to
         // i0 + R + 3: R + 4 samples.
         constexpr int W = R + 4;
-        constexpr int NE = W * W;         // window entries
-        constexpr int PE = (NE + G - 1) / G;
+        constexpr int NE = W * W;  // window entries
+        static_assert(R <= kMaxWindowR, "window loads past the table's padding");
         using L = Layout<R, G>;
         constexpr int LS = strip_len(R, G);
         // Pixel coordinates depend only on g and the slot; without this opaque copy
@@ -1198,7 +1204,6 @@ struct SplineGridModelT : SplineCommon<BG> {
         frame(t, R, F);
         const int i0 = cell_x(F, 0), j0 = cell_y(F, 0);
         const int Dy = Ny + 3, Dx = Nx + 3;
-        const float* slab = grid + (size_t)F.k * Dy * Dx;
         const size_t zs = (size_t)Dy * Dx;
         // (value, z-derivative) weight pairs: one 8-byte LDS read per stencil entry,
         // two entries per ds_read2_b64 at immediate offsets from one address; row
@@ -1222,21 +1227,73 @@ struct SplineGridModelT : SplineCommon<BG> {
             else if (u < 2 * R)
                 OY[u - R] = (Off)((cell_y(F, u - R) - j0) * S);
         }
-        // z-contraction of the window: rows j0.., columns i0.. (clamped to the grid;
-        // entries past the last cell are never read by a pixel).  Every read offset
-        // (cell - origin) * S + stencil stays inside the group's W x W window.  Two
-        // entries per trip keep 8 loads in flight per lane without holding the whole
-        // window.
-#pragma unroll 2
-        for (int u = 0; u < PE; u++) {
-            const int e = g + G * u;
-            if (NE % G == 0 || e < NE) {
-                const int r = e / W, c = e - (e / W) * W;
-                const int gy = min(j0 + r, Dy - 1), gx = min(i0 + c, Dx - 1);
-                const float* src = slab + (size_t)gy * Dx + gx;
-                const float v0 = src[0], v1 = src[zs], v2 = src[2 * zs], v3 = src[3 * zs];
-                T[r * S + c] = make_float2(v0 * F.wz[0] + v1 * F.wz[1] + v2 * F.wz[2] + v3 * F.wz[3],
-                                           v0 * F.wdz[0] + v1 * F.wdz[1] + v2 * F.wdz[2] + v3 * F.wdz[3]);
+        // z-contraction of the window: rows j0 .., columns i0 .. of slab k.  A lane takes
+        // one window column (a group narrower than the window: a second one; a group
+        // at least twice as wide: every NPL-th row of one) and walks its rows, so a
+        // row's and a z-plane's grid offset is the same for every lane: buffer loads take
+        // it as their scalar offset beside the lane's 32-bit byte offset, and an entry
+        // costs its 8 FMAs (with per-entry addresses it cost about 25 VALU instructions:
+        // 64-bit address arithmetic per load, the division by W, the clamps).  Nothing is
+        // clamped: an entry past the grid's last sample column or row holds whatever
+        // follows in memory (the next row, or the zero padding of pad_floats behind the
+        // table) and is never read, since a pixel's cell is clamped to the grid and its
+        // 4x4 stencil stays inside it; every read offset (cell - origin) * S + stencil
+        // stays inside the group's W x W window.
+        {
+            constexpr int NPL = G >= W ? G / W : 1;             // lanes per column
+            constexpr int NCOL = G >= W ? 1 : (W + G - 1) / G;  // columns per lane
+            constexpr int NROW = (W + NPL - 1) / NPL;           // rows per lane
+            constexpr int QB = 3;                               // rows per batch of loads
+            constexpr int NB = (NROW + QB - 1) / QB;
+            const int col = G >= W ? g % W : g, ph = G >= W ? g / W : 0;
+            const bool on = G < W || g < NPL * W;
+            const unsigned off = (unsigned)(((F.k * Dy + j0 + ph) * Dx + i0 + col) * 4);
+            // the table and its padding (under 2^31 bytes, checked where it is made)
+            const int nbytes = (int)((zs * (size_t)(Nz + 3) + pad_floats(Nx)) * 4);
+            const __amdgpu_buffer_rsrc_t rs =
+                __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(grid), (short)0, nbytes, 0x00020000);
+            // opaque per call: the row and plane offsets are then formed on the scalar unit
+            // beside their loads, not hoisted out of the fit loop into spilled SGPRs
+            unsigned zb = (unsigned)zs * 4, rb = (unsigned)(NPL * Dx * 4);
+            asm volatile("" : "+s"(zb), "+s"(rb));
+            float2* Tl = T + ph * S + col;
+#pragma unroll
+            for (int j = 0; j < NCOL; j++) {
+                if (on && (NCOL == 1 || col + G * j < W)) {
+                    // rows in batches of QB, the next batch's loads in flight while one is
+                    // contracted (a lane with NPL > 1 loads up to NPL - 1 rows past the
+                    // window, inside the padding, and stores only the window's rows)
+                    float v[2][QB][4];
+                    auto load = [&](int b) {
+#pragma unroll
+                        for (int u = 0; u < QB; u++) {
+                            const int q = b * QB + u;
+                            if (q < NROW) {
+#pragma unroll
+                                for (int m = 0; m < 4; m++)
+                                    v[b & 1][u][m] = __builtin_bit_cast(
+                                        float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                   rs, off, (unsigned)(m * zb + q * rb + G * j * 4), 0));
+                            }
+                        }
+                    };
+                    load(0);
+#pragma unroll
+                    for (int b = 0; b < NB; b++) {
+                        if (b + 1 < NB) load(b + 1);
+#pragma unroll
+                        for (int u = 0; u < QB; u++) {
+                            const int q = b * QB + u;
+                            if (q < NROW && (W % NPL == 0 || ph + q * NPL < W)) {
+                                const float* e = v[b & 1][u];
+                                Tl[q * NPL * S + G * j] =
+                                    make_float2(e[0] * F.wz[0] + e[1] * F.wz[1] + e[2] * F.wz[2] + e[3] * F.wz[3],
+                                                e[0] * F.wdz[0] + e[1] * F.wdz[1] + e[2] * F.wdz[2] + e[3] * F.wdz[3]);
+                            }
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                }
             }
         }
         // Strip layout: the strips walk the window one row per pixel, which holds when

@@ -426,3 +426,45 @@ def test_grid_spline_window_covers_rounded_cells(ctx):
     p1 = est.estimate_device(smp, crlb=False)["params"].cpu().numpy()
     p2 = est.estimate_device(smp, crlb=False)["params"].cpu().numpy()
     assert np.array_equal(p1, p2, equal_nan=True)
+
+
+@pytest.mark.parametrize("R,G", [(20, 32), (25, 64), (20, 64), (16, 16), (9, 16), (7, 8)])
+def test_grid_spline_window_at_grid_edges(ctx, R, G):
+    """Spots at the corners of the position range and at both ends of the z range.  Their
+    LDS windows run past the grid's last sample rows and columns (on the last z-slab, into
+    the zero padding behind the table), entries that no pixel reads; the lanes fill the
+    window by columns, one lane per column (20/32, 9/16), two with an odd or even window
+    width (25/64, 20/64) or two columns per lane (16/16, 7/8).  The fused CRLB (pixels
+    evaluated from the window) agrees with the stand-alone CRLB kernel (pixels evaluated
+    from the grid in HBM)."""
+    import torch
+    est, _, _ = MD.make("spline", ctx, R=R)
+    assert est.spline_form() == "grid"
+    est.SetGroupSize(G)
+    lo, hi = est.ParamLimits()
+    rng = np.random.default_rng(43)
+    n = 8192
+    xy = np.array([lo[0], lo[0] + 0.1, lo[0] + 0.6, hi[0] - 0.6, hi[0] - 0.1, hi[0]], np.float32)
+    dz = hi[2] - lo[2]
+    zs = np.array([lo[2] + 1e-3, lo[2] + 0.02 * dz, 0.5 * (lo[2] + hi[2]), hi[2] - 0.02 * dz, hi[2] - 1e-3],
+                  np.float32)
+    th = np.zeros((n, 5), np.float32)
+    th[:, 0] = rng.choice(xy, n)
+    th[:, 1] = rng.choice(xy, n)
+    th[:, 2] = rng.choice(zs, n)
+    th[:, 3] = rng.uniform(1000, 5000, n)
+    th[:, 4] = rng.choice(np.array([1.0, 5.0], np.float32), n)
+    d_th = torch.from_numpy(th).cuda()
+    smp = torch.poisson(est.expected_value_device(d_th))
+    est.SetLevMarParams(1e-18, 0)
+    fused = est.estimate_device(smp, initial=d_th, crlb=True)
+    c, x = est.crlb_device(d_th, smp)
+    torch.cuda.synchronize()
+    a = torch.cat([fused["crlb"], fused["chisq"][:, None]], 1).cpu().numpy()
+    b = torch.cat([c, x[:, None]], 1).cpu().numpy()
+    fin = np.all(np.isfinite(b), axis=1)
+    assert fin.mean() > 0.9
+    # summation order alone (the two kernels differ in it) moves a well-conditioned CRLB
+    # by ~1e-6 relative; a wrong window entry by far more
+    close = np.all(np.isclose(a, b, rtol=1e-3, atol=1e-6), axis=1)
+    assert close[fin].all(), f"{(~close[fin]).sum()} of {fin.sum()} spots differ, e.g. {th[fin][~close[fin]][:3]}"
