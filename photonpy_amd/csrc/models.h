@@ -1355,6 +1355,12 @@ struct SplineGridModelT : SplineCommon<BG> {
                 }
                 float mu, J[K];
                 finish(t, f, fx, fy, fz, x, y, R, mu, J);
+                // the pixel's value and derivatives are formed here: without this (an
+                // empty asm, no instruction) the compiler sinks every pixel's arithmetic
+                // below the LDS reads of all the lane's pixels, which at 28 or more slots
+                // held 16 x 28 pairs at once (20x20 at 16 lanes: 512 registers and 1.4 KB
+                // of scratch per lane; 191 registers with it)
+                asm volatile("" : "+v"(f), "+v"(fx), "+v"(fy), "+v"(fz) :: "memory");
                 fn(s, mu, J);
             }
             // one pixel at a time: without this the scheduler hoists the LDS reads of
