@@ -31,9 +31,9 @@ COMPILED_GROUPS = {3: [1], 4: [1], 5: [2], 6: [2], 7: [1, 2, 4], 8: [2], 9: [2, 
                    30: [16], 31: [16], 32: [16]}
 # grid-form spline (PPA_GRID_SIZES)
 GRID_GROUPS = {3: [4], 4: [4], 5: [4], 6: [4], 7: [4, 8], 8: [8], 9: [8, 16], 10: [8], 11: [8, 16], 12: [8],
-               13: [8, 16], 14: [16], 15: [16], 16: [16, 32], 17: [16], 18: [16], 19: [16], 20: [16, 32, 64],
-               21: [16], 22: [16], 23: [16], 24: [16], 25: [16, 32, 64], 26: [16], 27: [16], 28: [16], 29: [16],
-               30: [16], 31: [16], 32: [16]}
+               13: [8, 16], 14: [16], 15: [16], 16: [16, 32], 17: [16], 18: [32], 19: [32], 20: [16, 32, 64],
+               21: [32], 22: [32], 23: [64], 24: [64], 25: [16, 32, 64], 26: [64], 27: [64], 28: [64], 29: [64],
+               30: [64], 31: [64], 32: [64]}
 
 
 def key(est):

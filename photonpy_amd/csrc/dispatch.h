@@ -71,7 +71,9 @@ inline int default_group(int R) {
 inline int default_group_grid(int R) {
     if (R <= 6) return 4;
     if (R <= 12) return 8;
-    return 16;
+    if (R <= 17) return 16;
+    if (R <= 22) return 32;
+    return 64;
 }
 
 inline int default_group_for(const ModelDesc& d, int R) {

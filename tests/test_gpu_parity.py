@@ -428,7 +428,8 @@ def test_grid_spline_window_covers_rounded_cells(ctx):
     assert np.array_equal(p1, p2, equal_nan=True)
 
 
-@pytest.mark.parametrize("R,G", [(20, 32), (25, 64), (20, 64), (16, 16), (9, 16), (7, 8)])
+@pytest.mark.parametrize("R,G", [(20, 32), (25, 64), (20, 64), (16, 16), (9, 16), (7, 8), (22, 32), (28, 64),
+                                 (32, 64)])
 def test_grid_spline_window_at_grid_edges(ctx, R, G):
     """Spots at the corners of the position range and at both ends of the z range.  Their
     LDS windows run past the grid's last sample rows and columns (on the last z-slab, into
