@@ -1094,7 +1094,9 @@ struct SplineGridModelT : SplineCommon<BG> {
         J[0] = fx;  // -I folded into the x/y derivative weights, I / du into the z ones
         J[1] = fy;
         J[2] = fz;
-        const float psf = f < 0.0f ? 0.0f : f;
+        // max(f, 0) with NaN passed through, as f < 0 ? 0 : f, in one instruction
+        // (v_maximum_f32); it gives +0 for f = -0, which no sum below can tell apart
+        const float psf = __builtin_elementwise_maximum(f, 0.0f);
         J[3] = psf;
         Base::background(t, t[iI], psf, x, y, R, mu, J);
     }
@@ -1145,10 +1147,18 @@ struct SplineGridModelT : SplineCommon<BG> {
             const bool live = SL::NP % G == 0 || SL::M * G == SL::NP || p < SL::NP;
             if (!live) p = SL::NP - 1;  // padding lanes walk a real strip and use nothing
             const int c = p % R, y0 = (p / R) * LS;
-            const float2* B = T + y0 * S + (int)OX[c];
+            // rows r .. r + 3 from one base (ds_read2_b64 reaches 255 x 8 bytes past its
+            // address; four rows of stride S <= 52 stay within it), advanced every four
+            // rows through an opaque copy, which the compiler would otherwise fold into
+            // per-row address additions
+            int ob = y0 * S + (int)OX[c];
             float a[4], ad[4], az[4];
             auto row = [&](int r) {
-                const float2* e = B + r * S;
+                if (r > 0 && (r & 3) == 0) {
+                    ob += 4 * S;
+                    asm volatile("" : "+v"(ob));
+                }
+                const float2* e = T + ob + (r & 3) * S;
                 const float2 e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3];
                 const int k = r & 3;
                 a[k] = F.wx[0] * e0.x + F.wx[1] * e1.x + F.wx[2] * e2.x + F.wx[3] * e3.x;
