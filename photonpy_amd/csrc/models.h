@@ -1153,24 +1153,33 @@ struct SplineGridModelT : SplineCommon<BG> {
             // per-row address additions
             int ob = y0 * S + (int)OX[c];
             float a[4], ad[4], az[4];
-            auto row = [&](int r) {
+            // row r's four (value, z-derivative) entries, read one pixel ahead of their
+            // x-contraction so that the LDS latency overlaps the pixel before
+            float2 e0, e1, e2, e3;
+            auto load = [&](int r) {
                 if (r > 0 && (r & 3) == 0) {
                     ob += 4 * S;
                     asm volatile("" : "+v"(ob));
                 }
                 const float2* e = T + ob + (r & 3) * S;
-                const float2 e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3];
+                e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3];
+            };
+            auto contract = [&](int r) {
                 const int k = r & 3;
                 a[k] = F.wx[0] * e0.x + F.wx[1] * e1.x + F.wx[2] * e2.x + F.wx[3] * e3.x;
                 ad[k] = F.wdx[0] * e0.x + F.wdx[1] * e1.x + F.wdx[2] * e2.x + F.wdx[3] * e3.x;
                 az[k] = F.wx[0] * e0.y + F.wx[1] * e1.y + F.wx[2] * e2.y + F.wx[3] * e3.y;
             };
-            row(0);
-            row(1);
-            row(2);
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                load(r);
+                contract(r);
+            }
+            load(3);
 #pragma unroll
             for (int i = 0; i < LS; i++) {
-                row(i + 3);
+                contract(i + 3);
+                if (i + 1 < LS) load(i + 4);
                 const int y = y0 + i;
                 if (live && (SL::P * LS == R || y < R)) {
                     float f = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
