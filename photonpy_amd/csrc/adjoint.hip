@@ -197,16 +197,9 @@ PP_DEV void spline_cell_by_z(const float* c, const SplineModel::Frame& F, float&
     f = fx = fy = fz = 0.0f;
 #pragma unroll UNROLL_M
     for (int m = 0; m < 4; m++) {
-        float a = 0.0f, ax = 0.0f, ay = 0.0f;
-#pragma unroll
-        for (int n = 0; n < 4; n++) {
-            const float4 q = c4[m * 4 + n];
-            const float px = q.x + q.y * F.x1 + q.z * F.x2 + q.w * F.x3;
-            const float dpx = q.y + q.z * F.dx2 + q.w * F.dx3;
-            a += px * F.yp[n];
-            ax += dpx * F.yp[n];
-            ay += px * F.dyp[n];
-        }
+        float a, ax, ay;
+        const float4 q[4] = {c4[m * 4], c4[m * 4 + 1], c4[m * 4 + 2], c4[m * 4 + 3]};
+        SplineModel::cell_m(q, F, a, ax, ay);
         const float zm = m == 1 ? z : m == 2 ? z2 : z3;
         const float dzm = m == 2 ? dz2 : dz3;
         if (m == 0) {
@@ -371,19 +364,22 @@ __global__ void __launch_bounds__(256) adj_scatter_kernel(AdjArgs a, SplineAdjTi
 // UNROLL_M z-powers of 16 coefficients at a time (4: all 64 loads in flight; 1: 16
 // registers of coefficients, for the rarely taken per-lane cell of the round kernels,
 // which would otherwise set their register peak).
+// One z-power's part of the value: its 16 coefficients contracted with the x and y powers.
+PP_DEV float spline_value_m(const float4 (&q)[4], const SplineModel::Frame& F) {
+    float a = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 4; n++) a += (q[n].x + q[n].y * F.x1 + q[n].z * F.x2 + q[n].w * F.x3) * F.yp[n];
+    return a;
+}
+
 template <int UNROLL_M = 4>
 PP_DEV float spline_cell_value(const float* c, const SplineModel::Frame& F) {
     const float4* c4 = reinterpret_cast<const float4*>(c);
     float f = 0.0f, zm = 1.0f;
 #pragma unroll UNROLL_M
     for (int m = 0; m < 4; m++) {
-        float a = 0.0f;
-#pragma unroll
-        for (int n = 0; n < 4; n++) {
-            const float4 q = c4[m * 4 + n];
-            a += (q.x + q.y * F.x1 + q.z * F.x2 + q.w * F.x3) * F.yp[n];
-        }
-        f += a * zm;
+        const float4 q[4] = {c4[m * 4], c4[m * 4 + 1], c4[m * 4 + 2], c4[m * 4 + 3]};
+        f += spline_value_m(q, F) * zm;
         zm *= F.zp[1];
     }
     return f;
@@ -514,6 +510,18 @@ __global__ void __launch_bounds__(kRoundBlock) spline_adjoint_round_kernel(Splin
             const float* rowu = cf + 64 * ((size_t)(k0 * M.Ny + cyu) * M.Nx);
             const int cy = clampi((int)((float)y - F.ty), 0, M.Ny - 1);
             const float* rowo = cf + 64 * ((size_t)(ks * M.Ny + cy) * M.Nx);
+            // the shared cell's coefficients arrive by scalar loads one z-power (16 floats)
+            // at a time, and the next pixel's first two z-powers are issued as soon as this
+            // pixel's last two are, so that their latency overlaps the rest of this pixel
+            // instead of stalling the next one (the last pixel of a row loads a clamped,
+            // unused cell); the arithmetic is SplineModel::cell's or spline_cell_value's
+            float4 qa[4], qb[4];
+            auto ld4 = [&](const float* c, int m, float4 (&q)[4]) {
+                const float4* c4 = reinterpret_cast<const float4*>(c) + 4 * m;
+                q[0] = c4[0], q[1] = c4[1], q[2] = c4[2], q[3] = c4[3];
+            };
+            ld4(rowu + 64 * min(i0, M.Nx - 1), 0, qa);
+            ld4(rowu + 64 * min(i0, M.Nx - 1), 1, qb);
             for (int x = 0; x < R; x++) {
                 const int cxu = min(i0 + x, M.Nx - 1);
                 const int cx = M.col_index(F, x);
@@ -525,7 +533,21 @@ __global__ void __launch_bounds__(kRoundBlock) spline_adjoint_round_kernel(Splin
                     // the shared cell by scalar loads, a lane's own one (rare) 16 coefficients
                     // at a time: the same arithmetic in the same order, so a spot's results do
                     // not depend on which round it falls in
-                    SplineModel::cell(rowu + 64 * cxu, F, f, fx, fy, fz);
+                    {
+                        const float* cu = rowu + 64 * cxu;
+                        const float* cn = rowu + 64 * min(i0 + x + 1, M.Nx - 1);
+                        float fm[4], fxm[4], fym[4];
+                        float4 qc[4], qd[4];
+                        ld4(cu, 2, qc);
+                        SplineModel::cell_m(qa, F, fm[0], fxm[0], fym[0]);
+                        ld4(cu, 3, qd);
+                        SplineModel::cell_m(qb, F, fm[1], fxm[1], fym[1]);
+                        ld4(cn, 0, qa);
+                        SplineModel::cell_m(qc, F, fm[2], fxm[2], fym[2]);
+                        ld4(cn, 1, qb);
+                        SplineModel::cell_m(qd, F, fm[3], fxm[3], fym[3]);
+                        SplineModel::cell_combine(fm, fxm, fym, F, f, fx, fy, fz);
+                    }
                     if (!own) spline_cell_by_z<1>(rowo + 64 * cx, F, f, fx, fy, fz);
                     SplineModel::finish(F, t, f, fx, fy, fz, x, y, R, mu, J);
                     if (f < 0.0f) J[0] = J[1] = J[2] = 0.0f;  // mu does not depend on position where clamped
@@ -545,7 +567,25 @@ __global__ void __launch_bounds__(kRoundBlock) spline_adjoint_round_kernel(Splin
                         atomicAdd(&a.gshared[(size_t)cs * 64 + lane], g * b);
                     }
                 } else {
-                    float f = spline_cell_value(rowu + 64 * cxu, F);  // scalar loads
+                    float f;
+                    {
+                        const float* cu = rowu + 64 * cxu;
+                        const float* cn = rowu + 64 * min(i0 + x + 1, M.Nx - 1);
+                        float4 qc[4], qd[4];
+                        ld4(cu, 2, qc);
+                        float zm = 1.0f;
+                        f = 0.0f;
+                        f += spline_value_m(qa, F) * zm;
+                        zm *= F.zp[1];
+                        ld4(cu, 3, qd);
+                        f += spline_value_m(qb, F) * zm;
+                        zm *= F.zp[1];
+                        ld4(cn, 0, qa);
+                        f += spline_value_m(qc, F) * zm;
+                        zm *= F.zp[1];
+                        ld4(cn, 1, qb);
+                        f += spline_value_m(qd, F) * zm;
+                    }
                     if (!own) f = spline_cell_value<1>(rowo + 64 * cx, F);
                     const float psf = f < 0.0f ? 0.0f : f;
                     nll_weight(t[3] * psf + t[4], xs, sums[K]);

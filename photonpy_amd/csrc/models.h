@@ -864,27 +864,39 @@ struct SplineModelT : SplineCommon<BG> {
     }
 
     // value and x/y/z derivatives of the spline cell c at the frame's sub-voxel position
+    // One z-power of a cell: its 16 coefficients q (4 rows of 4 x-powers) contracted with
+    // the x and y powers into the value a and the x / y derivatives ax, ay.
+    PP_DEV static void cell_m(const float4 (&q)[4], const Frame& F, float& a, float& ax, float& ay) {
+        a = 0.0f, ax = 0.0f, ay = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const float px = q[n].x + q[n].y * F.x1 + q[n].z * F.x2 + q[n].w * F.x3;
+            const float dpx = q[n].y + q[n].z * F.dx2 + q[n].w * F.dx3;
+            a += px * F.yp[n];
+            ax += dpx * F.yp[n];
+            ay += px * F.dyp[n];
+        }
+    }
+
+    // The four z-powers' parts combined: f = fm0 + fm1 z + fm2 z^2 + fm3 z^3, its x / y
+    // derivatives alike, fz = fm1 + fm2 2z + fm3 3z^2.
+    PP_DEV static void cell_combine(const float (&fm)[4], const float (&fxm)[4], const float (&fym)[4],
+                                    const Frame& F, float& f, float& fx, float& fy, float& fz) {
+        f = fm[0] + fm[1] * F.zp[1] + fm[2] * F.zp[2] + fm[3] * F.zp[3];
+        fx = fxm[0] + fxm[1] * F.zp[1] + fxm[2] * F.zp[2] + fxm[3] * F.zp[3];
+        fy = fym[0] + fym[1] * F.zp[1] + fym[2] * F.zp[2] + fym[3] * F.zp[3];
+        fz = fm[1] + fm[2] * F.dzp[2] + fm[3] * F.dzp[3];
+    }
+
     PP_DEV static void cell(const float* c, const Frame& F, float& f, float& fx, float& fy, float& fz) {
         const float4* c4 = reinterpret_cast<const float4*>(c);
         float fm[4], fxm[4], fym[4];
 #pragma unroll
         for (int m = 0; m < 4; m++) {
-            float a = 0.0f, ax = 0.0f, ay = 0.0f;
-#pragma unroll
-            for (int n = 0; n < 4; n++) {
-                const float4 q = c4[m * 4 + n];
-                const float px = q.x + q.y * F.x1 + q.z * F.x2 + q.w * F.x3;
-                const float dpx = q.y + q.z * F.dx2 + q.w * F.dx3;
-                a += px * F.yp[n];
-                ax += dpx * F.yp[n];
-                ay += px * F.dyp[n];
-            }
-            fm[m] = a; fxm[m] = ax; fym[m] = ay;
+            const float4 q[4] = {c4[m * 4], c4[m * 4 + 1], c4[m * 4 + 2], c4[m * 4 + 3]};
+            cell_m(q, F, fm[m], fxm[m], fym[m]);
         }
-        f = fm[0] + fm[1] * F.zp[1] + fm[2] * F.zp[2] + fm[3] * F.zp[3];
-        fx = fxm[0] + fxm[1] * F.zp[1] + fxm[2] * F.zp[2] + fxm[3] * F.zp[3];
-        fy = fym[0] + fym[1] * F.zp[1] + fym[2] * F.zp[2] + fym[3] * F.zp[3];
-        fz = fm[1] + fm[2] * F.dzp[2] + fm[3] * F.dzp[3];
+        cell_combine(fm, fxm, fym, F, f, fx, fy, fz);
     }
 
     PP_DEV const float* row_ptr(const Frame& F, int y) const {
