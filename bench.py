@@ -38,13 +38,13 @@ PEAK_FP32 = 157.3  # TFLOP/s, MI355X FP32 vector peak
 # bound is the FP32 VALU roofline, labelled "valu".
 BOUND = "valu"
 COMPUTE_NOTE = "FP32 VALU roofline (no MFMA is issued); packed FP32 disabled"
-# Algorithmic FLOP per ROI of the spline adjoint's backward pass, per pixel of each spot:
+# Algorithmic FLOP per ROI of the spline adjoint pass (NLL and gradients), per pixel of each spot:
 # the spline value and x/y/z derivatives from the cell's 64 coefficients (139 FMA), mu, the
 # NLL weight and the 5 parameter-gradient terms (~22), and the spot's share of its cell's
 # coefficient gradient, sum_s W_s B_s[q] over the 64 coefficients (64 FMA): about 435 FLOP
 # per pixel, 174,000 per 20x20 ROI.
 ADJ_FLOP_PER_ROI = 174_000
-# ROI gradient evaluations per 65,536-ROI step: 20x20 px x 64 coefficients each.
+# Coefficient atomics per ROI of the direct (untiled) form: 20x20 px x 64 coefficients.
 ADJ_DIRECT_ATOMICS_PER_ROI = 400 * 64
 
 # ---------------------------------------------------------------------------
@@ -493,9 +493,11 @@ def spline_adjoint_rounds(th, R, shape, zmin, zmax):
 def bench_adjoint(steps, warmup, n=0):
     """Spline PSF calibration step: forward Poisson NLL + hand-written adjoint on 65,536
     20x20 ROIs, gradients w.r.t. per-spot parameters and all 2,638,976 spline coefficients
-    (mu and J recomputed in the backward pass, nothing stored).  The timed steps carry no
-    events; a second pass of the same steps times forward and backward with HIP events on
-    the launch stream."""
+    (mu and J recomputed from the parameters, nothing stored).  With inputs that need
+    gradients the forward runs the adjoint once for unit upstream weights, yielding the
+    NLL too, and the backward scales its gradients by the .sum()'s upstream gradient.
+    The timed steps carry no events; a second pass of the same steps times forward and
+    backward with HIP events on the launch stream."""
     import torch
     import photonpy_amd as pa
     from photonpy_amd import synthetic
@@ -542,12 +544,15 @@ def bench_adjoint(steps, warmup, n=0):
         del smp, params, coefs
     torch.cuda.empty_cache()
     ms = wall * 1e3 / steps
-    achieved = n * ADJ_FLOP_PER_ROI / (bwd_ms * 1e-3) / 1e12
+    # the adjoint pass (NLL, parameter and coefficient gradients for unit upstream weights)
+    # runs in the forward; the backward scales its results
+    achieved = n * ADJ_FLOP_PER_ROI / (fwd_ms * 1e-3) / 1e12
     atomics_tiled = rounds * R * R * 64  # at most one 64-lane atomic per pixel cell and round
-    roof = {"kernel": "backward pass (spline_adjoint_round_kernel<true>, reusing the forward's window sort)", "bound": BOUND,
+    roof = {"kernel": "forward pass (window sort + spline_adjoint_round_kernel<true>: NLL and gradients)",
+            "bound": BOUND,
             "compute": COMPUTE_NOTE, "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32, "traffic": None}
-    detail = {"forward_ms": fwd_ms, "backward_ms": bwd_ms, "backward_algorithmic_flop": n * ADJ_FLOP_PER_ROI,
+    detail = {"forward_ms": fwd_ms, "backward_ms": bwd_ms, "adjoint_algorithmic_flop": n * ADJ_FLOP_PER_ROI,
               "windows": windows, "rounds": rounds,
               "coefficient_atomics_per_step": atomics_tiled,
               "coefficient_atomics_per_step_direct_form": n * ADJ_DIRECT_ATOMICS_PER_ROI,

@@ -308,8 +308,11 @@ int Estimator_ChiSquareAndCRLBDevice(Estimator* e, int n, const float* d_params,
  *   spline the coefficient table [Nz,Ny,Nx,64]; free sigma: none.
  * d_gout [n]: upstream gradient of each spot's NLL (NULL = 1).  Outputs (each nullable):
  * d_nll [n], d_grad_params [n,K] (true derivatives), d_grad_shared (accumulated with +=).
- * The spline coefficient gradient sorts the spots by spline window in a scratch buffer the
- * estimator keeps (grown to the largest n seen). */
+ * The spline forms of R <= 28 sort the spots by spline window in a scratch buffer the
+ * estimator keeps (grown to the largest n seen); a gradient call with d_gout (a backward pass)
+ * reuses the sort of the previous call for as many spots on the same stream (its forward), a
+ * call without d_gout sorts (any order gives the same per-spot results; a sort of other
+ * parameters only costs speed). */
 int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const float* d_sample, const float* d_shared,
                             const float* d_gout, float* d_nll, float* d_grad_params, float* d_grad_shared,
                             void* stream);

@@ -7,8 +7,17 @@ estimator's PSF model.  ``params`` [N,K] are per-spot parameters, ``shared``
 the PSF parameters shared by all spots (``Estimator.shared_parameters()``):
 sigma_x/sigma_y (fixed-sigma Gaussian), {s0, gamma, d, A, B} per axis
 (astigmatic Gaussian) or the [Nz,Ny,Nx,64] coefficient table (cubic spline).
-The backward pass is a hand-written HIP adjoint kernel that recomputes the
-expected images and Jacobians from the parameters instead of storing them.
+The adjoint is a hand-written HIP kernel that recomputes the expected images and
+Jacobians from the parameters instead of storing them.  When an input needs a
+gradient, the forward pass runs it once with unit upstream weights, which also
+yields the NLL, and keeps the resulting gradients (n x K, and the shared
+gradient); the backward pass scales them by the upstream gradient.  A spot's
+parameter gradient is then the kernel's own g * sums (bit for bit where the kernel
+sums in FP32, the spline forms; within one FP32 rounding where it sums in FP64).
+The shared gradient is scaled as a whole when the upstream gradient is one value
+broadcast over the spots (a stride-0 tensor, what ``.sum()`` passes back), and
+recomputed with the per-spot weights otherwise (no host synchronisation either
+way).
 """
 from __future__ import annotations
 
@@ -31,23 +40,34 @@ class PoissonNLL(torch.autograd.Function):
             if shared.numel() != estimator.num_shared():
                 raise ValueError(f"shared must hold {estimator.num_shared()} values")
         nll = torch.empty(n, dtype=torch.float32, device=params.device)
-        estimator._nll_grad(params, samples, shared, None, nll, None, None)
+        need_p = ctx.needs_input_grad[1]
+        need_s = shared is not None and ctx.needs_input_grad[2]
+        # gradients for unit upstream weights, from the same pass as the NLL
+        gp1 = torch.empty_like(params) if need_p else None
+        gs1 = torch.zeros_like(shared) if need_s else None
+        estimator._nll_grad(params, samples, shared, None, nll, gp1, gs1)
+        empty = torch.empty(0, device=params.device)
         ctx.estimator = estimator
-        ctx.save_for_backward(params, shared if shared is not None else torch.empty(0, device=params.device), samples)
+        ctx.save_for_backward(params, shared if shared is not None else empty, samples,
+                              gp1 if need_p else empty, gs1 if need_s else empty)
         ctx.has_shared = shared is not None
         return nll
 
     @staticmethod
     def backward(ctx, gnll):
-        params, shared, samples = ctx.saved_tensors
+        params, shared, samples, gp1, gs1 = ctx.saved_tensors
         est = ctx.estimator
         shared = shared if ctx.has_shared else None
-        gnll = gnll.contiguous().float()
-        gp = torch.empty_like(params) if ctx.needs_input_grad[1] else None
-        gs = None
+        gnll = gnll.float()
+        gp = gs = None
+        if ctx.needs_input_grad[1]:
+            gp = gp1 * gnll.reshape(-1, 1)
         if shared is not None and ctx.needs_input_grad[2]:
-            gs = torch.zeros_like(shared)
-        est._nll_grad(params, samples, shared, gnll, None, gp, gs)
+            if gnll.dim() == 1 and gnll.stride(0) == 0:  # one weight for every spot
+                gs = gs1 * gnll[0]
+            else:
+                gs = torch.zeros_like(shared)
+                est._nll_grad(params, samples, shared, gnll.contiguous(), None, None, gs)
         return None, gp, gs, None
 
 

@@ -1030,9 +1030,12 @@ int Estimator_NLLGradDevice(Estimator* e, int n, const float* d_params, const fl
                 const long long nkeys = (long long)e->desc.sm.Nz * e->desc.sm.Ny * e->desc.sm.Nx;
                 if (nkeys > (1LL << 30)) throw std::runtime_error("photonpy_amd: spline table too large");
                 ws = e->adjoint_workspace(spline_adjoint_workspace(n, (int)nkeys));
-                // a gradient call reuses the window sort of the previous call for as many
-                // spots on the same stream (per-spot results do not depend on the order)
-                sorted = (d_grad_params || d_grad_shared) && e->adjSortN == n && e->adjSortStream == stream;
+                // a gradient call with upstream weights (the backward of an autograd step)
+                // reuses the window sort of the previous call for as many spots on the same
+                // stream (its forward); a call without them sorts (per-spot results do not
+                // depend on the order, a stale sort only costs speed)
+                sorted = (d_grad_params || d_grad_shared) && d_gout && e->adjSortN == n &&
+                         e->adjSortStream == stream;
                 e->adjSortN = -1;  // recorded again only once this call's launches succeeded
             }
             hip_check(launch_adjoint_any(e->desc, a, ws, sorted, as_stream(stream)), "adjoint kernel");

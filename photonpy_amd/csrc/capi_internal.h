@@ -145,8 +145,9 @@ struct Estimator {
     void* adjWs = nullptr;
     size_t adjWsBytes = 0;
     // the window sort held in adjWs: its spot count (-1: none) and the stream it was made
-    // on; a gradient call for as many spots on that stream reuses it (the forward NLL
-    // call of an autograd step sorts, the backward of the same parameters reuses)
+    // on; a gradient call with upstream weights for as many spots on that stream reuses
+    // it (the forward of an autograd step sorts, the backward of the same parameters
+    // reuses)
     int adjSortN = -1;
     void* adjSortStream = nullptr;
 

@@ -222,8 +222,9 @@ def test_spline_coefficient_gradients_many_spots(ctx, R):
 
 
 def test_spline_gradients_do_not_depend_on_the_window_sort(ctx):
-    """A gradient call reuses the window sort of the previous call for as many spots on
-    the same stream (the forward NLL of an autograd step sorts, its backward reuses).  With
+    """A gradient call with upstream weights reuses the window sort of the previous call
+    for as many spots on the same stream (an autograd step's forward sorts, its backward
+    reuses).  With
     the sort of other parameters (most spots then outside their round's window, evaluated
     from their own cells) the per-spot gradients are bit-identical to those after a fresh
     sort, and the coefficient gradient agrees to FP32 summation order."""
@@ -254,6 +255,41 @@ def test_spline_gradients_do_not_depend_on_the_window_sort(ctx):
     gp_fresh, gs_fresh = grads()
     assert np.array_equal(gp_stale, gp_fresh)
     assert np.max(np.abs(gs_stale - gs_fresh)) <= 1e-5 * np.abs(gs_fresh).max()
+
+
+@pytest.mark.parametrize("kind", ["spline", "astig"])
+def test_autograd_broadcast_weight_matches_direct_kernel(ctx, kind):
+    """The forward runs the adjoint once with unit weights (it also yields the NLL) and
+    the backward scales its gradients.  For one upstream weight broadcast over the spots
+    (.sum() passes a stride-0 gradient) the results equal one direct adjoint pass with
+    that weight per spot: the per-spot gradients bit for bit (0.5 is a power of two, so
+    scaling before or after the sums gives the same bits), the shared gradient to the
+    order of its atomic sums over spots, which differs from run to run.  The NLL equals
+    an NLL-only pass."""
+    import torch
+    R = 20 if kind == "spline" else 16
+    n = 1500
+    rng = np.random.default_rng(91)
+    th = C.synth_params(kind, R, n, rng)
+    est, om, _ = MD.make(kind, ctx, R=R)
+    smp = rng.poisson(om.expected_value(th, np.float64)).astype(np.float32)
+    th = th.astype(np.float32)
+    th[:, 0] += 0.1
+    shared0 = est.shared_parameters().detach().clone()
+    shared = shared0.clone().requires_grad_(True)
+    d_th = torch.from_numpy(th).cuda().requires_grad_(True)
+    d_smp = torch.from_numpy(smp).cuda()
+    nll = poisson_nll(est, d_th, d_smp, shared)
+    (nll.sum() * 0.5).backward()
+    gp, gs = torch.empty_like(d_th), torch.zeros_like(shared0)
+    nll_only = torch.empty(n, device="cuda")
+    est._nll_grad(d_th.detach(), d_smp, shared0, None, nll_only, None, None)
+    est._nll_grad(d_th.detach(), d_smp, shared0, torch.full((n,), 0.5, device="cuda"), None, gp, gs)
+    torch.cuda.synchronize()
+    assert np.array_equal(nll.detach().cpu().numpy(), nll_only.cpu().numpy())
+    assert np.array_equal(d_th.grad.cpu().numpy(), gp.cpu().numpy())
+    a, b = shared.grad.cpu().numpy(), gs.cpu().numpy()
+    assert np.max(np.abs(a - b)) <= 1e-5 * np.abs(b).max()
 
 
 def test_calibration_step_reduces_nll(ctx):
