@@ -48,11 +48,11 @@ ADJ_FLOP_PER_ROI = 174_000
 ADJ_DIRECT_ATOMICS_PER_ROI = 400 * 64
 
 # ---------------------------------------------------------------------------
-# Algorithmic FLOP count per localization (FP32 model + normal equations, FP64
-# solve and inversion counted as FLOPs too; erf/exp evaluations not counted).
+# Algorithmic FLOP count per localization (FP32 model, normal equations and damped
+# solve, the FP64 CRLB inversion counted as FLOPs too; erf/exp evaluations not counted).
 #   per pixel-iteration   K^2 + 4K + 4 (normal equations) + model terms
 #   per row/column-iter   row/column terms of the separable Gaussians
-#   per iteration         FP64 damping + Cholesky + solve: 4K^2 + 4K + 2K^3/3
+#   per iteration         damping + Cholesky + solve: 4K^2 + 4K + 2K^3/3
 #   initial estimate      3S (centre of mass)
 #   CRLB kernel           S (model + K^2 + 2K + 3) + (W+H) rowcol + 8K^3/3
 MODEL_TERMS = {"fixed": 5, "sigma": 8, "astig": 8, "spline": 422, "splinegrid": 136}

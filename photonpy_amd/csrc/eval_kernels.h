@@ -115,16 +115,16 @@ __global__ void __launch_bounds__(256) fit_large_kernel(M model, FitArgs a, int 
                 for (int i = 0; i < K; i++) acc[NA + i] += bf * J[i];
             }
             group_sum_n<kWave>(acc);
-            double step[K];
+            float step[K];
             bool stop;
-            if (!lm_step_f64<K>(acc, acc + NA, a.lambda, step)) {
+            if (!lm_step<K>(acc, acc + NA, a.lambda, step)) {
                 stop = true;  // zero pivot: the reference breaks before updating
             } else {
                 float last[K];
 #pragma unroll
                 for (int k = 0; k < K; k++) {
                     last[k] = th[k];
-                    th[k] += (float)step[k];
+                    th[k] += step[k];
                 }
                 model.check_limits(th, R);
                 bool small = true;

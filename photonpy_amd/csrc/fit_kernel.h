@@ -402,16 +402,16 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
         }
         group_sum_n<G>(acc);
 
-        double step[K];
+        float step[K];
         bool stop;
-        if (!lm_step_f64<K>(acc, acc + NA, lam, step)) {
+        if (!lm_step<K>(acc, acc + NA, lam, step)) {
             stop = true;  // zero pivot: the reference breaks before updating
         } else {
             float last[K];
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 last[k] = th[k];
-                th[k] += (float)step[k];
+                th[k] += step[k];
             }
             model.check_limits(th, R);
             bool small = true;

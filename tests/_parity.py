@@ -78,7 +78,7 @@ def check_fit_parity(kind, om, smp, p_gpu, it_gpu, iters, lam, limits, tag="", m
     assert len(bad) <= max(1, 1e-3 * conv.sum()), \
         f"{tag}: {len(bad)} of {conv.sum()} witnesses differ, e.g. gpu={p_gpu[conv][bad[:3]]} ref={p_ref[conv][bad[:3]]}"
     if kind in ("fixed", "sigma"):
-        # the FP64 solve can move a stop by one iteration; z / I stop limits of astig and
+        # the solve's rounding can move a stop by one iteration; z / I stop limits of astig and
         # spline are below one FP32 ulp, so their stop iteration is decided by rounding
         assert np.mean(it_gpu == it_ref) > 0.8, f"{tag}: iteration counts differ"
     return int(conv.sum()), int(len(bad))

@@ -49,9 +49,9 @@ def test_large_roi_estimate_matches_oracle(ctx, kind, R):
 def test_tiny_roi_follows_the_clamp_semantics(ctx):
     """roisize 2: the x/y clamp interval [1.5, R - 2.5] is empty, so every step puts x and
     y at 1.5, as the reference's fmaxf(1.5, fminf(v, -0.5)) does.  With 4 pixels for 4
-    parameters the normal equations are near-singular: where the oracle's FP32 Cholesky
-    meets a zero pivot it stops at the centre-of-mass start, while the FP64 solve takes the
-    step, so only the clamp is compared."""
+    parameters the normal equations are near-singular: where an FP32 Cholesky meets a zero
+    pivot the fit stops at the centre-of-mass start, and rounding decides which spots that
+    is, so only the clamp of the spots that stepped is compared."""
     R, n = 2, 200
     rng = np.random.default_rng(5)
     smp = rng.poisson(50.0, (n, R, R)).astype(np.float32)
@@ -62,8 +62,10 @@ def test_tiny_roi_follows_the_clamp_semantics(ctx):
     stepped = it_ref > 1
     assert stepped.any()
     assert np.all(p_ref[stepped, :2] == 1.5)
-    assert np.all(p_gpu[:, :2] == 1.5)
     assert np.all(est.last_iterations >= 1)
+    stepped_gpu = est.last_iterations > 1
+    assert stepped_gpu.any()
+    assert np.all(p_gpu[stepped_gpu, :2] == 1.5)
 
 
 def test_mixed_batch_with_a_large_bin(ctx):
