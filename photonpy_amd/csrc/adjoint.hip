@@ -2,9 +2,8 @@
 //
 // NLL of a spot: sum over pixels of mu' - x' log mu', with mu' = max(mu, 1e-8)
 // and x' = max(x, 1e-8) (the convention of ComputeLogLikelihood,
-// Estimation.h:138-156, negated).  The backward pass recomputes mu and the
-// Jacobian from the parameters (nothing is stored by the forward pass) and
-// returns TRUE derivatives:
+// Estimation.h:138-156, negated).  The adjoint recomputes mu and the Jacobian from
+// the parameters (nothing of an NLL pass is stored) and returns TRUE derivatives:
 //   d NLL / d theta_k  = sum_px g * (1 - x'/mu') * dmu/dtheta_k
 //   d NLL / d shared   = sum_spots sum_px g * (1 - x'/mu') * dmu/dshared
 // where g is the upstream gradient of the spot's NLL.  Shared parameters:

@@ -6,7 +6,7 @@
 // pixel layout says which: whole rows, or every G-th pixel), so HBM is read once
 // per spot.  Per iteration each lane evaluates its pixels, accumulates the packed
 // normal equations alpha (K(K+1)/2) and beta (K) in FP32, the group all-reduces
-// them with lane shuffles, and every lane solves the FP64 damped Cholesky system
+// them with lane shuffles, and every lane solves the FP32 damped Cholesky system
 // redundantly (identical inputs give identical steps, so the group stays
 // convergent).  Models that stage per-spot tables in LDS get a region of
 // M::lds_floats(R, G) floats per group.
