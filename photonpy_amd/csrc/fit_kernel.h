@@ -175,6 +175,13 @@ PP_DEV void load_samples(const float* src, int g, float (&s)[NS], float& sx, flo
 }
 
 // Fisher matrix (packed) and chi^2 of one spot, written by lane g == 0 of its group.
+// Whether a group of G >= 32 lanes sums N values through its LDS region (group_sum_lds):
+// the model's region, free between its per-iteration tables, holds G N + N floats.
+template <class M, int R, int G, int N>
+constexpr bool group_sum_in_lds() {
+    return G >= 32 && M::lds_floats(R, G) >= G * N + N;
+}
+
 template <class M, int R, int G>
 PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTraits<M, R, G>::NS],
                       const float (&pb)[FitTraits<M, R, G>::NB], const float (&th)[M::KT], int g, float* lds,
@@ -201,6 +208,8 @@ PP_DEV void crlb_spot(const M& model, const FitArgs& a, const float (&s)[FitTrai
         model.template accumulate_rows<R, G, true>(th, g, s, fi, lds);
     else
         model.template for_pixels<R, G>(th, g, lds, pixel_bg<M>(th, pb, body));
+    // (the LDS form measured 5% slower here: one sum per spot and pass, on a kernel that
+    // waits on its loads, pays the LDS round trips in full)
     group_sum_n<G>(fi);
     if (g == 0) {
         if (a.crlb) {
@@ -400,7 +409,10 @@ __device__ __forceinline__ void fit_spots(const M& model, const FitArgs& a, int 
         } else {
             model.template for_pixels<R, G>(th, gi, lds, pixel_bg<M>(th, pb, body));
         }
-        group_sum_n<G>(acc);
+        if constexpr (group_sum_in_lds<M, R, G, NA + K>())
+            group_sum_lds<G>(acc, lds);
+        else
+            group_sum_n<G>(acc);
 
         float step[K];
         bool stop;
