@@ -76,19 +76,36 @@ PP_DEV void group_sum_n(float (&v)[N]) {
     }
 }
 
+// Row stride, in floats, of the partial sums of group_sum_lds: N rounded up to 2 mod 4.
+// A lane's row stays 8-byte aligned (its writes pair into 64-bit stores), and the rows of
+// 16 consecutive lanes start on 16 different even banks, so each 64-bit store of a half
+// wavefront covers every bank twice, the least it can.  With a stride of 20 (20 sums),
+// lanes g and g + 8 started on the same bank: four-way conflicts.
+template <int N>
+constexpr int group_sum_stride() {
+    return N + ((2 - N) % 4 + 4) % 4;
+}
+
+// LDS floats group_sum_lds<G, N> uses: the G rows of partial sums, then the N totals.
+template <int G, int N>
+constexpr int group_sum_lds_floats() {
+    return G * group_sum_stride<N>() + N;
+}
+
 // The same all-reduce for groups of 32 or 64 lanes through the group's LDS region
-// (G N + N floats, free while the group holds no table in it): every lane writes its N
-// partial sums, lane g adds the G partials of values g, g + G, ... (< N) in lane order
-// and writes their totals, and every lane reads the N totals back, so all lanes end
-// with bit-identical totals.  About 2G/64 + N adds per wavefront where the DPP/row-swap butterfly takes
-// (log2 G + 1) N instructions (20 sums at 32 lanes: about 35 VALU instructions instead
-// of 120).
+// (group_sum_lds_floats<G, N>() floats, free while the group holds no table in it): every
+// lane writes its N partial sums, lane g adds the G partials of values g, g + G, ... (< N)
+// in lane order and writes their totals, and every lane reads the N totals back, so all
+// lanes end with bit-identical totals.  About 2G/64 + N adds per wavefront where the
+// DPP/row-swap butterfly takes (log2 G + 1) N instructions (20 sums at 32 lanes: about 35
+// VALU instructions instead of 120).
 template <int G, int N>
 PP_DEV void group_sum_lds(float (&v)[N], float* lds) {
     static_assert(G >= 32, "the butterfly is cheaper for small groups");
+    constexpr int S = group_sum_stride<N>();
     const int g = threadIdx.x & (G - 1);
 #pragma unroll
-    for (int i = 0; i < N; i++) lds[g * N + i] = v[i];
+    for (int i = 0; i < N; i++) lds[g * S + i] = v[i];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -98,15 +115,15 @@ PP_DEV void group_sum_lds(float (&v)[N], float* lds) {
         if (N % G == 0 || i < N) {
             float t = lds[i];
 #pragma unroll
-            for (int k = 1; k < G; k++) t += lds[k * N + i];
-            lds[G * N + i] = t;
+            for (int k = 1; k < G; k++) t += lds[k * S + i];
+            lds[G * S + i] = t;
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 #pragma unroll
-    for (int i = 0; i < N; i++) v[i] = lds[G * N + i];
+    for (int i = 0; i < N; i++) v[i] = lds[G * S + i];
     // every lane's reads complete before the region is written again
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();

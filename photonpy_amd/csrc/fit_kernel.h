@@ -176,10 +176,13 @@ PP_DEV void load_samples(const float* src, int g, float (&s)[NS], float& sx, flo
 
 // Fisher matrix (packed) and chi^2 of one spot, written by lane g == 0 of its group.
 // Whether a group of G >= 32 lanes sums N values through its LDS region (group_sum_lds):
-// the model's region, free between its per-iteration tables, holds G N + N floats.
+// the model's region, free between its per-iteration tables, holds what the sum uses.
 template <class M, int R, int G, int N>
 constexpr bool group_sum_in_lds() {
-    return G >= 32 && M::lds_floats(R, G) >= G * N + N;
+    if constexpr (G >= 32)
+        return M::lds_floats(R, G) >= group_sum_lds_floats<G, N>();
+    else
+        return false;
 }
 
 template <class M, int R, int G>
