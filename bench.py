@@ -141,7 +141,7 @@ def host_info():
     return {"nproc": os.cpu_count(), "usable_cpus": usable, "model": model}
 
 
-# The GPU box grants one GPU's share of its host: worker pools stay within 16 threads.
+# A one-GPU job gets one GPU's share of the host's CPUs: worker pools stay within 16 threads.
 CPU_SHARE = 16
 
 

@@ -14,7 +14,7 @@ import numpy as np
 import numpy.ctypeslib as ctl
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-# PHOTONPY_AMD_LIB selects another build of the library (e.g. for same-box A/B timing)
+# PHOTONPY_AMD_LIB selects another build of the library (e.g. to time two builds on one machine)
 LIB_PATH = os.environ.get("PHOTONPY_AMD_LIB") or os.path.join(_HERE, "_lib", "libphotonpy_amd.so")
 
 
